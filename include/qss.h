@@ -1,0 +1,219 @@
+/*
+ * qss.h — C ABI of the MI355X-native quasi-static xi-slice wake sweep.
+ *
+ * The reference (snytav/tertia) drives one layer step through C++ member
+ * functions of `Mesh` that in turn call a C-style `CUDA_WRAP_*` API with
+ * file-static device buffers.  This header replaces that surface with an
+ * explicit context: every entry point takes a `qss_ctx*`, returns an int
+ * status (QSS_OK or a negative QSS_E_* code, never exit()), and launches all
+ * device work on the context's HIP stream.  Plain C types only.
+ *
+ * Coordinates follow the reference: x = xi (propagation, layers processed
+ * from the front layer l_Mx-1 down to 0), y (index j, fastest) and z
+ * (index k).  Every real-space and spectral plane is Nz x Ny doubles,
+ * element (j,k) at offset j + Ny*k (reference `n = j + ny*k`,
+ * movefieldshydrolinlayersplit.cpp:597).
+ *
+ * Thread-compatibility: one context per host thread; contexts are
+ * independent.  Device buffers are owned by the context.
+ */
+#ifndef QSS_H
+#define QSS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ----------------------------------------------------- */
+#define QSS_OK              0
+#define QSS_E_ARG          -1   /* bad argument / unsupported size          */
+#define QSS_E_HIP          -2   /* HIP runtime error (message via last_error)*/
+#define QSS_E_NOMEM        -3   /* device allocation failed                 */
+#define QSS_E_STATE        -4   /* call out of order (e.g. no particles)    */
+#define QSS_E_OVERFLOW     -5   /* sub-cycle event buffer too small         */
+#define QSS_E_NONFINITE    -6   /* non-finite residual / field detected     */
+#define QSS_E_PARTICLE     -7   /* particle outside its cell (ref exit(-212))*/
+
+/* ---- enums ------------------------------------------------------------ */
+/* transform kinds along one axis: FFTW REDFT11 (cos) / RODFT11 (sin),
+ * reference movefieldshydrolinlayersplit.cpp:389-427                      */
+#define QSS_KIND_C 0
+#define QSS_KIND_S 1
+
+/* particle shape: NGP_REF reproduces the linked split path
+ * (movepart_splitlayer.cpp:1101-1217 gather at the start cell,
+ * :1547-1655 single-cell deposit); CIC uses the bilinear gather/deposit of
+ * movepart_layer_CIC.cpp:166-203,449-496 inside the same predictor-corrector */
+#define QSS_SHAPE_NGP_REF 0
+#define QSS_SHAPE_CIC     1
+
+/* precision of the GPU path (the oracle is always FP64) */
+#define QSS_PREC_FP64     0   /* FP64 push, FP64 MFMA transforms             */
+#define QSS_PREC_MIXED32  1   /* FP32 push arithmetic, FP32 MFMA transforms;
+                                 FP64 particle state + FP64 residual          */
+
+/* field ids for history / layer access */
+enum qss_field {
+  QSS_EX = 0, QSS_EY, QSS_EZ, QSS_BX, QSS_BY, QSS_BZ,   /* fields  (f_Fields)  */
+  QSS_JX, QSS_JY, QSS_JZ, QSS_RHO,                      /* currents, f_Dens    */
+  QSS_RHOBEAM, QSS_JXBEAM,                              /* beam sources        */
+  QSS_PSI,                                              /* wake potential      */
+  QSS_NFIELD
+};
+
+/* ---- parameters ------------------------------------------------------- */
+#define QSS_MAX_BUNCH 4
+
+/* One frozen bi-Gaussian driver/witness bunch.  Profile follows
+ * Specie::Density case 1 (plasma.cpp:350-377):
+ *   n = nb * exp(-dx^2/Rx^2 - dy^2/R^2 - dz^2/R^2), zero where arg > 16,
+ * with Rx = sqrt(2)*sigma_xi and R = sqrt(2)*sigma_r.  Macro-particles are
+ * seeded 4 per 3-D cell at (x=1/2, {1/3,2/3}^2) (mesh.cpp:140-163), weight =
+ * profile(cell corner)/4 * (1 + noise*u), u ~ U(-1,1) from the seed, and are
+ * deposited trilinearly (movebeam.cpp:662-770) with dRho = w*polarity,
+ * dJx = w*polarity*vx (movebeam.cpp:509-512).                               */
+typedef struct qss_bunch {
+  double nb;        /* peak density / n0                                    */
+  double sigma_r;   /* transverse rms size (k_p^-1)                         */
+  double sigma_xi;  /* longitudinal rms size (k_p^-1)                       */
+  double xi0;       /* centre, distance behind the front layer (k_p^-1)     */
+  double y0, z0;    /* transverse centre relative to the box centre         */
+  double polarity;  /* charge in electron units (-1 for electrons)          */
+  double vx;        /* frozen longitudinal velocity (1)                     */
+  double noise;     /* relative macro-particle weight noise amplitude       */
+} qss_bunch;
+
+typedef struct qss_params {
+  int32_t ny, nz;        /* transverse cells (l_My, l_Mz)                    */
+  double  ly, lz;        /* transverse box (Ylength, Zlength)                */
+  double  hx;            /* layer thickness Hx                               */
+  int32_t nx_split;      /* NxSplit (HxSplit = hx / nx_split)                */
+  int32_t nslices;       /* number of layers swept (l_Mx)                    */
+  int32_t n_iter;        /* predictor-corrector iterations nIter             */
+  double  dens;          /* plasma density n0 (units k_p^2)                  */
+  int32_t ppc_y, ppc_z;  /* plasma particles per cell along y, z (2,2)       */
+  double  jitter;        /* +- uniform position jitter, cell units           */
+  uint64_t seed;         /* seeds jitter and beam noise                      */
+  int32_t shape;         /* QSS_SHAPE_*                                      */
+  int32_t fix_subcycle;  /* 0: reproduce the sub-cycle restart quirk         */
+  int32_t precision;     /* QSS_PREC_*                                       */
+  int32_t nbunch;        /* bunches used                                     */
+  qss_bunch bunch[QSS_MAX_BUNCH];
+  /* optional piecewise-constant longitudinal profile replacing the
+   * Gaussian xi factor of bunch[0]: nbins values over [bin_xi0, bin_xi1) */
+  int32_t nbins;
+  double  bin_xi0, bin_xi1;
+  const double *bins;    /* host pointer, nbins values (copied at create)   */
+} qss_params;
+
+/* ---- context ---------------------------------------------------------- */
+typedef struct qss_ctx qss_ctx;
+
+/* Creates a context on HIP device `device`, allocates the layer buffers
+ * (SoA plasma particles, C/P field planes, spectral state, transform
+ * matrices) and seeds the front layer (Mesh::SeedFrontParticles,
+ * mesh.cpp:98-105).  history_mask: bit f set => store field f of every swept
+ * layer in device memory (nslices planes each). */
+int  qss_create(const qss_params *p, int device, uint32_t history_mask,
+                qss_ctx **out);
+int  qss_destroy(qss_ctx *c);
+const char *qss_last_error(void);
+/* Returns the compiled-in sizes: particles per layer etc. */
+int  qss_info(const qss_ctx *c, int64_t *n_particles, int64_t *n_cells);
+/* The HIP stream all work of this context runs on (hipStream_t). */
+void *qss_stream(qss_ctx *c);
+
+/* ---- whole sweep: Mesh::MoveAllSplitLayers (movepart_splitlayer.cpp:25) */
+/* Re-seeds the front layer, zeroes the spectral state and sweeps
+ * `nslices` layers front to back.  residuals (host, may be NULL) receives
+ * the final-pass residual of every layer step (nslices*nx_split). */
+int  qss_sweep(qss_ctx *c, double *residuals);
+/* Sweeps only layers [first, first+count) of the slice order (0 = first
+ * layer behind the front); used for timing windows and checkpoints.       */
+int  qss_sweep_range(qss_ctx *c, int first, int count, double *residuals);
+int  qss_reset(qss_ctx *c);   /* re-seed particles, zero fields/spectra    */
+
+/* ---- one layer step: Mesh::MoveSplitLayer (movepart_splitlayer.cpp:561) */
+/* slice: 0-based position behind the front; isplit < nx_split.
+ * Sets the C/P beam planes (interpolation, movepart_splitlayer.cpp:330-348),
+ * runs Guess, nIter x (deposit, Iterate), the full-step push and the final
+ * Iterate, then copies C -> P.  residual (host, may be NULL).              */
+int  qss_layer_step(qss_ctx *c, int slice, int isplit, double *residual);
+
+/* ---- the pieces of a layer step ---------------------------------------- */
+/* CUDA_WRAP_interpolateLayers (cuLayers.cu:437) + beam deposit of layers
+ * i and i+1 (CUDA_WRAP_beam_move deposit part, cuBeam.cu:739): fills the
+ * C and P beam planes for (slice, isplit).                                  */
+int  qss_beam_layer(qss_ctx *c, int slice, int isplit);
+/* Mesh::GuessFieldsHydroLinLayerSplit (movefieldshydrolinlayersplit.cpp:222):
+ * beam forward transforms, spectral predictor, inverse -> C-layer E, B.    */
+int  qss_guess(qss_ctx *c, int first_layer);
+/* Mesh::ClearCurrentsSplit + MoveParticlesLayerSplit + ExchangeCurrentsSplit
+ * (mesh.cpp:692, movepart_splitlayer.cpp:781,1685): push every plasma
+ * particle of P through E,B = (P+C)/2 and deposit Jx,Jy,Jz,Dens into C.
+ * full_step != 0 also stores the pushed particles as the C layer.          */
+int  qss_push_deposit(qss_ctx *c, int full_step);
+/* Mesh::IterateFieldsHydroLinLayerSplit (movefieldshydrolinlayersplit.cpp:
+ * 1173) + ExchangeFieldsSplit: corrector; residual = Eerr.  final != 0 also
+ * produces the real-space rho and psi planes.                               */
+int  qss_iterate(qss_ctx *c, int final_pass, double *residual);
+/* C -> P at the end of a split step (movepart_splitlayer.cpp:423-428).     */
+int  qss_copy_c_to_p(qss_ctx *c);
+
+/* ---- transforms: CUDA_WRAP_fourierHalfInteger2D (half_integer2D.cu:583) */
+/* out = scale * T_kz . in . T_ky on device pointers (Nz x Ny planes),
+ * T^c_{jk} = 2cos(pi(j+1/2)(k+1/2)/N), T^s = 2sin(...).  scale = 1 for the
+ * forward transform, 1/(4 Ny Nz) for the inverse (normalization.cu:241).
+ * `in` and `out` may not alias.                                             */
+int  qss_transform2d(qss_ctx *c, int kind_z, int kind_y, const double *in_dev,
+                     double *out_dev, double scale);
+/* Batched form: n transforms in one launch pair. */
+int  qss_transform2d_batched(qss_ctx *c, int n, const int *kind_z,
+                             const int *kind_y, const double *const *in_dev,
+                             double *const *out_dev, double scale);
+
+/* ---- data access (host buffers, Ny*Nz doubles, synchronous) ------------ */
+/* which: 0 = C layer, 1 = P layer.  field: QSS_EX..QSS_PSI. */
+int  qss_get_layer_field(qss_ctx *c, int which, int field, double *host);
+int  qss_set_layer_field(qss_ctx *c, int which, int field, const double *host);
+/* history of field f for slice s (requires history_mask bit). */
+int  qss_get_history(qss_ctx *c, int field, int slice, double *host);
+/* Device pointer of the history block of field f (nslices*Ny*Nz doubles). */
+int  qss_history_ptr(qss_ctx *c, int field, double **dev);
+/* Plasma particles in slot order (sorted by cell after every full step).
+ * State is exact: cell indices (j,k) plus in-cell fractions y,z in [0,1]
+ * (the reference's Particle f_Y/f_Z inside its Cell); id = seeding index.
+ * Any pointer may be NULL.  set: n <= the capacity fixed at create.        */
+int  qss_get_particles(qss_ctx *c, int32_t *cell_j, int32_t *cell_k, double *fy,
+                       double *fz, double *px, double *py, double *pz,
+                       int64_t *id);
+int  qss_set_particles(qss_ctx *c, int64_t n, const int32_t *cell_j,
+                       const int32_t *cell_k, const double *fy, const double *fz,
+                       const double *px, const double *py, const double *pz,
+                       const int64_t *id);
+/* Spectral state (fft_of_* arrays, movefieldshydrolinlayersplit.cpp:267-313):
+ * id 0..20 = Rho,Ex,Ey,Ez,Bx,By,Bz,Jx,Jy,Jz,RhoP,JxP,JyP,JzP,ExP,EyP,EzP,
+ * RhoBeam,JxBeam,RhoBeamP,JxBeamP.                                          */
+#define QSS_NSPECTRAL 21
+int  qss_get_spectral(qss_ctx *c, int id, double *host);
+int  qss_set_spectral(qss_ctx *c, int id, const double *host);
+
+/* ---- timing of the last qss_sweep/qss_sweep_range (HIP events) --------- */
+/* Kernel classes: 0 push, 1 transform GEMM, 2 spectral update, 3 beam,
+ * 4 counting sort (fill+scan+scatter), 5 deposit tile reduce.              */
+#define QSS_NKCLASS 6
+/* on != 0: record a HIP event pair around every launch of the next sweeps.
+ * ms_total: wall time of the last sweep window on the stream; ms_kernel[i],
+ * launches[i]: summed event time / launch count per class; gemm_flops:
+ * algorithmic flops (2*M*N*K per transform pass and field) of the GEMMs.   */
+int  qss_enable_timing(qss_ctx *c, int on);
+int  qss_timing(qss_ctx *c, double *ms_total, double *ms_kernel,
+                int64_t *launches, double *gemm_flops);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QSS_H */

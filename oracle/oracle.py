@@ -1,0 +1,226 @@
+"""ctypes wrapper of the FP64 CPU oracle (``oracle/liboracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py``.  It is the checker, never the
+thing measured or shipped; the product path (``tertia_amd``) never imports it.
+
+Every function of ``qss_oracle.cpp`` restates a reference routine; the
+file:line it follows is given at each function there.  Pinning: the
+transforms are pinned by scipy type-4 fixtures, brute force and the
+involution T.T = 2N.I; the spectral updates by single-mode hand-computed
+answers and linearity; the push by Boris-rotation invariants and closed
+forms; the sweep by the psi/Ex consistency and linear-wake convergence
+(see tests/test_oracle_*.py).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from tertia_amd.params import FIELD_ID, SPECTRAL_ID, Params, QssParams
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []),
+                       check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.oracle_create.restype = ctypes.c_void_p
+        L.oracle_create.argtypes = [ctypes.POINTER(QssParams), ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_uint32]
+        L.oracle_destroy.argtypes = [ctypes.c_void_p]
+        for name in ("oracle_reset", "oracle_n_particles"):
+            getattr(L, name).argtypes = [ctypes.c_void_p]
+        L.oracle_layer_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.oracle_sweep_range.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.oracle_sweep.argtypes = [ctypes.c_void_p, _dp]
+        L.oracle_beam_layer.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.oracle_beam_plane.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        L.oracle_guess.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.oracle_push_deposit.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.oracle_iterate.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.oracle_copy_c_to_p.argtypes = [ctypes.c_void_p]
+        L.oracle_get_layer_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.oracle_set_layer_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.oracle_get_spectral.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.oracle_set_spectral.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.oracle_get_history.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.oracle_get_particles.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp,
+                                           _i64p, _i32p, _i32p, _dp, _dp]
+        L.oracle_set_particles.argtypes = [ctypes.c_void_p, ctypes.c_long, _i32p, _i32p,
+                                           _dp, _dp, _dp, _dp, _dp, _i64p]
+        L.oracle_transform1d.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int]
+        L.oracle_transform2d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, _dp, _dp, ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def transform1d(x, kind, use_fft=False):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    lib().oracle_transform1d(x.size, int(kind), _p(x), _p(out), int(use_fft))
+    return out
+
+
+def transform2d(x, kind_z, kind_y, scale=1.0, use_fft=False, nthreads=1):
+    """x: (nz, ny) array; returns scale * T_kz . x . T_ky (FFTW r2r 2-D)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nz, ny = x.shape
+    out = np.empty_like(x)
+    lib().oracle_transform2d(ny, nz, int(kind_z), int(kind_y), _p(x), _p(out),
+                             float(scale), int(use_fft), int(nthreads))
+    return out
+
+
+class Oracle:
+    """FP64 CPU restatement of Mesh::MoveAllSplitLayers and its pieces."""
+
+    def __init__(self, params: Params, use_fft: bool = False, nthreads: int = 1,
+                 history=("Ex", "Ey", "Ez", "Bx", "By", "Bz", "Jx", "Jy", "Jz",
+                          "Rho", "RhoBeam", "JxBeam", "Psi")):
+        self.params = params
+        self._c = params.to_c()
+        mask = 0
+        for f in history or ():
+            mask |= 1 << FIELD_ID[f]
+        self.h = lib().oracle_create(ctypes.byref(self._c), int(use_fft), int(nthreads),
+                                     mask)
+        if not self.h:
+            raise ValueError("oracle_create rejected the parameters")
+        self.ny, self.nz = params.ny, params.nz
+
+    def close(self):
+        if self.h:
+            lib().oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n_particles(self):
+        return lib().oracle_n_particles(self.h)
+
+    def _plane(self):
+        return np.zeros((self.nz, self.ny), dtype=np.float64)
+
+    # -- sweep ---------------------------------------------------------
+    def reset(self):
+        lib().oracle_reset(self.h)
+
+    def sweep(self):
+        n = self.params.nslices * self.params.nx_split
+        res = np.zeros(n)
+        st = lib().oracle_sweep(self.h, _p(res))
+        return st, res
+
+    def sweep_range(self, first, count):
+        res = np.zeros(count * self.params.nx_split)
+        st = lib().oracle_sweep_range(self.h, first, count, _p(res))
+        return st, res
+
+    def layer_step(self, slice_, isplit=0):
+        r = np.zeros(1)
+        st = lib().oracle_layer_step(self.h, slice_, isplit, _p(r))
+        return st, float(r[0])
+
+    # -- pieces ----------------------------------------------------------
+    def beam_layer(self, slice_, isplit=0):
+        return lib().oracle_beam_layer(self.h, slice_, isplit)
+
+    def beam_plane(self, layer):
+        r, j = self._plane(), self._plane()
+        lib().oracle_beam_plane(self.h, layer, _p(r), _p(j))
+        return r, j
+
+    def guess(self, first_layer):
+        return lib().oracle_guess(self.h, int(first_layer))
+
+    def push_deposit(self, full_step):
+        return lib().oracle_push_deposit(self.h, int(full_step))
+
+    def iterate(self, final_pass):
+        r = np.zeros(1)
+        lib().oracle_iterate(self.h, int(final_pass), _p(r))
+        return float(r[0])
+
+    def copy_c_to_p(self):
+        return lib().oracle_copy_c_to_p(self.h)
+
+    # -- data ------------------------------------------------------------
+    def get_field(self, name, which=0):
+        out = self._plane()
+        lib().oracle_get_layer_field(self.h, which, FIELD_ID[name], _p(out))
+        return out
+
+    def set_field(self, name, arr, which=0):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.nz, self.ny)
+        lib().oracle_set_layer_field(self.h, which, FIELD_ID[name], _p(a))
+
+    def get_spectral(self, name):
+        out = self._plane()
+        lib().oracle_get_spectral(self.h, SPECTRAL_ID[name], _p(out))
+        return out
+
+    def set_spectral(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.nz, self.ny)
+        lib().oracle_set_spectral(self.h, SPECTRAL_ID[name], _p(a))
+
+    def history(self, name, slice_):
+        out = self._plane()
+        st = lib().oracle_get_history(self.h, FIELD_ID[name], slice_, _p(out))
+        if st != 0:
+            raise ValueError("no history for %s" % name)
+        return out
+
+    def particles(self):
+        n = self.n_particles
+        d = {k: np.zeros(n) for k in ("y", "z", "px", "py", "pz", "fy", "fz")}
+        d["id"] = np.zeros(n, dtype=np.int64)
+        d["cj"] = np.zeros(n, dtype=np.int32)
+        d["ck"] = np.zeros(n, dtype=np.int32)
+        lib().oracle_get_particles(self.h, _p(d["y"]), _p(d["z"]), _p(d["px"]),
+                                   _p(d["py"]), _p(d["pz"]),
+                                   d["id"].ctypes.data_as(_i64p),
+                                   d["cj"].ctypes.data_as(_i32p),
+                                   d["ck"].ctypes.data_as(_i32p),
+                                   _p(d["fy"]), _p(d["fz"]))
+        return d
+
+    def set_particles(self, cj, ck, fy, fz, px, py, pz, pid):
+        arrs = [np.ascontiguousarray(a, dtype=np.int32) for a in (cj, ck)]
+        f = [np.ascontiguousarray(a, dtype=np.float64) for a in (fy, fz, px, py, pz)]
+        pid = np.ascontiguousarray(pid, dtype=np.int64)
+        lib().oracle_set_particles(self.h, len(pid), arrs[0].ctypes.data_as(_i32p),
+                                   arrs[1].ctypes.data_as(_i32p), _p(f[0]), _p(f[1]),
+                                   _p(f[2]), _p(f[3]), _p(f[4]),
+                                   pid.ctypes.data_as(_i64p))

@@ -1,0 +1,934 @@
+// qss_oracle.cpp — FP64 CPU restatement of the reference quasi-static
+// xi-slice sweep (snytav/tertia CPU/FFTW path).
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker: it is loaded by the
+// tests, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg.  The
+// product (tertia_amd/libqss.so) never links or calls it.
+//
+// What it restates (reference file:line, paths relative to the reference):
+//   sweep driver          movepart_splitlayer.cpp:25-558   (oracle_sweep_range)
+//   one layer step        movepart_splitlayer.cpp:561-778  (oracle_layer_step)
+//   Guess (predictor)     movefieldshydrolinlayersplit.cpp:222-1169
+//   Iterate (corrector)   movefieldshydrolinlayersplit.cpp:1173-1671
+//   transforms            FFTW REDFT11/RODFT11 r2r 2-D, plans :389-427
+//   push + deposit        movepart_splitlayer.cpp:781-1544, 1547-1655
+//   CIC gather / deposit  movepart_layer_CIC.cpp:166-203, 449-496
+//   ghost exchange        movepart_splitlayer.cpp:1659-1711 (periodic)
+//   plasma seeding        mesh.cpp:98-105, 413-548; elements.cpp:97-107
+//   beam seeding/deposit  mesh.cpp:108-276; plasma.cpp:350-377;
+//                         movebeam.cpp:509-512, 662-770
+//   beam interpolation    movepart_splitlayer.cpp:330-348
+//
+// Build conventions that differ from the reference are listed at the
+// function that makes them.  Compile with -ffp-contract=off: the oracle's
+// push/deposit arithmetic is evaluated in the reference's operation order.
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <numeric>
+#include <string>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/qss.h"
+
+namespace {
+
+typedef std::complex<double> cplx;
+
+// ------------------------------------------------------------------------
+// Counter-based RNG (splitmix64) for jitter and beam noise.  Identical
+// integer arithmetic is used on the GPU, so both draw the same numbers.
+// ------------------------------------------------------------------------
+inline uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+// uniform in [-1, 1): 53 random bits
+inline double unif_pm1(uint64_t seed, uint64_t a, uint64_t b) {
+  uint64_t h = mix64(seed ^ mix64(a ^ mix64(b)));
+  return (double)(h >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+}
+
+// ------------------------------------------------------------------------
+// 1-D half-integer transforms (FFTW REDFT11 / RODFT11 definitions):
+//   REDFT11: Y_k = 2 sum_j X_j cos(pi (j+1/2)(k+1/2) / N)
+//   RODFT11: Y_k = 2 sum_j X_j sin(pi (j+1/2)(k+1/2) / N)
+// Two implementations: the dense matrix (the reference's `HalfInteger`
+// table form, CUDA_WRAP/fft_matrix.cu:10-24) and an N/2-point complex FFT
+// form used for the CPU speed baseline.
+// ------------------------------------------------------------------------
+struct Plan1D {
+  int n = 0;
+  std::vector<double> tc, ts;         // n x n tables 2cos / 2sin
+  // FFT form (n even, n/2 power of two)
+  bool fft_ok = false;
+  int m = 0;
+  std::vector<cplx> pre, post, tw;    // twiddles
+  std::vector<int> rev;
+
+  void init(int n_) {
+    n = n_;
+    tc.assign((size_t)n * n, 0.0);
+    ts.assign((size_t)n * n, 0.0);
+    // angle pi(j+1/2)(k+1/2)/n = 2*pi*m/(8n) with m = (2j+1)(2k+1) mod 8n,
+    // reduced exactly in integers so every entry is accurate to ~1 ulp
+    for (int j = 0; j < n; ++j)
+      for (int k = 0; k < n; ++k) {
+        long m = ((long)(2 * j + 1) * (2 * k + 1)) % (8L * n);
+        double a = M_PI * (double)m / (4.0 * n);
+        tc[(size_t)j * n + k] = 2.0 * std::cos(a);
+        ts[(size_t)j * n + k] = 2.0 * std::sin(a);
+      }
+    fft_ok = false;
+    if (n >= 4 && n % 2 == 0) {
+      m = n / 2;
+      if ((m & (m - 1)) == 0) {
+        fft_ok = true;
+        pre.resize(m); post.resize(m); tw.resize(m / 2 > 0 ? m / 2 : 1);
+        for (int q = 0; q < m; ++q) {
+          pre[q] = std::polar(1.0, -M_PI * q / n);
+          post[q] = std::polar(1.0, -M_PI * (4.0 * q + 1.0) / (4.0 * n));
+        }
+        for (int q = 0; q < m / 2; ++q) tw[q] = std::polar(1.0, -2.0 * M_PI * q / m);
+        rev.resize(m);
+        int lg = 0; while ((1 << lg) < m) ++lg;
+        for (int q = 0; q < m; ++q) {
+          int r = 0;
+          for (int b = 0; b < lg; ++b) if (q & (1 << b)) r |= 1 << (lg - 1 - b);
+          rev[q] = r;
+        }
+      }
+    }
+  }
+
+  // in-place radix-2 DIT forward FFT of length m
+  void fft(cplx *a) const {
+    for (int q = 0; q < m; ++q) if (q < rev[q]) std::swap(a[q], a[rev[q]]);
+    for (int len = 2; len <= m; len <<= 1) {
+      int half = len >> 1, step = m / len;
+      for (int s = 0; s < m; s += len)
+        for (int q = 0; q < half; ++q) {
+          cplx w = tw[q * step];
+          cplx u = a[s + q], v = a[s + q + half] * w;
+          a[s + q] = u + v;
+          a[s + q + half] = u - v;
+        }
+    }
+  }
+
+  // strided 1-D transform: in[i*stride], out[i*stride]
+  void apply(int kind, const double *in, long istride, double *out,
+             long ostride, bool use_fft, cplx *work, double *xbuf) const {
+    if (use_fft && fft_ok) {
+      // DST-IV(x)_k = (-1)^k DCT-IV(reverse x)_k
+      for (int j = 0; j < n; ++j)
+        xbuf[j] = (kind == QSS_KIND_C) ? in[j * istride] : in[(n - 1 - j) * istride];
+      // DCT-IV via N/2 complex FFT:
+      //   c_q = (x_{2q} + i x_{N-1-2q}) e^{-i pi q/N};  Z = FFT_M(c);
+      //   Z_m *= e^{-i pi (4m+1)/(4N)};  Y_{2m} = Re Z_m, Y_{N-1-2m} = -Im Z_m
+      for (int q = 0; q < m; ++q)
+        work[q] = cplx(xbuf[2 * q], xbuf[n - 1 - 2 * q]) * pre[q];
+      fft(work);
+      for (int q = 0; q < m; ++q) {
+        cplx z = work[q] * post[q];
+        xbuf[2 * q] = 2.0 * z.real();
+        xbuf[n - 1 - 2 * q] = -2.0 * z.imag();
+      }
+      for (int k = 0; k < n; ++k) {
+        double v = xbuf[k];
+        if (kind == QSS_KIND_S && (k & 1)) v = -v;
+        out[k * ostride] = v;
+      }
+      return;
+    }
+    const double *t = (kind == QSS_KIND_C) ? tc.data() : ts.data();
+    for (int k = 0; k < n; ++k) {
+      double s = 0.0;
+      for (int j = 0; j < n; ++j) s += in[j * istride] * t[(size_t)j * n + k];
+      xbuf[k] = s;
+    }
+    for (int k = 0; k < n; ++k) out[k * ostride] = xbuf[k];
+  }
+};
+
+// 2-D: out[k][j] = scale * sum T_kz(k,k') T_ky(j,j') in[k'][j'], Nz x Ny,
+// y fastest (fftw_plan_r2r_2d(nz, ny, ..., kind_z, kind_y)).
+void transform2d(const Plan1D &py, const Plan1D &pz, int kz, int ky,
+                 const double *in, double *out, double scale, bool use_fft,
+                 int nthreads) {
+  const int ny = py.n, nz = pz.n;
+  std::vector<double> tmp((size_t)ny * nz);
+  int nmax = std::max(ny, nz);
+#pragma omp parallel num_threads(nthreads)
+  {
+    std::vector<cplx> work(nmax);
+    std::vector<double> xbuf(nmax);
+#pragma omp for schedule(static)
+    for (int k = 0; k < nz; ++k)
+      py.apply(ky, in + (size_t)k * ny, 1, tmp.data() + (size_t)k * ny, 1,
+               use_fft, work.data(), xbuf.data());
+#pragma omp for schedule(static)
+    for (int j = 0; j < ny; ++j)
+      pz.apply(kz, tmp.data() + j, ny, out + j, ny, use_fft, work.data(),
+               xbuf.data());
+  }
+  if (scale != 1.0)
+    for (size_t i = 0; i < (size_t)ny * nz; ++i) out[i] *= scale;
+}
+
+enum { SP_RHO = 0, SP_EX, SP_EY, SP_EZ, SP_BX, SP_BY, SP_BZ, SP_JX, SP_JY,
+       SP_JZ, SP_RHOP, SP_JXP, SP_JYP, SP_JZP, SP_EXP, SP_EYP, SP_EZP,
+       SP_RHOB, SP_JXB, SP_RHOBP, SP_JXBP, SP_N };
+
+}  // namespace
+
+struct oracle {
+  qss_params p;
+  std::vector<double> bins;
+  int ny = 0, nz = 0;
+  long ncell = 0;
+  int nthreads = 1;
+  bool use_fft = false;
+  Plan1D ply, plz;
+  double hy = 0, hz = 0;
+  double pweight = 0;                  // plasma macro-particle weight
+  // layers: C (current) and P (previous), fields QSS_EX..QSS_JXBEAM
+  std::vector<double> C[QSS_NFIELD], P[QSS_NFIELD];
+  std::vector<double> S[SP_N];         // spectral state (fft_of_*)
+  // plasma particles (SoA), sorted by (cell, previous slot)
+  std::vector<double> y, z, px, py, pz;
+  std::vector<int> j, k;
+  std::vector<int64_t> id;
+  // history: field f of slice s
+  uint32_t hmask = 0;
+  std::vector<double> hist[QSS_NFIELD];
+  double last_residual = 0;
+};
+
+namespace {
+
+inline double &at(std::vector<double> &v, long n) { return v[n]; }
+
+// --------------------------------------------------------------------------
+// Plasma seeding: Mesh::SeedParticles for the front layer (mesh.cpp:413-548)
+// with ppc = ppc_y*ppc_z.  ppc 2x2 gives the reference's nseed==4 lattice
+// y,z in {1/3,2/3} (mesh.cpp:455-459); weight = (1/ppc)*n0 (elements.cpp:102
+// and mesh.cpp:468); id = cell*ppc + iseed with iy = iseed%ppc_y.  Build
+// addition: +-jitter uniform position noise (cell units) from the seed.
+// --------------------------------------------------------------------------
+void seed_plasma(oracle *o) {
+  const qss_params &p = o->p;
+  int ppc = p.ppc_y * p.ppc_z;
+  long np = o->ncell * ppc;
+  o->y.resize(np); o->z.resize(np); o->px.assign(np, 0.0); o->py.assign(np, 0.0);
+  o->pz.assign(np, 0.0); o->j.resize(np); o->k.resize(np); o->id.resize(np);
+  o->pweight = (1.0 / ppc) * p.dens;
+  long n = 0;
+  for (int kk = 0; kk < o->nz; ++kk)
+    for (int jj = 0; jj < o->ny; ++jj)
+      for (int is = 0; is < ppc; ++is, ++n) {
+        int iy = is % p.ppc_y, iz = is / p.ppc_y;
+        int64_t pid = ((int64_t)kk * o->ny + jj) * ppc + is;
+        double yy = (iy + 1.0) / (p.ppc_y + 1.0);
+        double zz = (iz + 1.0) / (p.ppc_z + 1.0);
+        if (p.jitter != 0.0) {
+          yy += p.jitter * unif_pm1(p.seed, (uint64_t)pid, 1);
+          zz += p.jitter * unif_pm1(p.seed, (uint64_t)pid, 2);
+        }
+        o->y[n] = yy; o->z[n] = zz; o->j[n] = jj; o->k[n] = kk; o->id[n] = pid;
+      }
+}
+
+// --------------------------------------------------------------------------
+// Beam density of one bunch at a seeding point, Specie::Density case 1
+// (plasma.cpp:350-377): nb*exp(-arg), arg = dx^2/Rx^2 + dy^2/R^2 + dz^2/R^2,
+// zero beyond arg > 16, R = sqrt(2)*sigma.  dx is measured in xi.  With the
+// piecewise profile (bins) the xi factor of bunch 0 is the bin value and the
+// cut applies to the transverse part.
+// --------------------------------------------------------------------------
+double bunch_density(const oracle *o, int b, double xi, double yc, double zc) {
+  const qss_bunch &B = o->p.bunch[b];
+  double Ry = std::sqrt(2.0) * B.sigma_r;
+  double dy = yc - B.y0, dz = zc - B.z0;
+  if (b == 0 && o->p.nbins > 0) {
+    double arg = dy * dy / Ry / Ry + dz * dz / Ry / Ry;
+    if (arg > 16) return 0.0;
+    double t = (xi - o->p.bin_xi0) / (o->p.bin_xi1 - o->p.bin_xi0) * o->p.nbins;
+    if (!(t >= 0.0) || t >= o->p.nbins) return 0.0;
+    int ib = (int)t;
+    return o->bins[ib] * std::exp(-arg);
+  }
+  double Rx = std::sqrt(2.0) * B.sigma_xi;
+  double dx = xi - B.xi0;
+  double arg = dx * dx / Rx / Rx + dy * dy / Ry / Ry + dz * dz / Ry / Ry;
+  if (arg > 16) return 0.0;
+  return B.nb * std::exp(-arg);
+}
+
+// --------------------------------------------------------------------------
+// Beam plane of reference layer i (0..L, L = nslices = front layer): frozen
+// beam macro-particles, 4 per 3-D cell at (x=1/2, {1/3,2/3}^2) seeded from
+// the density at the cell corner (mesh.cpp:108-163, xco = X(i), yco =
+// Y(j)-Ylength/2), deposited trilinearly (movebeam.cpp:662-770).  Layer i
+// receives axc=1/2 from cell i and axp=1/2 from cell i-1.  Cell ci lies at
+// xi = (L - ci)*hx behind the front.  Nodes outside [0,N) are dropped (the
+// beam is not ghost-folded in the reference).
+// --------------------------------------------------------------------------
+void beam_plane(oracle *o, int i, double *rho, double *jx) {
+  const qss_params &p = o->p;
+  const int ny = o->ny, nz = o->nz, L = p.nslices;
+  std::fill(rho, rho + o->ncell, 0.0);
+  std::fill(jx, jx + o->ncell, 0.0);
+  for (int side = 0; side < 2; ++side) {
+    int ci = (side == 0) ? i : i - 1;
+    if (ci < 0 || ci > L) continue;
+    double ax = 0.5;  // 1 - 1/2 (axc) or 1/2 (axp)
+    double xi = (double)(L - ci) * p.hx;
+    for (int b = 0; b < p.nbunch; ++b) {
+      const qss_bunch &B = p.bunch[b];
+      for (int ck = 0; ck < nz; ++ck)
+        for (int cj = 0; cj < ny; ++cj) {
+          double yc = o->hy * cj - p.ly / 2.0;
+          double zc = o->hz * ck - p.lz / 2.0;
+          double dn = bunch_density(o, b, xi, yc, zc);
+          if (!(dn > 0.0)) continue;
+          for (int is = 0; is < 4; ++is) {
+            double yq = 1.0 / 3.0 * (1.0 + is % 2);
+            double zq = 1.0 / 3.0 * (1.0 + is / 2);
+            double w = 0.25 * dn;
+            if (B.noise != 0.0) {
+              uint64_t cell = ((uint64_t)(ci + 1) * (uint64_t)nz + ck) * (uint64_t)ny + cj;
+              w *= 1.0 + B.noise * unif_pm1(p.seed ^ 0xB3A1ull, cell * 4 + is, (uint64_t)b + 7);
+            }
+            double drho = w * B.polarity;
+            double djx = w * B.polarity * B.vx;
+            for (int dz_ = 0; dz_ < 2; ++dz_) {
+              int nk = ck + dz_;
+              if (nk >= nz) continue;
+              double az = dz_ ? zq : 1.0 - zq;
+              for (int dy_ = 0; dy_ < 2; ++dy_) {
+                int nj = cj + dy_;
+                if (nj >= ny) continue;
+                double ay = dy_ ? yq : 1.0 - yq;
+                long n = nj + (long)ny * nk;
+                jx[n] += djx * ax * ay * az;
+                rho[n] += drho * ax * ay * az;
+              }
+            }
+          }
+        }
+    }
+  }
+}
+
+inline void wavenumbers(const oracle *o, int jj, int kk, double &aky,
+                        double &akz, double &ak2) {
+  akz = M_PI / o->p.lz * (kk + 0.5);
+  aky = M_PI / o->p.ly * (jj + 0.5);
+  ak2 = aky * aky + akz * akz;
+}
+
+void fwd(oracle *o, int kz, int ky, const std::vector<double> &in,
+         std::vector<double> &out) {
+  transform2d(o->ply, o->plz, kz, ky, in.data(), out.data(), 1.0, o->use_fft,
+              o->nthreads);
+}
+void inv(oracle *o, int kz, int ky, const std::vector<double> &in,
+         std::vector<double> &out) {
+  // normalisation / (4 ny nz): movefieldshydrolinlayersplit.cpp:1008-1023
+  std::vector<double> tmp(o->ncell);
+  transform2d(o->ply, o->plz, kz, ky, in.data(), tmp.data(), 1.0, o->use_fft,
+              o->nthreads);
+  double d = 4.0 * o->ny * o->nz;
+  for (long n = 0; n < o->ncell; ++n) out[n] = tmp[n] / d;
+}
+
+const int KC = QSS_KIND_C, KS = QSS_KIND_S;
+
+// Guess: movefieldshydrolinlayersplit.cpp:222-1169 (literal, 19 transforms)
+void guess(oracle *o, int first_layer) {
+  std::vector<double> *S = o->S;
+  const double dens = o->p.dens;
+  const double hx = o->p.hx / o->p.nx_split;
+  if (first_layer) {  // :479-494 zero all spectra at the first layer
+    for (int s = 0; s < SP_N; ++s) std::fill(S[s].begin(), S[s].end(), 0.0);
+  }
+  // :637-646 beam P spectrum = previous beam spectrum
+  S[SP_RHOBP] = S[SP_RHOB];
+  S[SP_JXBP] = S[SP_JXB];
+  // :647-700 real-space sources: P-layer rho,E,J; C-layer beam
+  fwd(o, KC, KC, o->C[QSS_RHOBEAM], S[SP_RHOB]);
+  fwd(o, KC, KC, o->C[QSS_JXBEAM], S[SP_JXB]);
+  fwd(o, KC, KC, o->P[QSS_RHO], S[SP_RHO]);
+  fwd(o, KC, KC, o->P[QSS_EX], S[SP_EX]);
+  fwd(o, KC, KS, o->P[QSS_EY], S[SP_EY]);
+  fwd(o, KS, KC, o->P[QSS_EZ], S[SP_EZ]);
+  fwd(o, KC, KC, o->P[QSS_JX], S[SP_JX]);
+  fwd(o, KC, KS, o->P[QSS_JY], S[SP_JY]);
+  fwd(o, KS, KC, o->P[QSS_JZ], S[SP_JZ]);
+  // :807-901 per-mode predictor
+  for (int kk = 0; kk < o->nz; ++kk)
+    for (int jj = 0; jj < o->ny; ++jj) {
+      double aky, akz, ak2;
+      wavenumbers(o, jj, kk, aky, akz, ak2);
+      long n1 = jj + (long)o->ny * kk;
+      double rp = S[SP_RHOP][n1] = S[SP_RHO][n1];
+      double jxp = S[SP_JXP][n1] = S[SP_JX][n1];
+      double jyp = S[SP_JYP][n1] = S[SP_JY][n1];
+      double jzp = S[SP_JZP][n1] = S[SP_JZ][n1];
+      double exp_ = S[SP_EXP][n1] = S[SP_EX][n1];
+      double eyp = S[SP_EYP][n1] = S[SP_EY][n1];
+      double ezp = S[SP_EZP][n1] = S[SP_EZ][n1];
+      double rb = S[SP_RHOB][n1];
+      double rbp = S[SP_RHOBP][n1];
+      double propagator = (4. - dens * hx * hx) / (4. + dens * hx * hx);
+      double denominator = 4. + dens * hx * hx;
+      double rho = propagator * rp - (rb + rbp) * dens * hx * hx / denominator +
+                   4. * exp_ * hx * (ak2 + dens) / denominator;
+      S[SP_RHO][n1] = rho;
+      double ex = propagator * exp_ -
+                  2. * (2. * rp + rb + rbp) * dens * hx / ((ak2 + dens) * denominator);
+      S[SP_EX][n1] = ex;
+      double ey = -eyp + aky * (rb + rbp + rp + rho) / (dens + ak2);
+      double ez = -ezp + akz * (rb + rbp + rp + rho) / (dens + ak2);
+      S[SP_EY][n1] = ey;
+      S[SP_EZ][n1] = ez;
+      double jy = jyp + hx * dens * (ey + eyp) / 2.;
+      double jz = jzp + hx * dens * (ez + ezp) / 2.;
+      double jx = jxp + hx * dens * (ex + exp_) / 2.;
+      S[SP_JY][n1] = jy; S[SP_JZ][n1] = jz; S[SP_JX][n1] = jx;
+      S[SP_BX][n1] = -aky / ak2 * jz + akz / ak2 * jy;
+      S[SP_BY][n1] = (-akz * (jx + S[SP_JXB][n1]) + dens * ez) / ak2;
+      S[SP_BZ][n1] = (aky * (jx + S[SP_JXB][n1]) - dens * ey) / ak2;
+    }
+  // :946-1062 ten inverse transforms + normalisation -> C layer
+  inv(o, KC, KC, S[SP_RHO], o->C[QSS_RHO]);
+  inv(o, KC, KC, S[SP_EX], o->C[QSS_EX]);
+  inv(o, KC, KS, S[SP_EY], o->C[QSS_EY]);
+  inv(o, KS, KC, S[SP_EZ], o->C[QSS_EZ]);
+  inv(o, KC, KC, S[SP_JX], o->C[QSS_JX]);
+  inv(o, KC, KS, S[SP_JY], o->C[QSS_JY]);
+  inv(o, KS, KC, S[SP_JZ], o->C[QSS_JZ]);
+  inv(o, KS, KS, S[SP_BX], o->C[QSS_BX]);
+  inv(o, KS, KC, S[SP_BY], o->C[QSS_BY]);
+  inv(o, KC, KS, S[SP_BZ], o->C[QSS_BZ]);
+}
+
+// Iterate: movefieldshydrolinlayersplit.cpp:1173-1671 (literal, 10
+// transforms).  Returns Eerr (:1584-1621, :1641-1643).  final_pass adds the
+// build's psi plane (not in the reference).
+double iterate(oracle *o, int final_pass) {
+  std::vector<double> *S = o->S;
+  const double dens = o->p.dens;
+  const double hx = o->p.hx / o->p.nx_split;
+  // :1252-1345 forward transforms of the fresh deposit (rRho unused)
+  fwd(o, KC, KC, o->C[QSS_JX], S[SP_JX]);
+  fwd(o, KC, KS, o->C[QSS_JY], S[SP_JY]);
+  fwd(o, KS, KC, o->C[QSS_JZ], S[SP_JZ]);
+  for (int kk = 0; kk < o->nz; ++kk)
+    for (int jj = 0; jj < o->ny; ++jj) {
+      double aky, akz, ak2;
+      wavenumbers(o, jj, kk, aky, akz, ak2);
+      long n1 = jj + (long)o->ny * kk;
+      double rp = S[SP_RHOP][n1];
+      double jy = S[SP_JY][n1];
+      double jz = S[SP_JZ][n1];
+      // :1390-1392 continuity
+      double rho = rp + S[SP_JX][n1] - S[SP_JXP][n1] -
+                   hx * (aky * (jy + S[SP_JYP][n1]) + akz * (jz + S[SP_JZP][n1])) / 2.;
+      double rb = S[SP_RHOB][n1];
+      double rbp = S[SP_RHOBP][n1];
+      double eyp = S[SP_EYP][n1];
+      double ezp = S[SP_EZP][n1];
+      double newEx = -(aky * jy + akz * jz) / ak2;
+      // :1423-1424 uses rho from before this update
+      double newEy = -eyp + aky * (rb + rbp + rp + S[SP_RHO][n1]) / (dens + ak2);
+      double newEz = -ezp + akz * (rb + rbp + rp + S[SP_RHO][n1]) / (dens + ak2);
+      S[SP_EY][n1] = newEy;
+      S[SP_EZ][n1] = newEz;
+      S[SP_RHO][n1] = rho;
+      S[SP_EX][n1] = newEx;
+      S[SP_BX][n1] = -aky / ak2 * S[SP_JZ][n1] + akz / ak2 * S[SP_JY][n1];
+      S[SP_BY][n1] = (-akz * (S[SP_JX][n1] + S[SP_JXB][n1]) + dens * newEz) / ak2;
+      S[SP_BZ][n1] = (aky * (S[SP_JX][n1] + S[SP_JXB][n1]) - dens * newEy) / ak2;
+    }
+  std::vector<double> nE[3];
+  for (int c = 0; c < 3; ++c) nE[c].resize(o->ncell);
+  inv(o, KC, KC, S[SP_RHO], o->C[QSS_RHO]);
+  inv(o, KC, KC, S[SP_EX], nE[0]);
+  inv(o, KC, KS, S[SP_EY], nE[1]);
+  inv(o, KS, KC, S[SP_EZ], nE[2]);
+  inv(o, KS, KS, S[SP_BX], o->C[QSS_BX]);
+  inv(o, KS, KC, S[SP_BY], o->C[QSS_BY]);
+  inv(o, KC, KS, S[SP_BZ], o->C[QSS_BZ]);
+  // residual :1584-1621: maxima floored at 1e-5
+  double err = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    double mx = 1e-5, dif = 0.0;
+    std::vector<double> &old = o->C[QSS_EX + c];
+    for (long n = 0; n < o->ncell; ++n) {
+      double a = std::fabs(nE[c][n]);
+      if (a > mx) mx = a;
+      double d = std::fabs(old[n] - nE[c][n]);
+      if (d > dif) dif = d;
+    }
+    err = std::max(err, dif / mx);
+    old.swap(nE[c]);
+  }
+  if (final_pass) {
+    // psi (build definition): psi_hat = (rho_hat - Jx_hat)/k_perp^2 in the
+    // cos-cos basis, so that d(psi)/d(xi) = Ex holds through continuity.
+    std::vector<double> ps(o->ncell);
+    for (int kk = 0; kk < o->nz; ++kk)
+      for (int jj = 0; jj < o->ny; ++jj) {
+        double aky, akz, ak2;
+        wavenumbers(o, jj, kk, aky, akz, ak2);
+        long n1 = jj + (long)o->ny * kk;
+        ps[n1] = (S[SP_RHO][n1] - S[SP_JX][n1]) / ak2;
+      }
+    inv(o, KC, KC, ps, o->C[QSS_PSI]);
+  }
+  o->last_residual = err;
+  return err;
+}
+
+// One particle deposit event (a sub-step of the move)
+struct Event {
+  int cj, ck;
+  double y, z;
+  double w[4];  // djx, djy, djz, drho already times part_step
+};
+
+// Push of one particle: MoveParticlesLayerSplit body
+// (movepart_splitlayer.cpp:880-1512), gather per `shape`.
+// Returns the number of events written to ev (>= 1).
+int push_one(const oracle *o, const std::vector<double> *F, long n,
+             std::vector<Event> &ev, double &npx, double &npy, double &npz,
+             int &status) {
+  const qss_params &p = o->p;
+  const int ny = o->ny, nz = o->nz;
+  const double hx = p.hx / p.nx_split, hy = o->hy, hz = o->hz;
+  const double q2m = -1.0, djx0 = -1.0, djy0 = -1.0, djz0 = -1.0, drho0 = -1.0;
+  const double weight = o->pweight;
+  int j = o->j[n], k = o->k[n];
+  double yp = o->y[n], zp = o->z[n];
+  if (!(yp >= 0 && yp <= 1 && zp >= 0 && zp <= 1)) status = QSS_E_PARTICLE;  // :945-951
+  double px = o->px[n], py = o->py[n], pz = o->pz[n];
+  double gammap = std::sqrt(1. + px * px + py * py + pz * pz);
+  double Vx = px / gammap;
+  double Vxp = Vx;
+  double Vyp = py / gammap;
+  double Vzp = pz / gammap;
+  // fields: (P + C)/2 at the start cell (NGP_REF, :1101-1217) or its
+  // bilinear interpolation at (j+yp, k+zp) (CIC, movepart_layer_CIC.cpp:166-203)
+  double f[6];
+  long n00 = j + (long)ny * k;
+  if (p.shape == QSS_SHAPE_NGP_REF) {
+    for (int c = 0; c < 6; ++c) f[c] = F[c][n00];
+  } else {
+    int j1 = (j + 1 == ny) ? 0 : j + 1;   // ghost = periodic copy
+    int k1 = (k + 1 == nz) ? 0 : k + 1;
+    long n10 = j1 + (long)ny * k, n01 = j + (long)ny * k1, n11 = j1 + (long)ny * k1;
+    double ayc = 1. - yp, ayp = yp, azc = 1. - zp, azp = zp;
+    double acc = ayc * azc, apc = ayp * azc, acp = ayc * azp, app = ayp * azp;
+    for (int c = 0; c < 6; ++c)
+      f[c] = acc * F[c][n00] + apc * F[c][n10] + acp * F[c][n01] + app * F[c][n11];
+  }
+  double ex = f[0], ey = f[1], ez = f[2], bx = f[3], by = f[4], bz = f[5];
+  ex *= q2m * hx / 2.;
+  ey *= q2m * hx / 2.;
+  ez *= q2m * hx / 2.;
+  px += ex / (1. - Vx);
+  py += ey / (1. - Vx);
+  pz += ez / (1. - Vx);
+  double gamma = std::sqrt(1. + px * px + py * py + pz * pz);
+  double gamma_r = 1. / gamma;
+  bx = bx * gamma_r * q2m / (1. - Vx) * hx / 2.;
+  by = by * gamma_r * q2m / (1. - Vx) * hx / 2.;
+  bz = bz * gamma_r * q2m / (1. - Vx) * hx / 2.;
+  double co = 2. / (1. + (bx * bx) + (by * by) + (bz * bz));
+  double p3x = py * bz - pz * by + px;
+  double p3y = pz * bx - px * bz + py;
+  double p3z = px * by - py * bx + pz;
+  p3x *= co;
+  p3y *= co;
+  p3z *= co;
+  double px_new = p3y * bz - p3z * by;
+  double py_new = p3z * bx - p3x * bz;
+  double pz_new = p3x * by - p3y * bx;
+  px += ex / (1. - Vx) + px_new;
+  py += ey / (1. - Vx) + py_new;
+  pz += ez / (1. - Vx) + pz_new;
+  gamma = std::sqrt(1. + px * px + py * py + pz * pz);
+  double Vxm = px / gamma;
+  double Vym = py / gamma;
+  double Vzm = pz / gamma;
+  Vx = 0.5 * (Vxm + Vxp);
+  double Vy = 0.5 * (Vym + Vyp);
+  double Vz = 0.5 * (Vzm + Vzp);
+  double djx = weight * djx0 * Vxm;
+  double djy = weight * djy0 * Vym;
+  double djz = weight * djz0 * Vzm;
+  double drho = weight * drho0;
+  djx *= 1. / (1. - Vx);
+  djy *= 1. / (1. - Vx);
+  djz *= 1. / (1. - Vx);
+  drho *= 1. / (1. - Vx);
+  double dy = Vy * hx / hy;
+  double dz = Vz * hx / hz;
+  dy = dy / (1. - Vx);
+  dz = dz / (1. - Vx);
+  // sub-cycling :1380-1466
+  double partdx = 0., step = 1.;
+  if (std::fabs(dy) > 1. || std::fabs(dz) > 1.) {
+    if (std::fabs(dy) > std::fabs(dz)) step = partdx = std::fabs(dy);
+    else step = partdx = std::fabs(dz);
+  }
+  if (partdx < 1.) partdx = step = 1.;
+  // build guard: more than 62 sub-steps or a non-finite displacement is an
+  // error (the reference would loop for ever on inf)
+  if (!(step < 63.0)) {
+    status = QSS_E_PARTICLE;
+    partdx = step = 1.;
+    dy = 0.;
+    dz = 0.;
+  }
+  double ytmp = yp, ztmp = zp;
+  int jcur = j, kcur = k;
+  int nev = 0;
+  while (partdx > 0.) {
+    double part_step;
+    if (partdx > 1.) { partdx -= 1.; part_step = 1. / step; }
+    else { part_step = partdx / step; partdx = 0.; }
+    // the reference restarts every sub-step from the start cell (its inner
+    // j_jump/k_jump shadow the outer ones, :1380-1381 vs :1434-1435)
+    int jb = p.fix_subcycle ? jcur : j;
+    int kb = p.fix_subcycle ? kcur : k;
+    ytmp += dy * part_step + jb;
+    ztmp += dz * part_step + kb;
+    if (!(ytmp > -2.0 * ny && ytmp < 3.0 * ny && ztmp > -2.0 * nz && ztmp < 3.0 * nz)) {
+      status = QSS_E_PARTICLE;
+      ytmp = 0.0;
+      ztmp = 0.0;
+    }
+    while (ytmp > ny) ytmp -= ny;
+    while (ytmp < 0) ytmp += ny;
+    while (ztmp > nz) ztmp -= nz;
+    while (ztmp < 0) ztmp += nz;
+    int jj = (int)ytmp;
+    int kk = (int)ztmp;
+    ytmp -= jj;
+    ztmp -= kk;
+    // build convention: a particle landing exactly on y = Ny (z = Nz) is
+    // wrapped to cell 0 instead of being parked in the ghost cell
+    if (jj >= ny) jj -= ny;
+    if (kk >= nz) kk -= nz;
+    Event e;
+    e.cj = jj; e.ck = kk; e.y = ytmp; e.z = ztmp;
+    e.w[0] = djx * part_step; e.w[1] = djy * part_step;
+    e.w[2] = djz * part_step; e.w[3] = drho * part_step;
+    ev.push_back(e);
+    ++nev;
+    jcur = jj; kcur = kk;
+  }
+  npx = px; npy = py; npz = pz;
+  return nev;
+}
+
+// MoveParticlesLayerSplit + DepositCurrentsInCellSplit + ExchangeCurrentsSplit
+// Events are reduced per target cell in event order (particle slot, then
+// sub-step) into four corner sums, then nodes are combined in the fixed
+// order c00 + c10 + c01 + c11 (periodic).  That equals the reference's
+// ghost fold (:1685-1711) up to summation order.
+int push_deposit(oracle *o, int full_step) {
+  const int ny = o->ny, nz = o->nz;
+  const long ncell = o->ncell;
+  const long np = (long)o->y.size();
+  // ClearCurrentsSplit / ClearRhoSplit (mesh.cpp:692-708)
+  for (int c = QSS_JX; c <= QSS_RHO; ++c) std::fill(o->C[c].begin(), o->C[c].end(), 0.0);
+  std::vector<double> F[6];
+  for (int c = 0; c < 6; ++c) {
+    F[c].resize(ncell);
+    for (long n = 0; n < ncell; ++n) F[c][n] = 0.5 * (o->P[c][n] + o->C[c][n]);
+  }
+  // per-particle events (parallel, deterministic)
+  std::vector<std::vector<Event>> evs(np);
+  std::vector<double> npx(np), npy(np), npz(np);
+  int status = QSS_OK;
+#pragma omp parallel for num_threads(o->nthreads) schedule(static) reduction(min:status)
+  for (long n = 0; n < np; ++n) {
+    int st = QSS_OK;
+    push_one(o, F, n, evs[n], npx[n], npy[n], npz[n], st);
+    status = std::min(status, st);
+  }
+  // deposit in slot order into per-cell corner sums
+  std::vector<double> corner((size_t)ncell * 16, 0.0);
+  const bool cic = (o->p.shape == QSS_SHAPE_CIC);
+  for (long n = 0; n < np; ++n)
+    for (const Event &e : evs[n]) {
+      double *cs = &corner[(size_t)(e.cj + (long)ny * e.ck) * 16];
+      if (cic) {
+        double ayp = e.y, ayc = 1. - ayp, azp = e.z, azc = 1. - azp;
+        for (int q = 0; q < 4; ++q) {
+          cs[0 * 4 + q] += e.w[q] * ayc * azc;
+          cs[1 * 4 + q] += e.w[q] * ayp * azc;
+          cs[2 * 4 + q] += e.w[q] * ayc * azp;
+          cs[3 * 4 + q] += e.w[q] * ayp * azp;
+        }
+      } else {
+        for (int q = 0; q < 4; ++q) cs[q] += e.w[q];
+      }
+    }
+  for (int kk = 0; kk < nz; ++kk)
+    for (int jj = 0; jj < ny; ++jj) {
+      int jm = (jj == 0) ? ny - 1 : jj - 1;
+      int km = (kk == 0) ? nz - 1 : kk - 1;
+      const double *c00 = &corner[(size_t)(jj + (long)ny * kk) * 16];
+      const double *c10 = &corner[(size_t)(jm + (long)ny * kk) * 16 + 4];
+      const double *c01 = &corner[(size_t)(jj + (long)ny * km) * 16 + 8];
+      const double *c11 = &corner[(size_t)(jm + (long)ny * km) * 16 + 12];
+      long nn = jj + (long)ny * kk;
+      for (int q = 0; q < 4; ++q)
+        o->C[QSS_JX + q][nn] = ((c00[q] + c10[q]) + c01[q]) + c11[q];
+    }
+  if (full_step) {
+    // particles move to the C layer: last sub-step position, new momenta;
+    // storage re-sorted by (cell, previous slot) -- stable.
+    std::vector<long> key(np);
+    for (long n = 0; n < np; ++n) {
+      const Event &e = evs[n].back();
+      key[n] = e.cj + (long)ny * e.ck;
+    }
+    std::vector<long> order(np);
+    std::iota(order.begin(), order.end(), 0L);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](long a, long b) { return key[a] < key[b]; });
+    std::vector<double> ny_(np), nz_(np), nx1(np), ny1(np), nz1(np);
+    std::vector<int> nj(np), nk(np);
+    std::vector<int64_t> nid(np);
+    for (long q = 0; q < np; ++q) {
+      long n = order[q];
+      const Event &e = evs[n].back();
+      ny_[q] = e.y; nz_[q] = e.z; nj[q] = e.cj; nk[q] = e.ck;
+      nx1[q] = npx[n]; ny1[q] = npy[n]; nz1[q] = npz[n]; nid[q] = o->id[n];
+    }
+    o->y.swap(ny_); o->z.swap(nz_); o->j.swap(nj); o->k.swap(nk);
+    o->px.swap(nx1); o->py.swap(ny1); o->pz.swap(nz1); o->id.swap(nid);
+  }
+  return status;
+}
+
+// Beam planes of the C layer for (slice, isplit): movepart_splitlayer.cpp:
+// 330-348 interpolation between reference layers i and i+1.
+void beam_layer(oracle *o, int slice, int isplit) {
+  const int L = o->p.nslices, S = o->p.nx_split;
+  int i = L - 1 - slice;
+  std::vector<double> rc(o->ncell), jc(o->ncell);
+  beam_plane(o, i, rc.data(), jc.data());
+  if (S == 1) {
+    o->C[QSS_RHOBEAM] = rc; o->C[QSS_JXBEAM] = jc;
+    return;
+  }
+  std::vector<double> rp(o->ncell), jp(o->ncell);
+  beam_plane(o, i + 1, rp.data(), jp.data());
+  for (long n = 0; n < o->ncell; ++n) {
+    o->C[QSS_RHOBEAM][n] = (rc[n] * (S - isplit) + rp[n] * isplit) / S;
+    o->C[QSS_JXBEAM][n] = (jc[n] * (S - isplit) + jp[n] * isplit) / S;
+  }
+}
+
+void copy_c_to_p(oracle *o) {
+  // Cell::operator= (cells.cpp:25-47): fields, currents and density
+  for (int f = QSS_EX; f <= QSS_RHO; ++f) o->P[f] = o->C[f];
+  o->P[QSS_PSI] = o->C[QSS_PSI];
+}
+
+int layer_step(oracle *o, int slice, int isplit, double *residual) {
+  beam_layer(o, slice, isplit);
+  guess(o, slice == 0);
+  int st = QSS_OK;
+  for (int it = 0; it < o->p.n_iter; ++it) {
+    st = std::min(st, push_deposit(o, 0));
+    iterate(o, 0);
+  }
+  st = std::min(st, push_deposit(o, 1));
+  double err = iterate(o, 1);
+  if (residual) *residual = err;
+  copy_c_to_p(o);
+  if (!std::isfinite(err)) st = std::min(st, QSS_E_NONFINITE);
+  return st;
+}
+
+void reset(oracle *o) {
+  for (int f = 0; f < QSS_NFIELD; ++f) {
+    std::fill(o->C[f].begin(), o->C[f].end(), 0.0);
+    std::fill(o->P[f].begin(), o->P[f].end(), 0.0);
+  }
+  for (int s = 0; s < SP_N; ++s) std::fill(o->S[s].begin(), o->S[s].end(), 0.0);
+  seed_plasma(o);
+}
+
+}  // namespace
+
+extern "C" {
+
+oracle *oracle_create(const qss_params *p, int use_fft, int nthreads,
+                      uint32_t history_mask) {
+  if (!p || p->ny < 2 || p->nz < 2 || p->nslices < 1 || p->nx_split < 1 ||
+      p->ppc_y < 1 || p->ppc_z < 1 || p->nbunch < 0 || p->nbunch > QSS_MAX_BUNCH)
+    return nullptr;
+  oracle *o = new oracle();
+  o->p = *p;
+  if (p->nbins > 0 && p->bins) o->bins.assign(p->bins, p->bins + p->nbins);
+  o->p.bins = nullptr;
+  o->ny = p->ny; o->nz = p->nz;
+  o->ncell = (long)p->ny * p->nz;
+  o->hy = p->ly / p->ny;
+  o->hz = p->lz / p->nz;
+  o->use_fft = use_fft != 0;
+  o->nthreads = nthreads > 0 ? nthreads : 1;
+  o->ply.init(p->ny);
+  o->plz.init(p->nz);
+  for (int f = 0; f < QSS_NFIELD; ++f) {
+    o->C[f].assign(o->ncell, 0.0);
+    o->P[f].assign(o->ncell, 0.0);
+  }
+  for (int s = 0; s < SP_N; ++s) o->S[s].assign(o->ncell, 0.0);
+  o->hmask = history_mask;
+  for (int f = 0; f < QSS_NFIELD; ++f)
+    if (history_mask & (1u << f))
+      o->hist[f].assign((size_t)o->ncell * p->nslices, 0.0);
+  seed_plasma(o);
+  return o;
+}
+
+void oracle_destroy(oracle *o) { delete o; }
+
+int oracle_reset(oracle *o) { reset(o); return QSS_OK; }
+
+int oracle_n_particles(const oracle *o) { return (int)o->y.size(); }
+
+int oracle_layer_step(oracle *o, int slice, int isplit, double *residual) {
+  return layer_step(o, slice, isplit, residual);
+}
+
+// movepart_splitlayer.cpp:165-478: per layer P <- previous layer, splits,
+// store the layer (history).
+int oracle_sweep_range(oracle *o, int first, int count, double *residuals) {
+  int st = QSS_OK;
+  for (int s = first; s < first + count && s < o->p.nslices; ++s) {
+    for (int is = 0; is < o->p.nx_split; ++is) {
+      double r = 0;
+      st = std::min(st, layer_step(o, s, is, &r));
+      if (residuals) residuals[(s - first) * o->p.nx_split + is] = r;
+    }
+    for (int f = 0; f < QSS_NFIELD; ++f)
+      if (!o->hist[f].empty())
+        std::copy(o->P[f].begin(), o->P[f].end(),
+                  o->hist[f].begin() + (size_t)s * o->ncell);
+  }
+  return st;
+}
+
+int oracle_sweep(oracle *o, double *residuals) {
+  reset(o);
+  return oracle_sweep_range(o, 0, o->p.nslices, residuals);
+}
+
+int oracle_beam_layer(oracle *o, int slice, int isplit) {
+  beam_layer(o, slice, isplit); return QSS_OK;
+}
+int oracle_beam_plane(oracle *o, int layer, double *rho, double *jx) {
+  beam_plane(o, layer, rho, jx); return QSS_OK;
+}
+int oracle_guess(oracle *o, int first_layer) { guess(o, first_layer); return QSS_OK; }
+int oracle_push_deposit(oracle *o, int full_step) { return push_deposit(o, full_step); }
+int oracle_iterate(oracle *o, int final_pass, double *residual) {
+  double r = iterate(o, final_pass);
+  if (residual) *residual = r;
+  return QSS_OK;
+}
+int oracle_copy_c_to_p(oracle *o) { copy_c_to_p(o); return QSS_OK; }
+
+int oracle_get_layer_field(oracle *o, int which, int field, double *out) {
+  if (field < 0 || field >= QSS_NFIELD) return QSS_E_ARG;
+  std::vector<double> &v = which ? o->P[field] : o->C[field];
+  std::copy(v.begin(), v.end(), out);
+  return QSS_OK;
+}
+int oracle_set_layer_field(oracle *o, int which, int field, const double *in) {
+  if (field < 0 || field >= QSS_NFIELD) return QSS_E_ARG;
+  std::vector<double> &v = which ? o->P[field] : o->C[field];
+  std::copy(in, in + o->ncell, v.begin());
+  return QSS_OK;
+}
+int oracle_get_spectral(oracle *o, int sid, double *out) {
+  if (sid < 0 || sid >= SP_N) return QSS_E_ARG;
+  std::copy(o->S[sid].begin(), o->S[sid].end(), out);
+  return QSS_OK;
+}
+int oracle_set_spectral(oracle *o, int sid, const double *in) {
+  if (sid < 0 || sid >= SP_N) return QSS_E_ARG;
+  std::copy(in, in + o->ncell, o->S[sid].begin());
+  return QSS_OK;
+}
+int oracle_get_history(oracle *o, int field, int slice, double *out) {
+  if (field < 0 || field >= QSS_NFIELD || o->hist[field].empty() || slice < 0 ||
+      slice >= o->p.nslices)
+    return QSS_E_ARG;
+  std::copy(o->hist[field].begin() + (size_t)slice * o->ncell,
+            o->hist[field].begin() + (size_t)(slice + 1) * o->ncell, out);
+  return QSS_OK;
+}
+// particles: y,z as global cell coordinates j+y, k+z
+int oracle_get_particles(oracle *o, double *y, double *z, double *px, double *py,
+                         double *pz, int64_t *id, int32_t *cj, int32_t *ck,
+                         double *fy, double *fz) {
+  long np = (long)o->y.size();
+  for (long n = 0; n < np; ++n) {
+    if (y) y[n] = o->j[n] + o->y[n];
+    if (z) z[n] = o->k[n] + o->z[n];
+    if (px) px[n] = o->px[n];
+    if (py) py[n] = o->py[n];
+    if (pz) pz[n] = o->pz[n];
+    if (id) id[n] = o->id[n];
+    if (cj) cj[n] = o->j[n];
+    if (ck) ck[n] = o->k[n];
+    if (fy) fy[n] = o->y[n];
+    if (fz) fz[n] = o->z[n];
+  }
+  return QSS_OK;
+}
+// set particles from cell index + in-cell fraction (exact state)
+int oracle_set_particles(oracle *o, long np, const int32_t *cj, const int32_t *ck,
+                         const double *fy, const double *fz, const double *px,
+                         const double *py, const double *pz, const int64_t *id) {
+  o->y.assign(fy, fy + np); o->z.assign(fz, fz + np);
+  o->j.assign(cj, cj + np); o->k.assign(ck, ck + np);
+  o->px.assign(px, px + np); o->py.assign(py, py + np); o->pz.assign(pz, pz + np);
+  o->id.assign(id, id + np);
+  return QSS_OK;
+}
+
+// standalone transforms (for fixtures / transform parity)
+void oracle_transform1d(int n, int kind, const double *in, double *out, int use_fft) {
+  Plan1D pl; pl.init(n);
+  std::vector<cplx> work(n); std::vector<double> xbuf(n);
+  pl.apply(kind, in, 1, out, 1, use_fft != 0, work.data(), xbuf.data());
+}
+void oracle_transform2d(int ny, int nz, int kind_z, int kind_y, const double *in,
+                        double *out, double scale, int use_fft, int nthreads) {
+  Plan1D py_, pz_; py_.init(ny); pz_.init(nz);
+  transform2d(py_, pz_, kind_z, kind_y, in, out, scale, use_fft != 0,
+              nthreads > 0 ? nthreads : 1);
+}
+
+}  // extern "C"

@@ -1,0 +1,15 @@
+"""tertia_amd — MI355X-native quasi-static xi-slice wake sweep.
+
+Product path: ``libqss.so`` (hand-written HIP kernels for gfx950 behind the C
+ABI of ``include/qss.h``) and the host mirror :class:`tertia_amd.mesh.Mesh`
+of the reference's ``Mesh`` layer-step interface.
+"""
+from .params import (Bunch, Params, config0, config1, config2, config4, small,  # noqa: F401
+                     FIELDS, FIELD_KINDS, QSS_KIND_C, QSS_KIND_S, QSS_SHAPE_CIC,
+                     QSS_SHAPE_NGP_REF, QSS_PREC_FP64, QSS_PREC_MIXED32)
+
+
+def Mesh(*args, **kwargs):
+    """Create a :class:`tertia_amd.mesh.Mesh` (loads libqss.so, raises if absent)."""
+    from .mesh import Mesh as _Mesh
+    return _Mesh(*args, **kwargs)

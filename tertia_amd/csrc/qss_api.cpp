@@ -1,0 +1,748 @@
+// qss_api.cpp — C ABI and host-side sweep driver of libqss.
+//
+// The host code mirrors the reference's orchestration:
+//   qss_sweep_range  <-> Mesh::MoveAllSplitLayers  (movepart_splitlayer.cpp:25-558)
+//   qss_layer_step   <-> Mesh::MoveSplitLayer      (movepart_splitlayer.cpp:561-778)
+//   qss_guess        <-> GuessFieldsHydroLinLayerSplit (movefieldshydrolinlayersplit.cpp:222)
+//   qss_push_deposit <-> ClearCurrentsSplit + MoveParticlesLayerSplit +
+//                        ExchangeCurrentsSplit (mesh.cpp:692, movepart_splitlayer.cpp:781,1685)
+//   qss_iterate      <-> IterateFieldsHydroLinLayerSplit + ExchangeFieldsSplit (:1173)
+// but all data stays in HBM: the reference's Cell copies P<-layer, C->P and
+// layer<-P (:170-187, :423-428, :467-473) become pointer swaps, and the
+// spectral P state is kept instead of being re-transformed.
+
+#include "qss_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+namespace {
+
+thread_local std::string g_err;
+
+#define HIPCHECK(x)                                                              \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess) {                                                      \
+      qss_set_error(std::string(#x) + ": " + hipGetErrorString(e_));             \
+      return QSS_E_HIP;                                                          \
+    }                                                                            \
+  } while (0)
+
+#define CHECK(x)                                                                 \
+  do {                                                                           \
+    int r_ = (x);                                                                \
+    if (r_ != QSS_OK) return r_;                                                 \
+  } while (0)
+
+constexpr int kNClass = QSS_NKCLASS;
+
+// ---- timing helpers --------------------------------------------------------
+struct Timed {
+  qss_ctx *c;
+  int cls;
+  hipEvent_t e1 = nullptr;
+  Timed(qss_ctx *c_, int cls_) : c(c_), cls(cls_) {
+    if (!c->timing) return;
+    auto &pool = c->ev_pool[cls];
+    size_t &used = c->ev_used[cls];
+    if (used == pool.size()) {
+      hipEvent_t a, b;
+      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        c->timing = false;
+        return;
+      }
+      pool.emplace_back(a, b);
+    }
+    (void)hipEventRecord(pool[used].first, c->stream);
+    e1 = pool[used].second;
+    ++used;
+  }
+  ~Timed() {
+    if (e1) (void)hipEventRecord(e1, c->stream);
+  }
+};
+
+template <class T>
+int dalloc(T **p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    qss_set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return QSS_E_NOMEM;
+  }
+  e = hipMemset(*p, 0, n * sizeof(T));
+  if (e != hipSuccess) {
+    qss_set_error(std::string("hipMemset: ") + hipGetErrorString(e));
+    return QSS_E_HIP;
+  }
+  return QSS_OK;
+}
+
+template <class T>
+void dfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// dense half-integer tables, exact integer argument reduction:
+// angle = pi (2j+1)(2k+1)/(4n) reduced mod 2 pi via (2j+1)(2k+1) mod 8n
+void make_tables(int n, std::vector<double> &tc, std::vector<double> &ts) {
+  tc.resize((size_t)n * n);
+  ts.resize((size_t)n * n);
+  for (int j = 0; j < n; ++j)
+    for (int k = 0; k < n; ++k) {
+      long m = ((long)(2 * j + 1) * (2 * k + 1)) % (8L * n);
+      double a = M_PI * (double)m / (4.0 * n);
+      tc[(size_t)j * n + k] = 2.0 * std::cos(a);
+      ts[(size_t)j * n + k] = 2.0 * std::sin(a);
+    }
+}
+
+int validate(const qss_params *p) {
+  if (!p) { qss_set_error("null params"); return QSS_E_ARG; }
+  if (p->ny < 2 || p->nz < 2 || p->ny > 16384 || p->nz > 16384) {
+    qss_set_error("ny, nz must be in [2, 16384]");
+    return QSS_E_ARG;
+  }
+  if (!(p->ly > 0) || !(p->lz > 0) || !(p->hx > 0) || !(p->dens > 0)) {
+    qss_set_error("ly, lz, hx, dens must be positive");
+    return QSS_E_ARG;
+  }
+  if (p->nslices < 1 || p->nx_split < 1 || p->n_iter < 0 || p->ppc_y < 1 ||
+      p->ppc_z < 1 || p->nbunch < 0 || p->nbunch > QSS_MAX_BUNCH) {
+    qss_set_error("bad nslices / nx_split / n_iter / ppc / nbunch");
+    return QSS_E_ARG;
+  }
+  if (p->shape != QSS_SHAPE_NGP_REF && p->shape != QSS_SHAPE_CIC) {
+    qss_set_error("unknown shape");
+    return QSS_E_ARG;
+  }
+  if (p->precision != QSS_PREC_FP64) {
+    qss_set_error("precision mode not available in this build");
+    return QSS_E_ARG;
+  }
+  if (p->nbins < 0 || (p->nbins > 0 && (!p->bins || !(p->bin_xi1 > p->bin_xi0)))) {
+    qss_set_error("bad piecewise profile");
+    return QSS_E_ARG;
+  }
+  long long np = (long long)p->ny * p->nz * p->ppc_y * p->ppc_z;
+  if (np > (1LL << 30)) {
+    qss_set_error("too many particles per layer");
+    return QSS_E_ARG;
+  }
+  return QSS_OK;
+}
+
+// ---- transforms ------------------------------------------------------------
+// out_b = alpha * T_kz[b] . in_b . T_ky[b]: y-pass into tmp, then z-pass
+int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
+                    const double *const *in, double *const *out, double alpha,
+                    const int *resid, const int *push, const double *const *pplane) {
+  if (n < 1 || n > kGemmMaxBatch) {
+    qss_set_error("transform batch size");
+    return QSS_E_ARG;
+  }
+  GemmArgs g{};
+  g.M = c->nz; g.N = c->ny; g.K = c->ny;
+  g.alpha = 1.0;
+  g.nbatch = n;
+  g.F = c->F;
+  g.resid = c->d_resid;
+  for (int b = 0; b < n; ++b) {
+    g.b[b].A = in[b];
+    g.b[b].B = c->d_T[ky[b] == QSS_KIND_C ? 0 : 1];
+    g.b[b].C = c->tmp[b];
+    g.b[b].P = nullptr;
+    g.b[b].resid = -1;
+    g.b[b].push = -1;
+  }
+  {
+    Timed t(c, KC_GEMM);
+    HIPCHECK(launch_gemm_f64(g, c->stream));
+  }
+  if (c->timing) c->gemm_flops += 2.0 * g.M * g.N * g.K * n;
+  g.M = c->nz; g.N = c->ny; g.K = c->nz;
+  g.alpha = alpha;
+  for (int b = 0; b < n; ++b) {
+    g.b[b].A = c->d_T[kz[b] == QSS_KIND_C ? 2 : 3];
+    g.b[b].B = c->tmp[b];
+    g.b[b].C = out[b];
+    g.b[b].resid = resid ? resid[b] : -1;
+    g.b[b].push = push ? push[b] : -1;
+    g.b[b].P = pplane ? pplane[b] : nullptr;
+  }
+  {
+    Timed t(c, KC_GEMM);
+    HIPCHECK(launch_gemm_f64(g, c->stream));
+  }
+  if (c->timing) c->gemm_flops += 2.0 * g.M * g.N * g.K * n;
+  return QSS_OK;
+}
+
+// push field (P+C)/2 from the current planes (after host-side edits)
+int refresh_push_field(qss_ctx *c) {
+  HIPCHECK(launch_push_field(c));
+  return QSS_OK;
+}
+
+// ---- the pieces of a layer step ---------------------------------------------
+int do_beam(qss_ctx *c, int slice, int isplit) {
+  Timed t(c, KC_BEAM);
+  HIPCHECK(launch_beam_plane(c, slice, isplit));
+  return QSS_OK;
+}
+
+int do_guess(qss_ctx *c, int first) {
+  if (first) {
+    Timed t(c, KC_SPEC);
+    for (int s = 0; s < SP_N; ++s) HIPCHECK(launch_fill_f64(c->S[s], c->ncell, 0.0, c->stream));
+  }
+  // beam P spectrum = previous beam spectrum (:637-646): swap buffers
+  std::swap(c->S[SP_RHOBP], c->S[SP_RHOB]);
+  std::swap(c->S[SP_JXBP], c->S[SP_JXB]);
+  {
+    const int kz[2] = {QSS_KIND_C, QSS_KIND_C}, ky[2] = {QSS_KIND_C, QSS_KIND_C};
+    const double *in[2] = {c->C[QSS_RHOBEAM], c->C[QSS_JXBEAM]};
+    double *out[2] = {c->S[SP_RHOB], c->S[SP_JXB]};
+    CHECK(transform_batch(c, 2, kz, ky, in, out, 1.0, nullptr, nullptr, nullptr));
+  }
+  {
+    Timed t(c, KC_SPEC);
+    HIPCHECK(launch_guess_update(c, first));
+  }
+  // six inverse transforms -> C-layer E,B, with the push-field epilogue
+  const int kz[6] = {QSS_KIND_C, QSS_KIND_C, QSS_KIND_S, QSS_KIND_S, QSS_KIND_S, QSS_KIND_C};
+  const int ky[6] = {QSS_KIND_C, QSS_KIND_S, QSS_KIND_C, QSS_KIND_S, QSS_KIND_C, QSS_KIND_S};
+  const double *in[6] = {c->S[SP_EX], c->S[SP_EY], c->S[SP_EZ],
+                         c->S[SP_BX], c->S[SP_BY], c->S[SP_BZ]};
+  double *out[6] = {c->C[QSS_EX], c->C[QSS_EY], c->C[QSS_EZ],
+                    c->C[QSS_BX], c->C[QSS_BY], c->C[QSS_BZ]};
+  const int push[6] = {0, 1, 2, 3, 4, 5};
+  const double *pp[6] = {c->P[QSS_EX], c->P[QSS_EY], c->P[QSS_EZ],
+                         c->P[QSS_BX], c->P[QSS_BY], c->P[QSS_BZ]};
+  const double alpha = 1.0 / (4.0 * c->ny * c->nz);
+  return transform_batch(c, 6, kz, ky, in, out, alpha, nullptr, push, pp);
+}
+
+int do_push_deposit(qss_ctx *c, int full) {
+  {
+    Timed t(c, KC_SORT);
+    HIPCHECK(launch_fill_i32(c->cnt, c->ncell, 0, c->stream));
+    if (full) HIPCHECK(launch_fill_i32(c->cnt_main, c->ncell, 0, c->stream));
+    HIPCHECK(launch_fill_i32(c->d_counters, 1, 0, c->stream));
+  }
+  {
+    Timed t(c, KC_PUSH);
+    HIPCHECK(launch_push(c, full));
+  }
+  {
+    Timed t(c, KC_SORT);
+    HIPCHECK(launch_sort_scan(c, full));
+    HIPCHECK(launch_scatter(c));
+  }
+  {
+    Timed t(c, KC_REDUCE);
+    HIPCHECK(launch_tile_reduce(c, full));
+  }
+  if (full) c->cur ^= 1;
+  return QSS_OK;
+}
+
+int do_iterate(qss_ctx *c, int final_pass, double *res_dev) {
+  {
+    const int kz[3] = {QSS_KIND_C, QSS_KIND_C, QSS_KIND_S};
+    const int ky[3] = {QSS_KIND_C, QSS_KIND_S, QSS_KIND_C};
+    const double *in[3] = {c->C[QSS_JX], c->C[QSS_JY], c->C[QSS_JZ]};
+    double *out[3] = {c->S[SP_JX], c->S[SP_JY], c->S[SP_JZ]};
+    CHECK(transform_batch(c, 3, kz, ky, in, out, 1.0, nullptr, nullptr, nullptr));
+  }
+  {
+    Timed t(c, KC_SPEC);
+    HIPCHECK(launch_iterate_update(c, final_pass));
+    HIPCHECK(launch_fill_u64(c->d_resid, 6, 0ull, c->stream));
+  }
+  // E (residual + push field), B (push field); final pass: rho and psi too
+  int kz[8] = {QSS_KIND_C, QSS_KIND_C, QSS_KIND_S, QSS_KIND_S, QSS_KIND_S, QSS_KIND_C,
+               QSS_KIND_C, QSS_KIND_C};
+  int ky[8] = {QSS_KIND_C, QSS_KIND_S, QSS_KIND_C, QSS_KIND_S, QSS_KIND_C, QSS_KIND_S,
+               QSS_KIND_C, QSS_KIND_C};
+  const double *in[8] = {c->S[SP_EX], c->S[SP_EY], c->S[SP_EZ], c->S[SP_BX],
+                         c->S[SP_BY], c->S[SP_BZ], c->S[SP_RHO], c->S[SP_PSI]};
+  double *out[8] = {c->C[QSS_EX], c->C[QSS_EY], c->C[QSS_EZ], c->C[QSS_BX],
+                    c->C[QSS_BY], c->C[QSS_BZ], c->C[QSS_RHO], c->C[QSS_PSI]};
+  int resid[8] = {0, 1, 2, -1, -1, -1, -1, -1};
+  int push[8] = {0, 1, 2, 3, 4, 5, -1, -1};
+  if (final_pass)
+    for (int b = 0; b < 6; ++b) push[b] = -1;  // the next Guess rebuilds it
+  const double *pp[8] = {c->P[QSS_EX], c->P[QSS_EY], c->P[QSS_EZ], c->P[QSS_BX],
+                         c->P[QSS_BY], c->P[QSS_BZ], nullptr, nullptr};
+  const double alpha = 1.0 / (4.0 * c->ny * c->nz);
+  CHECK(transform_batch(c, final_pass ? 8 : 6, kz, ky, in, out, alpha, resid, push, pp));
+  if (res_dev) {
+    Timed t(c, KC_SPEC);
+    HIPCHECK(launch_residual_final(c, res_dev));
+  }
+  return QSS_OK;
+}
+
+void swap_c_p(qss_ctx *c) {
+  // Cell::operator= moves fields, currents, density (cells.cpp:25-47)
+  for (int f = QSS_EX; f <= QSS_RHO; ++f) std::swap(c->C[f], c->P[f]);
+  std::swap(c->C[QSS_PSI], c->P[QSS_PSI]);
+}
+
+int do_layer_step(qss_ctx *c, int slice, int isplit, double *res_dev) {
+  CHECK(do_beam(c, slice, isplit));
+  CHECK(do_guess(c, slice == 0));
+  for (int it = 0; it < c->p.n_iter; ++it) {
+    CHECK(do_push_deposit(c, 0));
+    CHECK(do_iterate(c, 0, nullptr));
+  }
+  CHECK(do_push_deposit(c, 1));
+  CHECK(do_iterate(c, 1, res_dev));
+  swap_c_p(c);
+  return QSS_OK;
+}
+
+int check_status(qss_ctx *c) {
+  int32_t h[2] = {0, 0};
+  HIPCHECK(hipMemcpyAsync(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  if (h[1] & 2) { qss_set_error("sub-cycle overflow buffer exhausted"); return QSS_E_OVERFLOW; }
+  if (h[1] & 4) { qss_set_error("displacement needs more than 62 sub-steps"); return QSS_E_PARTICLE; }
+  if (h[1] & 1) { qss_set_error("particle outside its cell or non-finite"); return QSS_E_PARTICLE; }
+  return QSS_OK;
+}
+
+int do_reset(qss_ctx *c) {
+  for (int f = 0; f < QSS_NFIELD; ++f) {
+    HIPCHECK(launch_fill_f64(c->C[f], c->ncell, 0.0, c->stream));
+    HIPCHECK(launch_fill_f64(c->P[f], c->ncell, 0.0, c->stream));
+  }
+  for (int s = 0; s < SP_N; ++s) HIPCHECK(launch_fill_f64(c->S[s], c->ncell, 0.0, c->stream));
+  HIPCHECK(launch_fill_f64(c->F, c->ncell * kPushStride, 0.0, c->stream));
+  HIPCHECK(launch_fill_i32(c->d_counters, 2, 0, c->stream));
+  c->cur = 0;
+  c->np = c->np_cap;
+  HIPCHECK(launch_seed_plasma(c));
+  return QSS_OK;
+}
+
+}  // namespace
+
+void qss_set_error(const std::string &m) { g_err = m; }
+
+extern "C" {
+
+const char *qss_last_error(void) { return g_err.c_str(); }
+
+int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx **out) {
+  if (!out) { qss_set_error("null out"); return QSS_E_ARG; }
+  *out = nullptr;
+  CHECK(validate(p));
+  int ndev = 0;
+  HIPCHECK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) { qss_set_error("bad device"); return QSS_E_ARG; }
+  HIPCHECK(hipSetDevice(device));
+  qss_ctx *c = new qss_ctx();
+  c->p = *p;
+  if (p->nbins > 0) c->bins.assign(p->bins, p->bins + p->nbins);
+  c->p.bins = nullptr;
+  c->device = device;
+  c->ny = p->ny; c->nz = p->nz;
+  c->ncell = (long)p->ny * p->nz;
+  c->np_cap = c->ncell * p->ppc_y * p->ppc_z;
+  c->np = c->np_cap;
+  c->ov_cap = std::max<long>(c->np_cap, 4096);
+  c->hxs = p->hx / p->nx_split;
+  c->hy = p->ly / p->ny;
+  c->hz = p->lz / p->nz;
+  c->pweight = (1.0 / (p->ppc_y * p->ppc_z)) * p->dens;
+  c->nblk = (int)((c->ncell + 1023) / 1024);
+  int r = QSS_OK;
+  auto fail = [&](int code) { qss_destroy(c); return code; };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    qss_set_error("hipStreamCreate failed");
+    return fail(QSS_E_HIP);
+  }
+  const long nc = c->ncell;
+  for (int i = 0; i < 2; ++i) {
+    const int n = i == 0 ? c->ny : c->nz;
+    if ((r = dalloc(&c->d_T[2 * i], (size_t)n * n)) != QSS_OK) return fail(r);
+    if ((r = dalloc(&c->d_T[2 * i + 1], (size_t)n * n)) != QSS_OK) return fail(r);
+    std::vector<double> tc, ts;
+    make_tables(n, tc, ts);
+    if (hipMemcpy(c->d_T[2 * i], tc.data(), tc.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_T[2 * i + 1], ts.data(), ts.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+      qss_set_error("table upload failed");
+      return fail(QSS_E_HIP);
+    }
+  }
+  for (int f = 0; f < QSS_NFIELD; ++f) {
+    if ((r = dalloc(&c->C[f], nc)) != QSS_OK) return fail(r);
+    if ((r = dalloc(&c->P[f], nc)) != QSS_OK) return fail(r);
+  }
+  for (int s = 0; s < SP_N; ++s)
+    if ((r = dalloc(&c->S[s], nc)) != QSS_OK) return fail(r);
+  for (int b = 0; b < kGemmMaxBatch; ++b)
+    if ((r = dalloc(&c->tmp[b], nc)) != QSS_OK) return fail(r);
+  if ((r = dalloc(&c->F, nc * kPushStride)) != QSS_OK) return fail(r);
+  if (!c->bins.empty()) {
+    if ((r = dalloc(&c->d_bins, c->bins.size())) != QSS_OK) return fail(r);
+    if (hipMemcpy(c->d_bins, c->bins.data(), c->bins.size() * 8, hipMemcpyHostToDevice) !=
+        hipSuccess) {
+      qss_set_error("bins upload failed");
+      return fail(QSS_E_HIP);
+    }
+  }
+  const long np = c->np_cap, ov = c->ov_cap;
+  for (int b = 0; b < 2; ++b) {
+    if ((r = dalloc(&c->py_[b], np)) || (r = dalloc(&c->pz_[b], np)) ||
+        (r = dalloc(&c->ppx[b], np)) || (r = dalloc(&c->ppy[b], np)) ||
+        (r = dalloc(&c->ppz[b], np)) || (r = dalloc(&c->pcell[b], np)) ||
+        (r = dalloc(&c->pid[b], np)))
+      return fail(r);
+  }
+  if ((r = dalloc(&c->ev_cell, np)) || (r = dalloc(&c->ev_rank, np)) ||
+      (r = dalloc(&c->ev_sub, np)) || (r = dalloc(&c->ev_y, np)) ||
+      (r = dalloc(&c->ev_z, np)) || (r = dalloc(&c->ev_w, np * 4)) ||
+      (r = dalloc(&c->ev_p, np * 3)) || (r = dalloc(&c->ov_cell, ov)) ||
+      (r = dalloc(&c->ov_rank, ov)) || (r = dalloc(&c->ov_y, ov)) ||
+      (r = dalloc(&c->ov_z, ov)) || (r = dalloc(&c->ov_w, ov * 4)) ||
+      (r = dalloc(&c->ov_key, ov)) || (r = dalloc(&c->d_counters, 2)) ||
+      (r = dalloc(&c->cnt, nc)) || (r = dalloc(&c->cnt_main, nc)) ||
+      (r = dalloc(&c->off, nc)) || (r = dalloc(&c->off_main, nc)) ||
+      (r = dalloc(&c->blk, c->nblk)) || (r = dalloc(&c->blk_main, c->nblk)) ||
+      (r = dalloc(&c->list, np + ov)) || (r = dalloc(&c->d_resid, 6)) ||
+      (r = dalloc(&c->d_res_hist, (size_t)p->nslices * p->nx_split)))
+    return fail(r);
+  c->hmask = history_mask;
+  for (int f = 0; f < QSS_NFIELD; ++f)
+    if (history_mask & (1u << f))
+      if ((r = dalloc(&c->hist[f], (size_t)nc * p->nslices)) != QSS_OK) return fail(r);
+  if ((r = do_reset(c)) != QSS_OK) return fail(r);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    qss_set_error("seeding failed");
+    return fail(QSS_E_HIP);
+  }
+  *out = c;
+  return QSS_OK;
+}
+
+int qss_destroy(qss_ctx *c) {
+  if (!c) return QSS_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &p : c->d_T) dfree(p);
+  for (int f = 0; f < QSS_NFIELD; ++f) { dfree(c->C[f]); dfree(c->P[f]); dfree(c->hist[f]); }
+  for (auto &p : c->S) dfree(p);
+  for (auto &p : c->tmp) dfree(p);
+  dfree(c->F); dfree(c->d_bins);
+  for (int b = 0; b < 2; ++b) {
+    dfree(c->py_[b]); dfree(c->pz_[b]); dfree(c->ppx[b]); dfree(c->ppy[b]);
+    dfree(c->ppz[b]); dfree(c->pcell[b]); dfree(c->pid[b]);
+  }
+  dfree(c->ev_cell); dfree(c->ev_rank); dfree(c->ev_sub); dfree(c->ev_y); dfree(c->ev_z);
+  dfree(c->ev_w); dfree(c->ev_p); dfree(c->ov_cell); dfree(c->ov_rank); dfree(c->ov_y);
+  dfree(c->ov_z); dfree(c->ov_w); dfree(c->ov_key); dfree(c->d_counters);
+  dfree(c->cnt); dfree(c->cnt_main); dfree(c->off); dfree(c->off_main);
+  dfree(c->blk); dfree(c->blk_main); dfree(c->list); dfree(c->d_resid);
+  dfree(c->d_res_hist);
+  for (auto &pool : c->ev_pool)
+    for (auto &pr : pool) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+  if (c->t0) (void)hipEventDestroy(c->t0);
+  if (c->t1) (void)hipEventDestroy(c->t1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return QSS_OK;
+}
+
+int qss_info(const qss_ctx *c, int64_t *n_particles, int64_t *n_cells) {
+  if (!c) return QSS_E_ARG;
+  if (n_particles) *n_particles = c->np;
+  if (n_cells) *n_cells = c->ncell;
+  return QSS_OK;
+}
+
+void *qss_stream(qss_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int qss_reset(qss_ctx *c) {
+  if (!c) return QSS_E_ARG;
+  HIPCHECK(hipSetDevice(c->device));
+  CHECK(do_reset(c));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return QSS_OK;
+}
+
+int qss_sweep_range(qss_ctx *c, int first, int count, double *residuals) {
+  if (!c || first < 0 || count < 0 || first + count > c->p.nslices) {
+    qss_set_error("bad sweep range");
+    return QSS_E_ARG;
+  }
+  HIPCHECK(hipSetDevice(c->device));
+  for (auto &u : c->ev_used) u = 0;
+  c->gemm_flops = 0;
+  if (!c->t0) { HIPCHECK(hipEventCreate(&c->t0)); HIPCHECK(hipEventCreate(&c->t1)); }
+  HIPCHECK(hipEventRecord(c->t0, c->stream));
+  const int S = c->p.nx_split;
+  for (int s = first; s < first + count; ++s) {
+    for (int is = 0; is < S; ++is)
+      CHECK(do_layer_step(c, s, is, c->d_res_hist + (size_t)s * S + is));
+    // the finished layer: movepart_splitlayer.cpp:467-473
+    for (int f = 0; f < QSS_NFIELD; ++f)
+      if (c->hist[f])
+        HIPCHECK(hipMemcpyAsync(c->hist[f] + (size_t)s * c->ncell, c->P[f],
+                                c->ncell * sizeof(double), hipMemcpyDeviceToDevice,
+                                c->stream));
+  }
+  HIPCHECK(hipEventRecord(c->t1, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  std::vector<double> res((size_t)count * S);
+  if (count > 0)
+    HIPCHECK(hipMemcpy(res.data(), c->d_res_hist + (size_t)first * S,
+                       res.size() * sizeof(double), hipMemcpyDeviceToHost));
+  if (residuals) std::copy(res.begin(), res.end(), residuals);
+  CHECK(check_status(c));
+  for (double v : res)
+    if (!std::isfinite(v)) { qss_set_error("non-finite residual"); return QSS_E_NONFINITE; }
+  return QSS_OK;
+}
+
+int qss_sweep(qss_ctx *c, double *residuals) {
+  if (!c) return QSS_E_ARG;
+  HIPCHECK(hipSetDevice(c->device));
+  CHECK(do_reset(c));
+  return qss_sweep_range(c, 0, c->p.nslices, residuals);
+}
+
+int qss_layer_step(qss_ctx *c, int slice, int isplit, double *residual) {
+  if (!c || slice < 0 || slice >= c->p.nslices || isplit < 0 || isplit >= c->p.nx_split) {
+    qss_set_error("bad slice / split");
+    return QSS_E_ARG;
+  }
+  HIPCHECK(hipSetDevice(c->device));
+  double *dst = c->d_res_hist + (size_t)slice * c->p.nx_split + isplit;
+  CHECK(do_layer_step(c, slice, isplit, dst));
+  double r = 0;
+  HIPCHECK(hipMemcpyAsync(&r, dst, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  if (residual) *residual = r;
+  CHECK(check_status(c));
+  if (!std::isfinite(r)) { qss_set_error("non-finite residual"); return QSS_E_NONFINITE; }
+  return QSS_OK;
+}
+
+int qss_beam_layer(qss_ctx *c, int slice, int isplit) {
+  if (!c || slice < 0 || slice >= c->p.nslices || isplit < 0 || isplit >= c->p.nx_split)
+    return QSS_E_ARG;
+  HIPCHECK(hipSetDevice(c->device));
+  CHECK(do_beam(c, slice, isplit));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return QSS_OK;
+}
+
+int qss_guess(qss_ctx *c, int first_layer) {
+  if (!c) return QSS_E_ARG;
+  HIPCHECK(hipSetDevice(c->device));
+  CHECK(do_guess(c, first_layer));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->push_dirty = false;
+  return QSS_OK;
+}
+
+int qss_push_deposit(qss_ctx *c, int full_step) {
+  if (!c) return QSS_E_ARG;
+  HIPCHECK(hipSetDevice(c->device));
+  if (c->push_dirty) {
+    CHECK(refresh_push_field(c));
+    c->push_dirty = false;
+  }
+  CHECK(do_push_deposit(c, full_step));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return check_status(c);
+}
+
+int qss_iterate(qss_ctx *c, int final_pass, double *residual) {
+  if (!c) return QSS_E_ARG;
+  HIPCHECK(hipSetDevice(c->device));
+  double *dst = c->d_res_hist;
+  CHECK(do_iterate(c, final_pass, dst));
+  double r = 0;
+  HIPCHECK(hipMemcpyAsync(&r, dst, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  if (residual) *residual = r;
+  c->push_dirty = false;
+  return QSS_OK;
+}
+
+int qss_copy_c_to_p(qss_ctx *c) {
+  if (!c) return QSS_E_ARG;
+  swap_c_p(c);
+  c->push_dirty = true;
+  return QSS_OK;
+}
+
+int qss_transform2d(qss_ctx *c, int kind_z, int kind_y, const double *in_dev,
+                    double *out_dev, double scale) {
+  return qss_transform2d_batched(c, 1, &kind_z, &kind_y, &in_dev, &out_dev, scale);
+}
+
+int qss_transform2d_batched(qss_ctx *c, int n, const int *kind_z, const int *kind_y,
+                            const double *const *in_dev, double *const *out_dev,
+                            double scale) {
+  if (!c || !kind_z || !kind_y || !in_dev || !out_dev || n < 1 || n > kGemmMaxBatch)
+    return QSS_E_ARG;
+  for (int b = 0; b < n; ++b)
+    if ((kind_z[b] != 0 && kind_z[b] != 1) || (kind_y[b] != 0 && kind_y[b] != 1) ||
+        !in_dev[b] || !out_dev[b])
+      return QSS_E_ARG;
+  HIPCHECK(hipSetDevice(c->device));
+  CHECK(transform_batch(c, n, kind_z, kind_y, in_dev, out_dev, scale, nullptr, nullptr,
+                        nullptr));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return QSS_OK;
+}
+
+int qss_get_layer_field(qss_ctx *c, int which, int field, double *host) {
+  if (!c || !host || field < 0 || field >= QSS_NFIELD || (which != 0 && which != 1))
+    return QSS_E_ARG;
+  HIPCHECK(hipSetDevice(c->device));
+  double *src = which ? c->P[field] : c->C[field];
+  HIPCHECK(hipMemcpyAsync(host, src, c->ncell * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return QSS_OK;
+}
+
+int qss_set_layer_field(qss_ctx *c, int which, int field, const double *host) {
+  if (!c || !host || field < 0 || field >= QSS_NFIELD || (which != 0 && which != 1))
+    return QSS_E_ARG;
+  HIPCHECK(hipSetDevice(c->device));
+  double *dst = which ? c->P[field] : c->C[field];
+  HIPCHECK(hipMemcpyAsync(dst, host, c->ncell * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  if (field <= QSS_BZ) c->push_dirty = true;
+  return QSS_OK;
+}
+
+int qss_get_history(qss_ctx *c, int field, int slice, double *host) {
+  if (!c || !host || field < 0 || field >= QSS_NFIELD || !c->hist[field] || slice < 0 ||
+      slice >= c->p.nslices)
+    return QSS_E_ARG;
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipMemcpy(host, c->hist[field] + (size_t)slice * c->ncell,
+                     c->ncell * sizeof(double), hipMemcpyDeviceToHost));
+  return QSS_OK;
+}
+
+int qss_history_ptr(qss_ctx *c, int field, double **dev) {
+  if (!c || !dev || field < 0 || field >= QSS_NFIELD || !c->hist[field]) return QSS_E_ARG;
+  *dev = c->hist[field];
+  return QSS_OK;
+}
+
+int qss_get_spectral(qss_ctx *c, int id, double *host) {
+  if (!c || !host || id < 0 || id >= QSS_NSPECTRAL) return QSS_E_ARG;
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipMemcpyAsync(host, c->S[id], c->ncell * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return QSS_OK;
+}
+
+int qss_set_spectral(qss_ctx *c, int id, const double *host) {
+  if (!c || !host || id < 0 || id >= QSS_NSPECTRAL) return QSS_E_ARG;
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipMemcpyAsync(c->S[id], host, c->ncell * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return QSS_OK;
+}
+
+int qss_get_particles(qss_ctx *c, int32_t *cell_j, int32_t *cell_k, double *fy, double *fz,
+                      double *px, double *py, double *pz, int64_t *id) {
+  if (!c) return QSS_E_ARG;
+  HIPCHECK(hipSetDevice(c->device));
+  const int b = c->cur;
+  const size_t n = (size_t)c->np;
+  auto get = [&](void *dst, const void *src, size_t bytes) -> int {
+    if (!dst) return QSS_OK;
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    return QSS_OK;
+  };
+  std::vector<int32_t> cell(n);
+  CHECK(get(cell.data(), c->pcell[b], n * 4));
+  CHECK(get(fy, c->py_[b], n * 8));
+  CHECK(get(fz, c->pz_[b], n * 8));
+  CHECK(get(px, c->ppx[b], n * 8));
+  CHECK(get(py, c->ppy[b], n * 8));
+  CHECK(get(pz, c->ppz[b], n * 8));
+  CHECK(get(id, c->pid[b], n * 8));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < n; ++i) {
+    if (cell_j) cell_j[i] = cell[i] % c->ny;
+    if (cell_k) cell_k[i] = cell[i] / c->ny;
+  }
+  return QSS_OK;
+}
+
+int qss_set_particles(qss_ctx *c, int64_t n, const int32_t *cell_j, const int32_t *cell_k,
+                      const double *fy, const double *fz, const double *px, const double *py,
+                      const double *pz, const int64_t *id) {
+  if (!c || n < 0 || n > c->np_cap || !cell_j || !cell_k || !fy || !fz || !px || !py ||
+      !pz || !id)
+    return QSS_E_ARG;
+  std::vector<int32_t> cell((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    if (cell_j[i] < 0 || cell_j[i] >= c->ny || cell_k[i] < 0 || cell_k[i] >= c->nz) {
+      qss_set_error("particle cell out of range");
+      return QSS_E_ARG;
+    }
+    cell[i] = cell_j[i] + c->ny * cell_k[i];
+  }
+  HIPCHECK(hipSetDevice(c->device));
+  const int b = c->cur;
+  HIPCHECK(hipMemcpy(c->pcell[b], cell.data(), n * 4, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->py_[b], fy, n * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->pz_[b], fz, n * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->ppx[b], px, n * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->ppy[b], py, n * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->ppz[b], pz, n * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->pid[b], id, n * 8, hipMemcpyHostToDevice));
+  c->np = n;
+  return QSS_OK;
+}
+
+int qss_enable_timing(qss_ctx *c, int on) {
+  if (!c) return QSS_E_ARG;
+  c->timing = on != 0;
+  return QSS_OK;
+}
+
+int qss_timing(qss_ctx *c, double *ms_total, double *ms_kernel, int64_t *launches,
+               double *gemm_flops) {
+  if (!c) return QSS_E_ARG;
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  if (c->t0 && c->t1) HIPCHECK(hipEventElapsedTime(&ms, c->t0, c->t1));
+  if (ms_total) *ms_total = ms;
+  for (int k = 0; k < kNClass; ++k) {
+    double sum = 0;
+    for (size_t i = 0; i < c->ev_used[k]; ++i) {
+      float e = 0.f;
+      HIPCHECK(hipEventElapsedTime(&e, c->ev_pool[k][i].first, c->ev_pool[k][i].second));
+      sum += e;
+    }
+    if (ms_kernel) ms_kernel[k] = sum;
+    if (launches) launches[k] = (int64_t)c->ev_used[k];
+  }
+  if (gemm_flops) *gemm_flops = c->gemm_flops;
+  return QSS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,122 @@
+// qss_internal.h — device-side layout and kernel launchers of libqss.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/qss.h"
+
+// spectral state ids (fft_of_* arrays, movefieldshydrolinlayersplit.cpp:267-313)
+enum {
+  SP_RHO = 0, SP_EX, SP_EY, SP_EZ, SP_BX, SP_BY, SP_BZ, SP_JX, SP_JY, SP_JZ,
+  SP_RHOP, SP_JXP, SP_JYP, SP_JZP, SP_EXP, SP_EYP, SP_EZP,
+  SP_RHOB, SP_JXB, SP_RHOBP, SP_JXBP, SP_PSI, SP_N
+};
+
+// interleaved push field: (P+C)/2 of Ex,Ey,Ez,Bx,By,Bz per node, padded to 8
+constexpr int kPushStride = 8;
+
+// ---------------------------------------------------------------------------
+// transform GEMM (tertia_amd/csrc/gemm.hip)
+// C_b = alpha * A_b . B_b, all row-major, M x K . K x N, per-batch epilogue
+// ---------------------------------------------------------------------------
+constexpr int kGemmMaxBatch = 8;
+struct GemmBatch {
+  const double *A;
+  const double *B;
+  double *C;
+  const double *P;   // P-layer plane for the push-field epilogue (or null)
+  int resid;         // -1, or component 0..2: residual max|new-old|, max|new|
+  int push;          // -1, or component 0..5 of the interleaved push field
+};
+struct GemmArgs {
+  int M, N, K;
+  double alpha;
+  int nbatch;
+  double *F;                    // interleaved push field (kPushStride doubles/node)
+  unsigned long long *resid;    // 6 slots: dif[3], max[3] as IEEE bit patterns
+  GemmBatch b[kGemmMaxBatch];
+};
+hipError_t launch_gemm_f64(const GemmArgs &a, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct qss_ctx {
+  qss_params p{};
+  std::vector<double> bins;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int ny = 0, nz = 0;
+  long ncell = 0;
+  long np = 0;            // plasma particles in the layer
+  long np_cap = 0;        // particle capacity (ny*nz*ppc)
+  bool push_dirty = false;   // C/P E,B edited from the host: rebuild F
+  double gemm_flops = 0;     // algorithmic flops of timed GEMM launches
+  long ov_cap = 0;        // capacity of the sub-cycle overflow event buffer
+  double hxs = 0, hy = 0, hz = 0, pweight = 0;
+
+  double *d_T[4] = {};    // Cy, Sy, Cz, Sz transform matrices
+  double *C[QSS_NFIELD] = {};
+  double *P[QSS_NFIELD] = {};
+  double *S[SP_N] = {};
+  double *tmp[kGemmMaxBatch] = {};   // transform intermediates
+  double *F = nullptr;               // push field
+  double *d_bins = nullptr;
+
+  // particles (SoA), ping-pong for the full-step reorder
+  double *py_[2] = {}, *pz_[2] = {}, *ppx[2] = {}, *ppy[2] = {}, *ppz[2] = {};
+  int32_t *pcell[2] = {};
+  int64_t *pid[2] = {};
+  int cur = 0;
+
+  // deposit events: main (one per particle) + overflow (extra sub-steps)
+  int32_t *ev_cell = nullptr, *ev_rank = nullptr, *ev_sub = nullptr;
+  double *ev_y = nullptr, *ev_z = nullptr, *ev_w = nullptr, *ev_p = nullptr;
+  int32_t *ov_cell = nullptr, *ov_rank = nullptr;
+  double *ov_y = nullptr, *ov_z = nullptr, *ov_w = nullptr;
+  uint64_t *ov_key = nullptr;
+  int32_t *d_counters = nullptr;   // [0] overflow count, [1] status flags
+  // counting sort by target cell
+  int32_t *cnt = nullptr, *cnt_main = nullptr;
+  int32_t *off = nullptr, *off_main = nullptr;
+  int32_t *blk = nullptr, *blk_main = nullptr;
+  int nblk = 0;
+  int32_t *list = nullptr;
+
+  unsigned long long *d_resid = nullptr;   // 6 slots per pass
+  double *d_res_hist = nullptr;            // residual of each layer step
+  int res_count = 0;
+
+  uint32_t hmask = 0;
+  double *hist[QSS_NFIELD] = {};
+
+  // timing
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool[QSS_NKCLASS];
+  size_t ev_used[QSS_NKCLASS] = {};
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+};
+
+// kernel classes for timing (QSS_NKCLASS of them)
+enum { KC_PUSH = 0, KC_GEMM = 1, KC_SPEC = 2, KC_BEAM = 3, KC_SORT = 4, KC_REDUCE = 5 };
+
+// launchers (tertia_amd/csrc/*.hip); all enqueue on c->stream
+hipError_t launch_fill_i32(int32_t *p, long n, int32_t v, hipStream_t s);
+hipError_t launch_fill_f64(double *p, long n, double v, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long *p, long n, unsigned long long v, hipStream_t s);
+hipError_t launch_seed_plasma(qss_ctx *c);
+hipError_t launch_beam_plane(qss_ctx *c, int slice, int isplit);
+hipError_t launch_guess_update(qss_ctx *c, int first_layer);
+hipError_t launch_iterate_update(qss_ctx *c, int final_pass);
+hipError_t launch_push(qss_ctx *c, int full_step);
+hipError_t launch_sort_scan(qss_ctx *c, int full_step);
+hipError_t launch_scatter(qss_ctx *c);
+hipError_t launch_tile_reduce(qss_ctx *c, int full_step);
+hipError_t launch_residual_final(qss_ctx *c, double *dst);
+hipError_t launch_push_field(qss_ctx *c);
+
+// host helpers
+void qss_set_error(const std::string &m);

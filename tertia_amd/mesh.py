@@ -1,0 +1,242 @@
+"""Host-side mirror of the reference ``Mesh`` layer-step interface.
+
+The reference drives the quasi-static sweep through member functions of its
+single ``Mesh`` (mesh.h:51-84).  This class keeps those names and argument
+meanings so that a caller of the reference finds the same call sequence, but
+every call runs on the GPU through ``libqss.so``:
+
+=============================================  =====================================
+reference (file:line)                          here
+=============================================  =====================================
+Mesh::MoveAllSplitLayers   movepart_splitlayer.cpp:25    ``MoveAllSplitLayers``
+Mesh::MoveSplitLayer       movepart_splitlayer.cpp:561   ``MoveSplitLayer``
+beam C/P interpolation     movepart_splitlayer.cpp:330   ``InterpolateBeamLayers``
+GuessFieldsHydroLinLayerSplit  movefieldshydrolinlayersplit.cpp:222  ``GuessFieldsHydroLinLayerSplit``
+IterateFieldsHydroLinLayerSplit :1173 (returns Eerr)     ``IterateFieldsHydroLinLayerSplit``
+MoveParticlesLayerSplit    movepart_splitlayer.cpp:781   ``MoveParticlesLayerSplit``
+ClearCurrentsSplit/ClearRhoSplit mesh.cpp:692-708        fused into the deposit
+ExchangeCurrentsSplit/ExchangeFieldsSplit :1659-1711    fused (periodic wrap)
+=============================================  =====================================
+
+Layer indices follow the reference: ``iLayer`` runs from ``l_Mx-1`` (first
+layer behind the front) down to 0; ``slice = l_Mx-1-iLayer``.  Errors raise
+:class:`QssError` where the reference calls ``exit()``.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Iterable, Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import QssError, check
+from .params import FIELD_ID, SPECTRAL_ID, Params
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Mesh:
+    """One quasi-static wake on one GPU (a ``qss_ctx``)."""
+
+    def __init__(self, params: Params, device: int = 0,
+                 history: Optional[Iterable[str]] = ("Ex", "Psi")):
+        self.params = params
+        self._c = params.to_c()
+        mask = 0
+        for f in history or ():
+            mask |= 1 << FIELD_ID[f]
+        self.L = _lib.load()
+        h = ctypes.c_void_p()
+        check(self.L.qss_create(ctypes.byref(self._c), device, mask, ctypes.byref(h)),
+              "qss_create")
+        self.h = h
+        self.ny, self.nz = params.ny, params.nz
+        self.l_Mx = params.nslices
+
+    # -- lifetime --------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.qss_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def n_particles(self) -> int:
+        n = ctypes.c_int64()
+        check(self.L.qss_info(self.h, ctypes.byref(n), None), "qss_info")
+        return n.value
+
+    @property
+    def stream(self) -> int:
+        return self.L.qss_stream(self.h)
+
+    def _slice(self, iLayer: int) -> int:
+        s = self.l_Mx - 1 - int(iLayer)
+        if not 0 <= s < self.l_Mx:
+            raise QssError(-1, "layer index", "iLayer %d outside [0, %d)" % (iLayer, self.l_Mx))
+        return s
+
+    # -- the sweep -------------------------------------------------------
+    def MoveAllSplitLayers(self) -> np.ndarray:
+        """Seed the front layer, sweep every layer; returns the final-pass
+        residual of every layer step (movepart_splitlayer.cpp:25-558)."""
+        res = np.zeros(self.l_Mx * self.params.nx_split)
+        check(self.L.qss_sweep(self.h, _p(res)), "qss_sweep")
+        return res
+
+    def sweep_range(self, first_slice: int, count: int) -> np.ndarray:
+        res = np.zeros(count * self.params.nx_split)
+        check(self.L.qss_sweep_range(self.h, first_slice, count, _p(res)), "qss_sweep_range")
+        return res
+
+    def reset(self):
+        check(self.L.qss_reset(self.h), "qss_reset")
+
+    def MoveSplitLayer(self, iLayer: int, iSplit: int = 0) -> float:
+        """One layer step (movepart_splitlayer.cpp:561-778), beam interpolation
+        and the closing C->P copy included; returns the final Eerr."""
+        r = ctypes.c_double()
+        check(self.L.qss_layer_step(self.h, self._slice(iLayer), iSplit, ctypes.byref(r)),
+              "qss_layer_step")
+        return r.value
+
+    # -- pieces of a layer step -------------------------------------------
+    def InterpolateBeamLayers(self, iLayer: int, iSplit: int = 0):
+        check(self.L.qss_beam_layer(self.h, self._slice(iLayer), iSplit), "qss_beam_layer")
+
+    def GuessFieldsHydroLinLayerSplit(self, iLayer: int, iSplit: int = 0):
+        check(self.L.qss_guess(self.h, int(iLayer == self.l_Mx - 1)), "qss_guess")
+
+    def MoveParticlesLayerSplit(self, iLayer: int, iSplit: int, iFullStep: int,
+                                part: float = 1.0):
+        if part != 1.0:
+            raise QssError(-1, "MoveParticlesLayerSplit", "only part=1 is used by the sweep")
+        check(self.L.qss_push_deposit(self.h, int(iFullStep)), "qss_push_deposit")
+
+    def IterateFieldsHydroLinLayerSplit(self, iLayer: int, iSplit: int, N_iter: int) -> float:
+        r = ctypes.c_double()
+        check(self.L.qss_iterate(self.h, int(N_iter < 0), ctypes.byref(r)), "qss_iterate")
+        return r.value
+
+    def ClearCurrentsSplit(self):
+        """Fused into MoveParticlesLayerSplit (mesh.cpp:692-700)."""
+
+    def ClearRhoSplit(self):
+        """Fused into MoveParticlesLayerSplit (mesh.cpp:702-708)."""
+
+    def ExchangeCurrentsSplit(self, iLayer: int = 0):
+        """Periodic ghost fold, fused into the deposit (movepart_splitlayer.cpp:1685)."""
+
+    def ExchangeFieldsSplit(self, iLayer: int = 0):
+        """Periodic ghost copy, fused into the gather (movepart_splitlayer.cpp:1659)."""
+
+    def CopyCtoP(self):
+        check(self.L.qss_copy_c_to_p(self.h), "qss_copy_c_to_p")
+
+    # -- data --------------------------------------------------------------
+    def _plane(self):
+        return np.zeros((self.nz, self.ny), dtype=np.float64)
+
+    def get_field(self, name: str, layer: str = "C") -> np.ndarray:
+        out = self._plane()
+        check(self.L.qss_get_layer_field(self.h, 0 if layer == "C" else 1, FIELD_ID[name],
+                                         _p(out)), "qss_get_layer_field")
+        return out
+
+    def set_field(self, name: str, arr, layer: str = "C"):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.nz, self.ny)
+        check(self.L.qss_set_layer_field(self.h, 0 if layer == "C" else 1, FIELD_ID[name],
+                                         _p(a)), "qss_set_layer_field")
+
+    def get_spectral(self, name: str) -> np.ndarray:
+        out = self._plane()
+        check(self.L.qss_get_spectral(self.h, SPECTRAL_ID[name], _p(out)), "qss_get_spectral")
+        return out
+
+    def set_spectral(self, name: str, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.nz, self.ny)
+        check(self.L.qss_set_spectral(self.h, SPECTRAL_ID[name], _p(a)), "qss_set_spectral")
+
+    def history(self, name: str, slice_: int) -> np.ndarray:
+        out = self._plane()
+        check(self.L.qss_get_history(self.h, FIELD_ID[name], slice_, _p(out)),
+              "qss_get_history")
+        return out
+
+    def history_ptr(self, name: str) -> int:
+        p = ctypes.c_void_p()
+        check(self.L.qss_history_ptr(self.h, FIELD_ID[name], ctypes.byref(p)),
+              "qss_history_ptr")
+        return p.value
+
+    def particles(self) -> dict:
+        n = self.n_particles
+        d = {k: np.zeros(n) for k in ("fy", "fz", "px", "py", "pz")}
+        d["cj"] = np.zeros(n, dtype=np.int32)
+        d["ck"] = np.zeros(n, dtype=np.int32)
+        d["id"] = np.zeros(n, dtype=np.int64)
+        check(self.L.qss_get_particles(self.h, d["cj"].ctypes.data_as(_i32p),
+                                       d["ck"].ctypes.data_as(_i32p), _p(d["fy"]),
+                                       _p(d["fz"]), _p(d["px"]), _p(d["py"]), _p(d["pz"]),
+                                       d["id"].ctypes.data_as(_i64p)), "qss_get_particles")
+        d["y"] = d["cj"] + d["fy"]
+        d["z"] = d["ck"] + d["fz"]
+        return d
+
+    def set_particles(self, cj, ck, fy, fz, px, py, pz, pid):
+        ii = [np.ascontiguousarray(a, dtype=np.int32) for a in (cj, ck)]
+        ff = [np.ascontiguousarray(a, dtype=np.float64) for a in (fy, fz, px, py, pz)]
+        pid = np.ascontiguousarray(pid, dtype=np.int64)
+        check(self.L.qss_set_particles(self.h, len(pid), ii[0].ctypes.data_as(_i32p),
+                                       ii[1].ctypes.data_as(_i32p), *[_p(a) for a in ff],
+                                       pid.ctypes.data_as(_i64p)), "qss_set_particles")
+
+    def transform2d(self, x, kind_z: int, kind_y: int, scale: float = 1.0):
+        """scale * T_kz . x . T_ky on the GPU (CUDA_WRAP_fourierHalfInteger2D,
+        half_integer2D.cu:583).  ``x``: (nz, ny) numpy array or CUDA tensor;
+        returns the same kind.  Device memory comes from torch."""
+        import torch
+        as_numpy = isinstance(x, np.ndarray)
+        xt = torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64)) if as_numpy else x
+        xt = xt.to(device="cuda", dtype=torch.float64).contiguous()
+        if tuple(xt.shape) != (self.nz, self.ny):
+            raise QssError(-1, "transform2d", "plane shape must be (nz, ny)")
+        out = torch.empty_like(xt)
+        torch.cuda.synchronize()
+        check(self.L.qss_transform2d(self.h, int(kind_z), int(kind_y), xt.data_ptr(),
+                                     out.data_ptr(), float(scale)), "qss_transform2d")
+        return out.cpu().numpy() if as_numpy else out
+
+    # -- timing ------------------------------------------------------------
+    def enable_timing(self, on: bool = True):
+        check(self.L.qss_enable_timing(self.h, int(on)), "qss_enable_timing")
+
+    def timing(self) -> dict:
+        tot = ctypes.c_double()
+        ms = (ctypes.c_double * 6)()
+        n = (ctypes.c_int64 * 6)()
+        fl = ctypes.c_double()
+        check(self.L.qss_timing(self.h, ctypes.byref(tot), ms, n, ctypes.byref(fl)),
+              "qss_timing")
+        names = ["push", "gemm", "spectral", "beam", "sort", "reduce"]
+        return {"ms_total": tot.value, "gemm_flops": fl.value,
+                "ms": {k: ms[i] for i, k in enumerate(names)},
+                "launches": {k: n[i] for i, k in enumerate(names)}}

@@ -1,0 +1,169 @@
+"""GPU parity of the pieces of one layer step against the oracle, on identical
+seeded state: beam deposit, Guess, Iterate, push + deposit.
+
+Tolerances (FP64 both sides):
+  * beam planes: max-rel 1e-14 (same sums, same order; exp may differ by 1 ulp)
+  * Guess/Iterate: max-rel 1e-12 per spectral/real plane (inputs differ only
+    by transform rounding, ~1e-15)
+  * push + deposit: max-rel 1e-14 on J, rho and the pushed particles (same
+    arithmetic in the same order, FMA contraction off on both sides)
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import max_rel, random_particles, smooth_field
+from tertia_amd.params import Bunch, Params, config0, small
+
+pytestmark = pytest.mark.gpu
+
+SPEC_IN = {"Rho": (0, 0), "Ex": (0, 0), "Ey": (0, 1), "Ez": (1, 0), "Jx": (0, 0),
+           "Jy": (0, 1), "Jz": (1, 0)}
+EB = ["Ex", "Ey", "Ez", "Bx", "By", "Bz"]
+
+
+def make_pair(oracle_mod, p):
+    from tertia_amd.mesh import Mesh
+    return Mesh(p, history=()), oracle_mod.Oracle(p, history=())
+
+
+@pytest.mark.parametrize("cfg", ["c0", "two_bunch_noise", "split3", "bins"])
+def test_beam_layer(oracle_mod, cfg):
+    if cfg == "c0":
+        p = config0(nslices=60)
+        slices = [0, 20, 35, 40, 59]
+    elif cfg == "two_bunch_noise":
+        p = small(ny=48, nz=40, nslices=30,
+                  bunches=[Bunch(nb=4, sigma_r=0.5, sigma_xi=0.3, xi0=0.5, noise=0.02),
+                           Bunch(nb=8, sigma_r=0.3, sigma_xi=0.1, xi0=1.0, y0=0.2)])
+        slices = [0, 9, 10, 19, 20, 29]
+    elif cfg == "split3":
+        p = small(nslices=20, nx_split=3)
+        slices = [0, 5, 11, 12]
+    else:
+        bins = np.random.default_rng(1).uniform(0, 3, size=17)
+        p = small(nslices=30, bins=bins, bin_xi0=0.1, bin_xi1=1.2)
+        slices = [0, 3, 10, 22]
+    m, o = make_pair(oracle_mod, p)
+    for s in slices:
+        for isplit in range(p.nx_split):
+            m.InterpolateBeamLayers(p.nslices - 1 - s, isplit)
+            o.beam_layer(s, isplit)
+            for f in ("RhoBeam", "JxBeam"):
+                ref = o.get_field(f)
+                got = m.get_field(f)
+                assert max_rel(got, ref) <= 1e-14, (cfg, s, isplit, f)
+    m.close()
+
+
+def _state(rng, p):
+    ny, nz = p.ny, p.nz
+    st = {}
+    for f in ("Rho", "Ex", "Ey", "Ez", "Jx", "Jy", "Jz", "Bx", "By", "Bz"):
+        st["P" + f] = smooth_field(rng, nz, ny, 0.3)
+    for f in ("RhoBeam", "JxBeam"):
+        st["C" + f] = smooth_field(rng, nz, ny, 0.5)
+    st["S_RhoBeam"] = smooth_field(rng, nz, ny, 20.0)
+    st["S_JxBeam"] = smooth_field(rng, nz, ny, 20.0)
+    return st
+
+
+def _load_guess_state(m, o, O, st):
+    for f in ("Rho", "Ex", "Ey", "Ez", "Jx", "Jy", "Jz", "Bx", "By", "Bz"):
+        o.set_field(f, st["P" + f], which=1)
+        m.set_field(f, st["P" + f], layer="P")
+    for f in ("RhoBeam", "JxBeam"):
+        o.set_field(f, st["C" + f])
+        m.set_field(f, st["C" + f], layer="C")
+    for f in ("RhoBeam", "JxBeam"):
+        o.set_spectral(f, st["S_" + f])
+        m.set_spectral(f, st["S_" + f])
+    # the GPU keeps P spectra; the oracle re-transforms the P layer
+    for f, (kz, ky) in SPEC_IN.items():
+        m.set_spectral(f, O.transform2d(st["P" + f], kz, ky))
+
+
+@pytest.mark.parametrize("nz,ny", [(32, 32), (40, 24)])
+def test_guess_and_iterate(oracle_mod, nz, ny):
+    rng = np.random.default_rng(nz + 7 * ny)
+    p = small(ny=ny, nz=nz, nslices=4)
+    m, o = make_pair(oracle_mod, p)
+    st = _state(rng, p)
+    _load_guess_state(m, o, oracle_mod, st)
+    o.guess(False)
+    m.GuessFieldsHydroLinLayerSplit(p.nslices - 2)
+    for f in ("Rho", "Ex", "Ey", "Ez", "Jx", "Jy", "Jz", "Bx", "By", "Bz", "RhoP", "ExP",
+              "JxP", "RhoBeam", "JxBeam", "RhoBeamP", "JxBeamP"):
+        assert max_rel(m.get_spectral(f), o.get_spectral(f)) < 1e-12, ("guess spectral", f)
+    for f in EB:
+        assert max_rel(m.get_field(f), o.get_field(f)) < 1e-12, ("guess C", f)
+    # corrector on a fresh deposit
+    for it, final in ((0, False), (1, True)):
+        for f in ("Jx", "Jy", "Jz"):
+            J = smooth_field(rng, nz, ny, 0.2)
+            o.set_field(f, J)
+            m.set_field(f, J)
+        ro = o.iterate(final)
+        rg = m.IterateFieldsHydroLinLayerSplit(0, 0, -1 if final else it)
+        assert abs(rg - ro) <= 1e-10 * max(1.0, abs(ro)), (it, rg, ro)
+        for f in ("Rho", "Ex", "Ey", "Ez", "Bx", "By", "Bz", "Jx", "Jy", "Jz"):
+            assert max_rel(m.get_spectral(f), o.get_spectral(f)) < 1e-12, ("iter", it, f)
+        fields = EB + (["Rho", "Psi"] if final else [])
+        for f in fields:
+            assert max_rel(m.get_field(f), o.get_field(f)) < 1e-12, ("iter C", it, f)
+    m.close()
+
+
+@pytest.mark.parametrize("shape", [0, 1])
+@pytest.mark.parametrize("fix_sub", [0, 1])
+def test_push_deposit(oracle_mod, shape, fix_sub):
+    rng = np.random.default_rng(100 + 10 * shape + fix_sub)
+    ny, nz = 32, 24
+    # a thick layer makes fast particles sub-cycle (|dy| > 1 cell)
+    p = small(ny=ny, nz=nz, nslices=3, hx=0.4, shape=shape, fix_subcycle=fix_sub)
+    m, o = make_pair(oracle_mod, p)
+    for which, lay in ((0, "C"), (1, "P")):
+        for f in EB:
+            F = smooth_field(rng, nz, ny, 0.8)
+            o.set_field(f, F, which=which)
+            m.set_field(f, F, layer=lay)
+    n = ny * nz * 4
+    parts = random_particles(rng, ny, nz, n, pmax=0.8, hot_fraction=0.05, hot_p=4.0)
+    parts[4][: n // 20] = 3.0   # px: Vx -> 1 makes dy large
+    o.set_particles(*parts)
+    m.set_particles(*parts)
+    for full in (0, 0, 1):
+        o.push_deposit(full)
+        m.MoveParticlesLayerSplit(0, 0, full)
+        for f in ("Jx", "Jy", "Jz", "Rho"):
+            ref, got = o.get_field(f), m.get_field(f)
+            assert max_rel(got, ref) <= 1e-14, (full, f)
+    po, pg = o.particles(), m.particles()
+    assert np.array_equal(po["id"], pg["id"])
+    assert np.array_equal(po["cj"], pg["cj"]) and np.array_equal(po["ck"], pg["ck"])
+    for k in ("fy", "fz", "px", "py", "pz"):
+        assert np.abs(po[k] - pg[k]).max() <= 1e-14 * max(1.0, np.abs(po[k]).max()), k
+    m.close()
+
+
+def test_push_error_paths(oracle_mod):
+    """The reference exits on a particle outside its cell (movepart_splitlayer.cpp:
+    945-951); here the call returns QSS_E_PARTICLE and every kernel finishes."""
+    from tertia_amd._lib import QssError
+    rng = np.random.default_rng(3)
+    p = small(ny=16, nz=16, nslices=2)
+    m, o = make_pair(oracle_mod, p)
+    parts = list(random_particles(rng, 16, 16, 256))
+    parts[2] = parts[2].copy()
+    parts[2][7] = 1.5
+    m.set_particles(*parts)
+    with pytest.raises(QssError) as e:
+        m.MoveParticlesLayerSplit(0, 0, 0)
+    assert e.value.code == -7
+    # non-finite field: bounded wrap loops, error reported, no hang
+    m.reset()
+    bad = np.zeros((16, 16))
+    bad[3, 4] = np.inf
+    m.set_field("Ex", bad, layer="C")
+    with pytest.raises(QssError):
+        m.MoveParticlesLayerSplit(0, 0, 0)
+    m.close()
