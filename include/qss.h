@@ -179,6 +179,9 @@ int  qss_transform2d_batched(qss_ctx *c, int n, const int *kind_z,
 /* which: 0 = C layer, 1 = P layer.  field: QSS_EX..QSS_PSI. */
 int  qss_get_layer_field(qss_ctx *c, int which, int field, double *host);
 int  qss_set_layer_field(qss_ctx *c, int which, int field, const double *host);
+/* Device pointer of a layer plane; valid until the next layer step (the C
+ * and P planes trade places at the end of every step).                      */
+int  qss_layer_ptr(qss_ctx *c, int which, int field, double **dev);
 /* history of field f for slice s (requires history_mask bit). */
 int  qss_get_history(qss_ctx *c, int field, int slice, double *host);
 /* Device pointer of the history block of field f (nslices*Ny*Nz doubles). */

@@ -46,6 +46,7 @@ SIGNATURES = {
     "qss_set_layer_field": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
     "qss_get_history": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
     "qss_history_ptr": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "qss_layer_ptr": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "qss_get_particles": (ctypes.c_int, [_vp, _i32p, _i32p, _dp, _dp, _dp, _dp, _dp, _i64p]),
     "qss_set_particles": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, _i32p, _dp, _dp, _dp,
                                          _dp, _dp, _i64p]),

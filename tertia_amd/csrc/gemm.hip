@@ -105,7 +105,8 @@ __global__ __launch_bounds__(NT) void k_gemm_f64(GemmArgs a) {
     }
   }
 
-  // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 tile
+  // epilogue: v_mfma_f64_16x16x4f64 leaves D[lane/16 + 4r][lane%16] in
+  // register r of each lane (measured on gfx950: rows interleave by 4)
   double dmax = 0.0, vmax = 0.0;
   const double alpha = a.alpha;
 #pragma unroll
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(NT) void k_gemm_f64(GemmArgs a) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * 32 + i * 16 + lk * 4 + r;
+        const int row = m0 + wm * 32 + i * 16 + lk + 4 * r;
         const int col = n0 + wn * 32 + j * 16 + li;
         if (row < M && col < N) {
           const long n = (long)row * N + col;

@@ -197,8 +197,14 @@ int do_beam(qss_ctx *c, int slice, int isplit) {
 
 int do_guess(qss_ctx *c, int first) {
   if (first) {
+    // The reference zeroes every spectrum at the first layer (:479-494) but
+    // then re-transforms the P layer, so only the beam spectrum (which
+    // becomes the beam P spectrum below) really restarts from zero.  The P
+    // spectra here are the context's state: zero after a reset, the split-0
+    // result for later splits of the first layer.
     Timed t(c, KC_SPEC);
-    for (int s = 0; s < SP_N; ++s) HIPCHECK(launch_fill_f64(c->S[s], c->ncell, 0.0, c->stream));
+    HIPCHECK(launch_fill_f64(c->S[SP_RHOB], c->ncell, 0.0, c->stream));
+    HIPCHECK(launch_fill_f64(c->S[SP_JXB], c->ncell, 0.0, c->stream));
   }
   // beam P spectrum = previous beam spectrum (:637-646): swap buffers
   std::swap(c->S[SP_RHOBP], c->S[SP_RHOB]);
@@ -636,6 +642,13 @@ int qss_get_history(qss_ctx *c, int field, int slice, double *host) {
   HIPCHECK(hipSetDevice(c->device));
   HIPCHECK(hipMemcpy(host, c->hist[field] + (size_t)slice * c->ncell,
                      c->ncell * sizeof(double), hipMemcpyDeviceToHost));
+  return QSS_OK;
+}
+
+int qss_layer_ptr(qss_ctx *c, int which, int field, double **dev) {
+  if (!c || !dev || field < 0 || field >= QSS_NFIELD || (which != 0 && which != 1))
+    return QSS_E_ARG;
+  *dev = which ? c->P[field] : c->C[field];
   return QSS_OK;
 }
 

@@ -35,7 +35,7 @@ struct SpecArgs {
   double *S[SP_N];
   int ny, nz;
   double ly, lz, hx, dens;
-  int flag;  // guess: first layer; iterate: final pass
+  int flag;  // iterate: final pass
 };
 
 __device__ inline void wavenumbers(const SpecArgs &a, int j, int k, double &aky,
@@ -53,16 +53,15 @@ __global__ __launch_bounds__(NT) void k_guess(SpecArgs a) {
   wavenumbers(a, j, k, aky, akz, ak2);
   double **S = a.S;
   const double dens = a.dens, hx = a.hx;
-  // the beam P spectrum is the previous beam spectrum (:637-646); at the
-  // first layer every spectrum starts from zero (:479-494)
-  const bool first = a.flag != 0;
-  const double rp = first ? 0.0 : S[SP_RHO][n1];
-  const double jxp = first ? 0.0 : S[SP_JX][n1];
-  const double jyp = first ? 0.0 : S[SP_JY][n1];
-  const double jzp = first ? 0.0 : S[SP_JZ][n1];
-  const double exp_ = first ? 0.0 : S[SP_EX][n1];
-  const double eyp = first ? 0.0 : S[SP_EY][n1];
-  const double ezp = first ? 0.0 : S[SP_EZ][n1];
+  // P spectra = the spectra left by the previous Iterate (or zero after a
+  // reset); the beam P spectrum is the previous beam spectrum (:637-646)
+  const double rp = S[SP_RHO][n1];
+  const double jxp = S[SP_JX][n1];
+  const double jyp = S[SP_JY][n1];
+  const double jzp = S[SP_JZ][n1];
+  const double exp_ = S[SP_EX][n1];
+  const double eyp = S[SP_EY][n1];
+  const double ezp = S[SP_EZ][n1];
   S[SP_RHOP][n1] = rp;
   S[SP_JXP][n1] = jxp;
   S[SP_JYP][n1] = jyp;

@@ -209,21 +209,28 @@ class Mesh:
                                        ii[1].ctypes.data_as(_i32p), *[_p(a) for a in ff],
                                        pid.ctypes.data_as(_i64p)), "qss_set_particles")
 
-    def transform2d(self, x, kind_z: int, kind_y: int, scale: float = 1.0):
+    def layer_ptr(self, name: str, layer: str = "C") -> int:
+        p = ctypes.c_void_p()
+        check(self.L.qss_layer_ptr(self.h, 0 if layer == "C" else 1, FIELD_ID[name],
+                                   ctypes.byref(p)), "qss_layer_ptr")
+        return p.value
+
+    def transform2d(self, x: np.ndarray, kind_z: int, kind_y: int,
+                    scale: float = 1.0) -> np.ndarray:
         """scale * T_kz . x . T_ky on the GPU (CUDA_WRAP_fourierHalfInteger2D,
-        half_integer2D.cu:583).  ``x``: (nz, ny) numpy array or CUDA tensor;
-        returns the same kind.  Device memory comes from torch."""
-        import torch
-        as_numpy = isinstance(x, np.ndarray)
-        xt = torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64)) if as_numpy else x
-        xt = xt.to(device="cuda", dtype=torch.float64).contiguous()
-        if tuple(xt.shape) != (self.nz, self.ny):
+        half_integer2D.cu:583) of a host (nz, ny) plane.  The C-layer Rho and
+        Psi planes serve as device staging (their contents are restored)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape != (self.nz, self.ny):
             raise QssError(-1, "transform2d", "plane shape must be (nz, ny)")
-        out = torch.empty_like(xt)
-        torch.cuda.synchronize()
-        check(self.L.qss_transform2d(self.h, int(kind_z), int(kind_y), xt.data_ptr(),
-                                     out.data_ptr(), float(scale)), "qss_transform2d")
-        return out.cpu().numpy() if as_numpy else out
+        saved = (self.get_field("Rho"), self.get_field("Psi"))
+        self.set_field("Rho", x)
+        check(self.L.qss_transform2d(self.h, int(kind_z), int(kind_y), self.layer_ptr("Rho"),
+                                     self.layer_ptr("Psi"), float(scale)), "qss_transform2d")
+        out = self.get_field("Psi")
+        self.set_field("Rho", saved[0])
+        self.set_field("Psi", saved[1])
+        return out
 
     # -- timing ------------------------------------------------------------
     def enable_timing(self, on: bool = True):
