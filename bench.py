@@ -79,6 +79,7 @@ def main():
         print("warmup + steps must be <= %d slices" % p.nslices, file=sys.stderr)
         return 2
     m = Mesh(p, history=())
+    choice = m.autotune(3)
     npart = m.n_particles
     m.reset()
     if a.warmup:
@@ -109,6 +110,7 @@ def main():
                                "electrons/slice), dxi=0.01, 2048 slices, driver nb=4 "
                                "sr=0.25 sxi=0.7 + witness nb=8 sr=0.1 sxi=0.1, CIC, nIter=2",
                    "slices_per_s": a.steps / dt,
+                   "gemm_tiles": choice,
                    "max_residual": float(max(res)) if len(res) else None},
     }
     if tm is not None:

@@ -175,6 +175,22 @@ int  qss_transform2d_batched(qss_ctx *c, int n, const int *kind_z,
                              const int *kind_y, const double *const *in_dev,
                              double *const *out_dev, double scale);
 
+/* ---- transform tile shapes --------------------------------------------- */
+/* The reference plans its transforms with FFTW_ESTIMATE
+ * (movefieldshydrolinlayersplit.cpp:389-427).  Here the transform is a
+ * batched MFMA GEMM compiled for several tile shapes; qss_autotune times
+ * every shape on this context's buffers for every launch shape the sweep
+ * uses (1..8 fields, both passes) and keeps the fastest.  reps >= 1.       */
+int  qss_autotune(qss_ctx *c, int reps, float *table_ms);
+/* table_ms (may be NULL): 2*8*qss_gemm_variants() floats, the mean time of
+ * one launch for [pass][nbatch-1][variant].                                 */
+/* variant >= 0 forces one tile shape for every launch (testing); -1 returns
+ * to the tuned/default choice.  qss_gemm_variants returns the count.       */
+int  qss_set_gemm_variant(qss_ctx *c, int variant);
+int  qss_gemm_variants(void);
+/* chosen[pass*9 + nbatch] for pass 0 (X.T_y), 1 (T_z.Y), nbatch 1..8.      */
+int  qss_gemm_choice(const qss_ctx *c, int *chosen);
+
 /* ---- data access (host buffers, Ny*Nz doubles, synchronous) ------------ */
 /* which: 0 = C layer, 1 = P layer.  field: QSS_EX..QSS_PSI. */
 int  qss_get_layer_field(qss_ctx *c, int which, int field, double *host);

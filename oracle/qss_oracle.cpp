@@ -644,10 +644,11 @@ int push_one(const oracle *o, const std::vector<double> *F, long n,
 }
 
 // MoveParticlesLayerSplit + DepositCurrentsInCellSplit + ExchangeCurrentsSplit
-// Events are reduced per target cell in event order (particle slot, then
-// sub-step) into four corner sums, then nodes are combined in the fixed
-// order c00 + c10 + c01 + c11 (periodic).  That equals the reference's
-// ghost fold (:1685-1711) up to summation order.
+// Events are deposited in slot order (particle, then sub-step) straight into
+// the nodes, like DepositCurrentsInCellSplit (:1612-1615) / MoveInCell
+// (movepart_layer_CIC.cpp:470-495), on planes with one ghost node per axis;
+// the ghosts are then folded periodically, y first, then z, as
+// ExchangeCurrentsSplit does (:1685-1711).
 int push_deposit(oracle *o, int full_step) {
   const int ny = o->ny, nz = o->nz;
   const long ncell = o->ncell;
@@ -669,36 +670,41 @@ int push_deposit(oracle *o, int full_step) {
     push_one(o, F, n, evs[n], npx[n], npy[n], npz[n], st);
     status = std::min(status, st);
   }
-  // deposit in slot order into per-cell corner sums
-  std::vector<double> corner((size_t)ncell * 16, 0.0);
+  // deposit in slot order onto (ny+1) x (nz+1) planes (one ghost per axis)
+  const int ey = ny + 1, ez = nz + 1;
+  std::vector<double> J[4];
+  for (int q = 0; q < 4; ++q) J[q].assign((size_t)ey * ez, 0.0);
   const bool cic = (o->p.shape == QSS_SHAPE_CIC);
   for (long n = 0; n < np; ++n)
     for (const Event &e : evs[n]) {
-      double *cs = &corner[(size_t)(e.cj + (long)ny * e.ck) * 16];
+      const long ccc = e.cj + (long)ey * e.ck;
       if (cic) {
-        double ayp = e.y, ayc = 1. - ayp, azp = e.z, azc = 1. - azp;
+        const double ayc = 1. - e.y, ayp = e.y, azc = 1. - e.z, azp = e.z;
+        const long cpc = ccc + 1, ccp = ccc + ey, cpp = ccp + 1;
         for (int q = 0; q < 4; ++q) {
-          cs[0 * 4 + q] += e.w[q] * ayc * azc;
-          cs[1 * 4 + q] += e.w[q] * ayp * azc;
-          cs[2 * 4 + q] += e.w[q] * ayc * azp;
-          cs[3 * 4 + q] += e.w[q] * ayp * azp;
+          J[q][ccc] += e.w[q] * ayc * azc;
+          J[q][cpc] += e.w[q] * ayp * azc;
+          J[q][ccp] += e.w[q] * ayc * azp;
+          J[q][cpp] += e.w[q] * ayp * azp;
         }
       } else {
-        for (int q = 0; q < 4; ++q) cs[q] += e.w[q];
+        for (int q = 0; q < 4; ++q) J[q][ccc] += e.w[q];
       }
     }
-  for (int kk = 0; kk < nz; ++kk)
-    for (int jj = 0; jj < ny; ++jj) {
-      int jm = (jj == 0) ? ny - 1 : jj - 1;
-      int km = (kk == 0) ? nz - 1 : kk - 1;
-      const double *c00 = &corner[(size_t)(jj + (long)ny * kk) * 16];
-      const double *c10 = &corner[(size_t)(jm + (long)ny * kk) * 16 + 4];
-      const double *c01 = &corner[(size_t)(jj + (long)ny * km) * 16 + 8];
-      const double *c11 = &corner[(size_t)(jm + (long)ny * km) * 16 + 12];
-      long nn = jj + (long)ny * kk;
-      for (int q = 0; q < 4; ++q)
-        o->C[QSS_JX + q][nn] = ((c00[q] + c10[q]) + c01[q]) + c11[q];
+  for (int q = 0; q < 4; ++q) {
+    std::vector<double> &A = J[q];
+    for (int kk = 0; kk <= nz; ++kk) {        // ghost column j = ny onto j = 0
+      A[0 + (long)ey * kk] += A[ny + (long)ey * kk];
+      A[ny + (long)ey * kk] = 0.;
     }
+    for (int jj = 0; jj <= ny; ++jj) {        // ghost row k = nz onto k = 0
+      A[jj] += A[jj + (long)ey * nz];
+      A[jj + (long)ey * nz] = 0.;
+    }
+    for (int kk = 0; kk < nz; ++kk)
+      for (int jj = 0; jj < ny; ++jj)
+        o->C[QSS_JX + q][jj + (long)ny * kk] = A[jj + (long)ey * kk];
+  }
   if (full_step) {
     // particles move to the C layer: last sub-step position, new momenta;
     // storage re-sorted by (cell, previous slot) -- stable.

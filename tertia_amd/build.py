@@ -4,7 +4,8 @@
 hipcc for gfx950 and links ``tertia_amd/libqss.so``.  The particle, beam and
 spectral kernels are compiled with ``-ffp-contract=off`` so that their
 arithmetic is evaluated in the reference's operation order (no FMA fusion);
-the transform GEMMs keep contraction (MFMA accumulation has its own order).
+the transform GEMMs keep contraction (MFMA accumulation has its own order)
+and keep their MFMA accumulators in VGPRs.
 All HIP code is compiled without packed-FP32 instructions.
 """
 from __future__ import annotations
@@ -25,8 +26,13 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
           "-Wno-unused-command-line-argument"]
-NO_CONTRACT = {"particles.hip", "beam.hip", "spectral.hip"}
-SOURCES = ["gemm.hip", "spectral.hip", "beam.hip", "particles.hip", "qss_api.cpp"]
+NO_CONTRACT = {"particles.hip", "push_tile.hip", "beam.hip", "spectral.hip"}
+# MFMA accumulators in VGPRs: with AGPR accumulators the compiler copies
+# every loop-carried accumulator AGPR->VGPR->AGPR at the K-loop back-edge,
+# draining the MFMA pipeline once per iteration
+MFMA_VGPR = {"gemm.hip"}
+SOURCES = ["gemm.hip", "spectral.hip", "beam.hip", "particles.hip", "push_tile.hip",
+           "qss_api.cpp"]
 
 
 def _compile(src: str) -> str:
@@ -34,6 +40,8 @@ def _compile(src: str) -> str:
     flags = list(COMMON)
     if src in NO_CONTRACT:
         flags.append("-ffp-contract=off")
+    if src in MFMA_VGPR:
+        flags += ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]
     if src.endswith(".cpp"):
         flags = ["-x", "hip"] + flags
     cmd = [HIPCC] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]

@@ -1,0 +1,569 @@
+// push_tile.hip — tiled predictor-corrector push with a deterministic
+// LDS deposit (the default particle path).
+//
+// Particles are stored sorted by 16x16-cell tile (tile_start[] gives each
+// tile's slot range).  One work-group owns one tile for a whole pass:
+//   1. it stages the tile's (P+C)/2 field window (17x17 nodes) in LDS, so
+//      the CIC/NGP gathers of MoveParticlesLayerSplit
+//      (movepart_splitlayer.cpp:1101-1217, movepart_layer_CIC.cpp:166-203)
+//      never leave the CU;
+//   2. each thread pushes one particle per chunk of 256 (same arithmetic as
+//      the reference, -ffp-contract=off);
+//   3. every deposit event (one per sub-step, :1391-1466) is keyed by its
+//      window cell; the chunk's keys are bitonic-sorted in LDS (register
+//      shuffles inside a wave, LDS across waves) and every window node sums
+//      the events of its four CIC corner cells in sorted order — a segmented
+//      reduction with no atomics (DepositCurrentsInCellSplit :1547-1655 /
+//      MoveInCell movepart_layer_CIC.cpp:449-496);
+//   4. the 21x21 node window (tile + 2-cell halo + 1) goes to HBM and
+//      k_combine adds the windows overlapping each node in a fixed order,
+//      which is the periodic ghost fold of ExchangeCurrentsSplit (:1685-1711).
+// With the reference's sub-cycle restart every sub-step lands within one
+// cell of the start cell, so every event falls inside the window.
+//
+// Full step: the pushed particle is staged with its destination tile
+// direction and its rank among the tile's particles going that way (slot
+// order); k_tile_scan / k_permute rebuild the tile-sorted storage: each
+// destination tile holds the groups from its 9 source directions in a fixed
+// order, each group in source slot order.  The storage order, and so every
+// summation order, is a pure function of the input.
+
+#include "qss_internal.h"
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int T = kTile;          // cells per tile edge
+constexpr int H = 2;              // halo cells
+constexpr int W = T + 2 * H + 1;  // window nodes per edge (21)
+constexpr int NWIN = W * W;       // 441
+constexpr int FW = T + 1;         // gathered field nodes per edge (17)
+constexpr int NF = FW * FW;       // 289
+constexpr int kMaxSub = 64;
+constexpr int kInvalid = 0x7fffffff;
+
+enum { ST_PARTICLE = 1, ST_OVERFLOW = 2, ST_SUBSTEPS = 4, ST_FAR = 8 };
+
+struct TileArgs {
+  const double *y, *z, *px, *py, *pz;
+  const int32_t *cell;
+  const int64_t *pid;
+  const int32_t *tile_start;
+  int ntj, ntk, ny, nz;
+  const double *F;
+  double hx, hy, hz, weight;
+  int shape, full;
+  double *win;            // ntiles x NWIN x 4
+  // full step staging (indexed by the old slot)
+  double *sy, *sz, *spx, *spy, *spz;
+  int32_t *scell, *sdest;  // sdest = direction | rank << 4
+  int64_t *spid;
+  int32_t *tcount;         // ntiles x 9
+  int32_t *counters;
+};
+
+__device__ inline int wrapi(int a, int n) {
+  a %= n;
+  return a < 0 ? a + n : a;
+}
+
+// bitonic sort of 256 int keys, one per thread: shuffles inside a wave,
+// LDS for the two cross-wave distances
+__device__ int sort256(int key, int *sh) {
+  const int t = threadIdx.x;
+  for (int k = 2; k <= NT; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      int other;
+      if (j >= 64) {
+        sh[t] = key;
+        __syncthreads();
+        other = sh[t ^ j];
+        __syncthreads();
+      } else {
+        other = __shfl_xor(key, j, 64);
+      }
+      const bool up = (t & k) == 0;
+      const bool lower = (t & j) == 0;
+      // lower element keeps min if ascending block, max otherwise
+      const int mn = min(key, other), mx = max(key, other);
+      key = (lower == up) ? mn : mx;
+    }
+  }
+  return key;
+}
+
+__global__ __launch_bounds__(NT) void k_push_tile(TileArgs a) {
+  __shared__ double Fw[NF * 6];
+  __shared__ int keys_sh[NT];
+  __shared__ int sorted[NT];
+  __shared__ double ev_y[NT], ev_z[NT];
+  __shared__ double4 ev_w[NT];
+  __shared__ short cstart[NWIN], cend[NWIN];
+  __shared__ int dir_total[9];
+  __shared__ int wave_cnt[NT / 64][9];
+  __shared__ int flag_sh;
+
+  const int t = threadIdx.x;
+  const int tile = blockIdx.x;
+  const int tj = tile % a.ntj, tk = tile / a.ntj;
+  const int j0 = tj * T, k0 = tk * T;
+  const int ny = a.ny, nz = a.nz;
+  const double hx = a.hx, hy = a.hy, hz = a.hz;
+  const double q2m = -1.0, djx0 = -1.0, djy0 = -1.0, djz0 = -1.0, drho0 = -1.0;
+  const double weight = a.weight;
+  const bool cic = a.shape == QSS_SHAPE_CIC;
+
+  // 1. field window: nodes (j0..j0+T, k0..k0+T), periodic
+  for (int i = t; i < NF; i += NT) {
+    const int lj = i % FW, lk = i / FW;
+    const long g = wrapi(j0 + lj, ny) + (long)ny * wrapi(k0 + lk, nz);
+    const double2 *src = reinterpret_cast<const double2 *>(a.F + g * kPushStride);
+    const double2 u = src[0], v = src[1], w = src[2];
+    Fw[i * 6 + 0] = u.x; Fw[i * 6 + 1] = u.y; Fw[i * 6 + 2] = v.x;
+    Fw[i * 6 + 3] = v.y; Fw[i * 6 + 4] = w.x; Fw[i * 6 + 5] = w.y;
+  }
+  if (t < 9) dir_total[t] = 0;
+  double acc0[4] = {0, 0, 0, 0}, acc1[4] = {0, 0, 0, 0};
+  const int n0 = t, n1 = t + NT;   // owned window nodes
+  __syncthreads();
+
+  const int base = a.tile_start[tile];
+  const int np = a.tile_start[tile + 1] - base;
+  for (int c0 = 0; c0 < np; c0 += NT) {
+    const int slot = base + c0 + t;
+    const bool active = c0 + t < np;
+    int j = 0, k = 0;
+    double yp = 0, zp = 0, px = 0, py = 0, pz = 0;
+    double djx = 0, djy = 0, djz = 0, drho = 0, dy = 0, dz = 0, partdx = 0, step = 1;
+    if (active) {
+      const int32_t c = a.cell[slot];
+      j = c % ny;
+      k = c / ny;
+      yp = a.y[slot];
+      zp = a.z[slot];
+      if (!(yp >= 0 && yp <= 1 && zp >= 0 && zp <= 1)) atomicOr(&a.counters[1], ST_PARTICLE);
+      px = a.px[slot]; py = a.py[slot]; pz = a.pz[slot];
+      const double gammap = sqrt(1. + px * px + py * py + pz * pz);
+      double Vx = px / gammap;
+      const double Vxp = Vx, Vyp = py / gammap, Vzp = pz / gammap;
+      // gather from the staged window: local node (j - j0, k - k0)
+      int lj = j - j0, lk = k - k0;
+      if (lj < 0 || lj >= T || lk < 0 || lk >= T) {   // not in this tile: never expected
+        atomicOr(&a.counters[1], ST_FAR);
+        lj = min(max(lj, 0), T - 1);
+        lk = min(max(lk, 0), T - 1);
+      }
+      double f[6];
+      const double *f00 = &Fw[(lj + FW * lk) * 6];
+      if (!cic) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) f[q] = f00[q];
+      } else {
+        const double *f10 = f00 + 6, *f01 = f00 + 6 * FW, *f11 = f01 + 6;
+        const double ayc = 1. - yp, ayp = yp, azc = 1. - zp, azp = zp;
+        const double acc = ayc * azc, apc = ayp * azc, acp = ayc * azp, app = ayp * azp;
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+          f[q] = acc * f00[q] + apc * f10[q] + acp * f01[q] + app * f11[q];
+      }
+      double ex = f[0], ey = f[1], ez = f[2], bx = f[3], by = f[4], bz = f[5];
+      ex *= q2m * hx / 2.;
+      ey *= q2m * hx / 2.;
+      ez *= q2m * hx / 2.;
+      px += ex / (1. - Vx);
+      py += ey / (1. - Vx);
+      pz += ez / (1. - Vx);
+      double gamma = sqrt(1. + px * px + py * py + pz * pz);
+      const double gamma_r = 1. / gamma;
+      bx = bx * gamma_r * q2m / (1. - Vx) * hx / 2.;
+      by = by * gamma_r * q2m / (1. - Vx) * hx / 2.;
+      bz = bz * gamma_r * q2m / (1. - Vx) * hx / 2.;
+      const double co = 2. / (1. + (bx * bx) + (by * by) + (bz * bz));
+      double p3x = py * bz - pz * by + px;
+      double p3y = pz * bx - px * bz + py;
+      double p3z = px * by - py * bx + pz;
+      p3x *= co;
+      p3y *= co;
+      p3z *= co;
+      const double px_new = p3y * bz - p3z * by;
+      const double py_new = p3z * bx - p3x * bz;
+      const double pz_new = p3x * by - p3y * bx;
+      px += ex / (1. - Vx) + px_new;
+      py += ey / (1. - Vx) + py_new;
+      pz += ez / (1. - Vx) + pz_new;
+      gamma = sqrt(1. + px * px + py * py + pz * pz);
+      const double Vxm = px / gamma, Vym = py / gamma, Vzm = pz / gamma;
+      Vx = 0.5 * (Vxm + Vxp);
+      const double Vy = 0.5 * (Vym + Vyp), Vz = 0.5 * (Vzm + Vzp);
+      djx = weight * djx0 * Vxm;
+      djy = weight * djy0 * Vym;
+      djz = weight * djz0 * Vzm;
+      drho = weight * drho0;
+      djx *= 1. / (1. - Vx);
+      djy *= 1. / (1. - Vx);
+      djz *= 1. / (1. - Vx);
+      drho *= 1. / (1. - Vx);
+      dy = Vy * hx / hy;
+      dz = Vz * hx / hz;
+      dy = dy / (1. - Vx);
+      dz = dz / (1. - Vx);
+      if (fabs(dy) > 1. || fabs(dz) > 1.) {
+        if (fabs(dy) > fabs(dz)) step = partdx = fabs(dy);
+        else step = partdx = fabs(dz);
+      }
+      if (partdx < 1.) partdx = step = 1.;
+      if (!(step < (double)(kMaxSub - 1))) {
+        atomicOr(&a.counters[1], ST_SUBSTEPS);
+        partdx = step = 1.;
+        dy = 0.;
+        dz = 0.;
+      }
+    }
+    double ytmp = yp, ztmp = zp;
+    int jj = j, kk = k;
+    // sub-step rounds: round r deposits every particle's r-th sub-step
+    for (;;) {
+      const bool have = active && partdx > 0.;
+      int key = kInvalid;
+      if (have) {
+        double part_step;
+        if (partdx > 1.) { partdx -= 1.; part_step = 1. / step; }
+        else { part_step = partdx / step; partdx = 0.; }
+        // the reference restarts every sub-step from the start cell
+        ytmp += dy * part_step + j;
+        ztmp += dz * part_step + k;
+        if (!(ytmp > -2.0 * ny && ytmp < 3.0 * ny && ztmp > -2.0 * nz && ztmp < 3.0 * nz)) {
+          atomicOr(&a.counters[1], ST_PARTICLE);
+          ytmp = 0.0;
+          ztmp = 0.0;
+        }
+        while (ytmp > ny) ytmp -= ny;
+        while (ytmp < 0) ytmp += ny;
+        while (ztmp > nz) ztmp -= nz;
+        while (ztmp < 0) ztmp += nz;
+        jj = (int)ytmp;
+        kk = (int)ztmp;
+        ytmp -= jj;
+        ztmp -= kk;
+        if (jj >= ny) jj -= ny;
+        if (kk >= nz) kk -= nz;
+        // window cell: any periodic image with both CIC nodes inside
+        int dj = jj - j0, dk = kk - k0;
+        if (dj + H < 0) dj += ny;
+        else if (dj + H > W - 2) dj -= ny;
+        if (dk + H < 0) dk += nz;
+        else if (dk + H > W - 2) dk -= nz;
+        const int lj = dj + H, lk = dk + H;
+        if (lj >= 0 && lj <= W - 2 && lk >= 0 && lk <= W - 2) {
+          key = ((lk * W + lj) << 8) | t;
+          ev_y[t] = ytmp;
+          ev_z[t] = ztmp;
+          ev_w[t] = make_double4(djx * part_step, djy * part_step, djz * part_step,
+                                 drho * part_step);
+        } else {
+          atomicOr(&a.counters[1], ST_FAR);
+        }
+      }
+      // segmented reduction of this round's events
+      for (int i = t; i < NWIN; i += NT) { cstart[i] = 0; cend[i] = 0; }
+      const int sk = sort256(key, keys_sh);
+      sorted[t] = sk;
+      __syncthreads();
+      if (sk != kInvalid) {
+        const int cellw = sk >> 8;
+        const int prev = t > 0 ? sorted[t - 1] : kInvalid;
+        const int next = t + 1 < NT ? sorted[t + 1] : kInvalid;
+        if (t == 0 || (prev >> 8) != cellw) cstart[cellw] = (short)t;
+        if (next == kInvalid || (next >> 8) != cellw) cend[cellw] = (short)(t + 1);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int o = 0; o < 2; ++o) {
+        const int n = o == 0 ? n0 : n1;
+        if (n >= NWIN) continue;
+        double *accn = o == 0 ? acc0 : acc1;
+        const int nl_j = n % W, nl_k = n / W;
+        // corners: own cell (1-y)(1-z); left y(1-z); below (1-y)z; both yz
+#pragma unroll
+        for (int cn = 0; cn < 4; ++cn) {
+          if (!cic && cn > 0) break;
+          const int cj = nl_j - (cn & 1), ck = nl_k - (cn >> 1);
+          if (cj < 0 || ck < 0) continue;
+          const int cw = ck * W + cj;
+          const int e1 = cend[cw];
+          for (int i = cstart[cw]; i < e1; ++i) {
+            const int e = sorted[i] & 255;
+            const double4 wv = ev_w[e];
+            const double wq[4] = {wv.x, wv.y, wv.z, wv.w};
+            if (cic) {
+              const double ayp = ev_y[e], ayc = 1. - ayp, azp = ev_z[e], azc = 1. - azp;
+              const double wy = (cn & 1) ? ayp : ayc;
+              const double wz = (cn >> 1) ? azp : azc;
+#pragma unroll
+              for (int q = 0; q < 4; ++q) accn[q] += wq[q] * wy * wz;
+            } else {
+#pragma unroll
+              for (int q = 0; q < 4; ++q) accn[q] += wq[q];
+            }
+          }
+        }
+      }
+      if (t == 0) flag_sh = 0;
+      __syncthreads();
+      if (active && partdx > 0.) flag_sh = 1;
+      __syncthreads();
+      const int more = flag_sh;
+      __syncthreads();
+      if (!more) break;
+    }
+    if (a.full) {
+      // destination tile direction (canonical for 1 or 2 tiles per axis)
+      int dtj = 0, dtk = 0;
+      if (active) {
+        const int ntj_ = a.ntj, ntk_ = a.ntk;
+        int d1 = wrapi(jj / T - tj, ntj_);
+        int d2 = wrapi(kk / T - tk, ntk_);
+        dtj = d1 == 0 ? 0 : (d1 == ntj_ - 1 && ntj_ > 2 ? -1 : 1);
+        dtk = d2 == 0 ? 0 : (d2 == ntk_ - 1 && ntk_ > 2 ? -1 : 1);
+        if (d1 > 1 && d1 < ntj_ - 1) atomicOr(&a.counters[1], ST_FAR);
+        if (d2 > 1 && d2 < ntk_ - 1) atomicOr(&a.counters[1], ST_FAR);
+      }
+      const int dir = (dtj + 1) + 3 * (dtk + 1);
+      // rank among this tile's particles with the same direction, slot order
+      const int lane = t & 63, wv = t >> 6;
+      const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+      int myrank_w = 0;
+      for (int d = 0; d < 9; ++d) {
+        const unsigned long long m = __ballot(active && dir == d);
+        if (lane == 0) wave_cnt[wv][d] = __popcll(m);
+        if (active && dir == d) myrank_w = __popcll(m & below);
+      }
+      __syncthreads();
+      int off = dir_total[dir];
+      for (int q = 0; q < wv; ++q) off += wave_cnt[q][dir];
+      if (active) {
+        a.sy[slot] = ytmp;
+        a.sz[slot] = ztmp;
+        a.spx[slot] = px;
+        a.spy[slot] = py;
+        a.spz[slot] = pz;
+        a.scell[slot] = jj + ny * kk;
+        a.spid[slot] = a.pid[slot];
+        a.sdest[slot] = dir | ((off + myrank_w) << 4);
+      }
+      __syncthreads();
+      if (t < 9) {
+        int s = 0;
+        for (int q = 0; q < NT / 64; ++q) s += wave_cnt[q][t];
+        dir_total[t] += s;
+      }
+      __syncthreads();
+    }
+  }
+  // 4. window to HBM
+  double *wout = a.win + (long)tile * NWIN * 4;
+  reinterpret_cast<double4 *>(wout)[n0] = make_double4(acc0[0], acc0[1], acc0[2], acc0[3]);
+  if (n1 < NWIN)
+    reinterpret_cast<double4 *>(wout)[n1] = make_double4(acc1[0], acc1[1], acc1[2], acc1[3]);
+  if (a.full && t < 9) a.tcount[tile * 9 + t] = dir_total[t];
+}
+
+// sum of the windows covering each node, fixed (dtk, dtj) order
+__global__ __launch_bounds__(NT) void k_combine(const double *win, int ntj, int ntk, int ny,
+                                                int nz, double *J0, double *J1, double *J2,
+                                                double *J3) {
+  const long n = (long)blockIdx.x * NT + threadIdx.x;
+  if (n >= (long)ny * nz) return;
+  const int nj = (int)(n % ny), nk = (int)(n / ny);
+  const int tj = nj / T, tk = nk / T;
+  double s[4] = {0, 0, 0, 0};
+  for (int dtk = -1; dtk <= 1; ++dtk) {
+    const int utk = tk + dtk;                 // unwrapped tile row
+    const int lk = nk - (utk * T - H);
+    if (lk < 0 || lk >= W) continue;
+    const int wtk = wrapi(utk, ntk);
+    for (int dtj = -1; dtj <= 1; ++dtj) {
+      const int utj = tj + dtj;
+      const int lj = nj - (utj * T - H);
+      if (lj < 0 || lj >= W) continue;
+      const int wtj = wrapi(utj, ntj);
+      const double4 v =
+          reinterpret_cast<const double4 *>(win + ((long)(wtk * ntj + wtj) * NWIN) * 4)[lk * W + lj];
+      s[0] += v.x; s[1] += v.y; s[2] += v.z; s[3] += v.w;
+    }
+  }
+  J0[n] = s[0]; J1[n] = s[1]; J2[n] = s[2]; J3[n] = s[3];
+}
+
+// new tile starts and group offsets: destination tile u receives the groups
+// of source tiles u - dir(d) for d = 0..8 in that order
+__global__ __launch_bounds__(NT) void k_tile_scan(const int32_t *tcount, int ntj, int ntk,
+                                                  int32_t *tile_start, int32_t *goff) {
+  __shared__ int sh[NT];
+  const int nt = ntj * ntk;
+  int carry = 0;
+  for (int b0 = 0; b0 < nt; b0 += NT) {
+    const int u = b0 + threadIdx.x;
+    int tot = 0;
+    int g[9];
+    if (u < nt) {
+      const int uj = u % ntj, uk = u / ntj;
+      for (int d = 0; d < 9; ++d) {
+        const int dj = d % 3 - 1, dk = d / 3 - 1;
+        const int src = wrapi(uk - dk, ntk) * ntj + wrapi(uj - dj, ntj);
+        g[d] = tcount[src * 9 + d];
+        tot += g[d];
+      }
+    }
+    // block exclusive scan of tot
+    sh[threadIdx.x] = tot;
+    __syncthreads();
+    for (int o = 1; o < NT; o <<= 1) {
+      const int v = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += v;
+      __syncthreads();
+    }
+    const int incl = sh[threadIdx.x];
+    const int start = carry + incl - tot;
+    if (u < nt) {
+      tile_start[u] = start;
+      int o = start;
+      for (int d = 0; d < 9; ++d) {
+        goff[u * 9 + d] = o;
+        o += g[d];
+      }
+    }
+    const int blk_tot = sh[NT - 1];
+    __syncthreads();
+    carry += blk_tot;
+  }
+  if (threadIdx.x == 0) tile_start[nt] = carry;
+}
+
+__global__ __launch_bounds__(NT) void k_permute(TileArgs a, const int32_t *goff, long np,
+                                                double *y, double *z, double *px, double *py,
+                                                double *pz, int32_t *cell, int64_t *pid) {
+  const long s = (long)blockIdx.x * NT + threadIdx.x;
+  if (s >= np) return;
+  const int32_t c = a.scell[s];
+  const int dst_tile = ((c / a.ny) / T) * a.ntj + (c % a.ny) / T;
+  const int sd = a.sdest[s];
+  const long o = goff[dst_tile * 9 + (sd & 15)] + (sd >> 4);
+  y[o] = a.sy[s];
+  z[o] = a.sz[s];
+  px[o] = a.spx[s];
+  py[o] = a.spy[s];
+  pz[o] = a.spz[s];
+  cell[o] = c;
+  pid[o] = a.spid[s];
+}
+
+// seeding in tile-major slot order: tile, local cell (j fastest), seed
+__global__ void k_seed_tiled(double *y, double *z, double *px, double *py, double *pz,
+                             int32_t *cell, int64_t *pid, long np, int ny, int ntj,
+                             int ppc_y, int ppc_z, double jitter, uint64_t seed) {
+  const long n = (long)blockIdx.x * NT + threadIdx.x;
+  if (n >= np) return;
+  const int ppc = ppc_y * ppc_z;
+  const long per_tile = (long)T * T * ppc;
+  const long tile = n / per_tile;
+  const long r = n - tile * per_tile;
+  const int lc = (int)(r / ppc), is = (int)(r - (long)lc * ppc);
+  const int gj = (int)(tile % ntj) * T + lc % T;
+  const int gk = (int)(tile / ntj) * T + lc / T;
+  const long c = gj + (long)ny * gk;
+  const long id = c * ppc + is;
+  const int iy = is % ppc_y, iz = is / ppc_y;
+  double yy = (iy + 1.0) / (ppc_y + 1.0);
+  double zz = (iz + 1.0) / (ppc_z + 1.0);
+  if (jitter != 0.0) {
+    // the same counter-based draws as the cell-ordered seeding (by id)
+    auto mix64 = [](uint64_t x) {
+      x += 0x9E3779B97F4A7C15ull;
+      x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+      x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+      return x ^ (x >> 31);
+    };
+    auto unif = [&](uint64_t a2, uint64_t b2) {
+      const uint64_t h = mix64(seed ^ mix64(a2 ^ mix64(b2)));
+      return (double)(h >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    };
+    yy += jitter * unif((uint64_t)id, 1);
+    zz += jitter * unif((uint64_t)id, 2);
+  }
+  y[n] = yy;
+  z[n] = zz;
+  px[n] = 0.0;
+  py[n] = 0.0;
+  pz[n] = 0.0;
+  cell[n] = (int32_t)c;
+  pid[n] = id;
+}
+
+__global__ void k_uniform_tile_start(int32_t *ts, int nt, int per_tile) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i <= nt) ts[i] = i * per_tile;
+}
+
+inline dim3 grid1(long n) { return dim3((unsigned)((n + NT - 1) / NT)); }
+
+TileArgs make_args(qss_ctx *c, int full) {
+  TileArgs a;
+  const int b = c->cur;
+  a.y = c->py_[b]; a.z = c->pz_[b]; a.px = c->ppx[b]; a.py = c->ppy[b]; a.pz = c->ppz[b];
+  a.cell = c->pcell[b]; a.pid = c->pid[b];
+  a.tile_start = c->tile_start[b];
+  a.ntj = c->ntj; a.ntk = c->ntk; a.ny = c->ny; a.nz = c->nz;
+  a.F = c->F;
+  a.hx = c->hxs; a.hy = c->hy; a.hz = c->hz; a.weight = c->pweight;
+  a.shape = c->p.shape; a.full = full;
+  a.win = c->win;
+  a.sy = c->ev_y; a.sz = c->ev_z; a.spx = c->ev_p; a.spy = c->ev_p + c->np_cap;
+  a.spz = c->ev_p + 2 * c->np_cap;
+  a.scell = c->ev_cell; a.sdest = c->ev_rank;
+  a.spid = reinterpret_cast<int64_t *>(c->ov_key);   // ov_cap >= np_cap
+  a.tcount = c->tcount;
+  a.counters = c->d_counters;
+  return a;
+}
+
+}  // namespace
+
+hipError_t launch_push_tile(qss_ctx *c, int full) {
+  const int nt = c->ntj * c->ntk;
+  hipLaunchKernelGGL(k_push_tile, dim3(nt), dim3(NT), 0, c->stream, make_args(c, full));
+  return hipGetLastError();
+}
+
+hipError_t launch_combine(qss_ctx *c) {
+  hipLaunchKernelGGL(k_combine, grid1(c->ncell), dim3(NT), 0, c->stream, c->win, c->ntj,
+                     c->ntk, c->ny, c->nz, c->C[QSS_JX], c->C[QSS_JY], c->C[QSS_JZ],
+                     c->C[QSS_RHO]);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_reorder(qss_ctx *c) {
+  const int nb = c->cur ^ 1;
+  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(NT), 0, c->stream, c->tcount, c->ntj, c->ntk,
+                     c->tile_start[nb], c->goff);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_permute, grid1(c->np), dim3(NT), 0, c->stream, make_args(c, 1),
+                     c->goff, c->np, c->py_[nb], c->pz_[nb], c->ppx[nb], c->ppy[nb],
+                     c->ppz[nb], c->pcell[nb], c->pid[nb]);
+  return hipGetLastError();
+}
+
+hipError_t launch_seed_tiled(qss_ctx *c) {
+  const int b = c->cur;
+  hipLaunchKernelGGL(k_seed_tiled, grid1(c->np), dim3(NT), 0, c->stream, c->py_[b],
+                     c->pz_[b], c->ppx[b], c->ppy[b], c->ppz[b], c->pcell[b], c->pid[b], c->np,
+                     c->ny, c->ntj, c->p.ppc_y, c->p.ppc_z, c->p.jitter, (uint64_t)c->p.seed);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int nt = c->ntj * c->ntk;
+  hipLaunchKernelGGL(k_uniform_tile_start, grid1(nt + 1), dim3(NT), 0, c->stream,
+                     c->tile_start[b], nt, T * T * c->p.ppc_y * c->p.ppc_z);
+  return hipGetLastError();
+}

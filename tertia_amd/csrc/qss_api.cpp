@@ -15,7 +15,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <numeric>
 #include <string>
 
 namespace {
@@ -137,6 +139,10 @@ int validate(const qss_params *p) {
 }
 
 // ---- transforms ------------------------------------------------------------
+inline int gemm_pick(const qss_ctx *c, int pass, int n) {
+  return c->gemm_force >= 0 ? c->gemm_force : c->gemm_var[pass][n];
+}
+
 // out_b = alpha * T_kz[b] . in_b . T_ky[b]: y-pass into tmp, then z-pass
 int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
                     const double *const *in, double *const *out, double alpha,
@@ -161,7 +167,7 @@ int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
   }
   {
     Timed t(c, KC_GEMM);
-    HIPCHECK(launch_gemm_f64(g, c->stream));
+    HIPCHECK(launch_gemm_f64(g, c->stream, gemm_pick(c, 0, n)));
   }
   if (c->timing) c->gemm_flops += 2.0 * g.M * g.N * g.K * n;
   g.M = c->nz; g.N = c->ny; g.K = c->nz;
@@ -176,7 +182,7 @@ int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
   }
   {
     Timed t(c, KC_GEMM);
-    HIPCHECK(launch_gemm_f64(g, c->stream));
+    HIPCHECK(launch_gemm_f64(g, c->stream, gemm_pick(c, 1, n)));
   }
   if (c->timing) c->gemm_flops += 2.0 * g.M * g.N * g.K * n;
   return QSS_OK;
@@ -234,6 +240,19 @@ int do_guess(qss_ctx *c, int first) {
 }
 
 int do_push_deposit(qss_ctx *c, int full) {
+  if (c->tiled) {
+    {
+      Timed t(c, KC_PUSH);
+      HIPCHECK(launch_push_tile(c, full));
+    }
+    {
+      Timed t(c, KC_REDUCE);
+      HIPCHECK(launch_combine(c));
+      if (full) HIPCHECK(launch_tile_reorder(c));
+    }
+    if (full) c->cur ^= 1;
+    return QSS_OK;
+  }
   {
     Timed t(c, KC_SORT);
     HIPCHECK(launch_fill_i32(c->cnt, c->ncell, 0, c->stream));
@@ -319,6 +338,7 @@ int check_status(qss_ctx *c) {
   HIPCHECK(hipStreamSynchronize(c->stream));
   if (h[1] & 2) { qss_set_error("sub-cycle overflow buffer exhausted"); return QSS_E_OVERFLOW; }
   if (h[1] & 4) { qss_set_error("displacement needs more than 62 sub-steps"); return QSS_E_PARTICLE; }
+  if (h[1] & 8) { qss_set_error("deposit outside the tile window"); return QSS_E_OVERFLOW; }
   if (h[1] & 1) { qss_set_error("particle outside its cell or non-finite"); return QSS_E_PARTICLE; }
   return QSS_OK;
 }
@@ -333,7 +353,8 @@ int do_reset(qss_ctx *c) {
   HIPCHECK(launch_fill_i32(c->d_counters, 2, 0, c->stream));
   c->cur = 0;
   c->np = c->np_cap;
-  HIPCHECK(launch_seed_plasma(c));
+  if (c->tiled) HIPCHECK(launch_seed_tiled(c));
+  else HIPCHECK(launch_seed_plasma(c));
   return QSS_OK;
 }
 
@@ -368,6 +389,10 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
   c->hz = p->lz / p->nz;
   c->pweight = (1.0 / (p->ppc_y * p->ppc_z)) * p->dens;
   c->nblk = (int)((c->ncell + 1023) / 1024);
+  c->tiled = p->ny % kTile == 0 && p->nz % kTile == 0 && !p->fix_subcycle &&
+             std::getenv("QSS_GLOBAL_DEPOSIT") == nullptr;
+  c->ntj = p->ny / kTile;
+  c->ntk = p->nz / kTile;
   int r = QSS_OK;
   auto fail = [&](int code) { qss_destroy(c); return code; };
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -425,6 +450,13 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
       (r = dalloc(&c->list, np + ov)) || (r = dalloc(&c->d_resid, 6)) ||
       (r = dalloc(&c->d_res_hist, (size_t)p->nslices * p->nx_split)))
     return fail(r);
+  if (c->tiled) {
+    const long nt = (long)c->ntj * c->ntk;
+    if ((r = dalloc(&c->tile_start[0], nt + 1)) || (r = dalloc(&c->tile_start[1], nt + 1)) ||
+        (r = dalloc(&c->win, nt * 441 * 4)) || (r = dalloc(&c->tcount, nt * 9)) ||
+        (r = dalloc(&c->goff, nt * 9)))
+      return fail(r);
+  }
   c->hmask = history_mask;
   for (int f = 0; f < QSS_NFIELD; ++f)
     if (history_mask & (1u << f))
@@ -457,6 +489,8 @@ int qss_destroy(qss_ctx *c) {
   dfree(c->cnt); dfree(c->cnt_main); dfree(c->off); dfree(c->off_main);
   dfree(c->blk); dfree(c->blk_main); dfree(c->list); dfree(c->d_resid);
   dfree(c->d_res_hist);
+  dfree(c->tile_start[0]); dfree(c->tile_start[1]); dfree(c->win); dfree(c->tcount);
+  dfree(c->goff);
   for (auto &pool : c->ev_pool)
     for (auto &pr : pool) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
   if (c->t0) (void)hipEventDestroy(c->t0);
@@ -603,8 +637,11 @@ int qss_transform2d_batched(qss_ctx *c, int n, const int *kind_z, const int *kin
     return QSS_E_ARG;
   for (int b = 0; b < n; ++b)
     if ((kind_z[b] != 0 && kind_z[b] != 1) || (kind_y[b] != 0 && kind_y[b] != 1) ||
-        !in_dev[b] || !out_dev[b])
+        !in_dev[b] || !out_dev[b] || ((uintptr_t)in_dev[b] & 15) ||
+        ((uintptr_t)out_dev[b] & 15)) {
+      qss_set_error("transform planes must be non-null and 16-byte aligned");
       return QSS_E_ARG;
+    }
   HIPCHECK(hipSetDevice(c->device));
   CHECK(transform_batch(c, n, kind_z, kind_y, in_dev, out_dev, scale, nullptr, nullptr,
                         nullptr));
@@ -717,15 +754,43 @@ int qss_set_particles(qss_ctx *c, int64_t n, const int32_t *cell_j, const int32_
     }
     cell[i] = cell_j[i] + c->ny * cell_k[i];
   }
+  // the tiled path needs slot order grouped by tile (stable: the given order
+  // is kept inside a tile)
+  std::vector<int64_t> order((size_t)n);
+  std::iota(order.begin(), order.end(), (int64_t)0);
+  std::vector<int32_t> tstart;
+  if (c->tiled) {
+    auto tile_of = [&](int64_t i) {
+      return (cell_k[i] / kTile) * c->ntj + cell_j[i] / kTile;
+    };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return tile_of(a) < tile_of(b); });
+    const int nt = c->ntj * c->ntk;
+    tstart.assign(nt + 1, 0);
+    for (int64_t i = 0; i < n; ++i) ++tstart[tile_of(i) + 1];
+    for (int q = 0; q < nt; ++q) tstart[q + 1] += tstart[q];
+  }
+  std::vector<int32_t> ocell((size_t)n);
+  std::vector<double> ofy((size_t)n), ofz((size_t)n), opx((size_t)n), opy((size_t)n),
+      opz((size_t)n);
+  std::vector<int64_t> oid((size_t)n);
+  for (int64_t q = 0; q < n; ++q) {
+    const int64_t i = order[q];
+    ocell[q] = cell[i]; ofy[q] = fy[i]; ofz[q] = fz[i];
+    opx[q] = px[i]; opy[q] = py[i]; opz[q] = pz[i]; oid[q] = id[i];
+  }
   HIPCHECK(hipSetDevice(c->device));
   const int b = c->cur;
-  HIPCHECK(hipMemcpy(c->pcell[b], cell.data(), n * 4, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(c->py_[b], fy, n * 8, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(c->pz_[b], fz, n * 8, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(c->ppx[b], px, n * 8, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(c->ppy[b], py, n * 8, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(c->ppz[b], pz, n * 8, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(c->pid[b], id, n * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->pcell[b], ocell.data(), n * 4, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->py_[b], ofy.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->pz_[b], ofz.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->ppx[b], opx.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->ppy[b], opy.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->ppz[b], opz.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->pid[b], oid.data(), n * 8, hipMemcpyHostToDevice));
+  if (c->tiled)
+    HIPCHECK(hipMemcpy(c->tile_start[b], tstart.data(), tstart.size() * 4,
+                       hipMemcpyHostToDevice));
   c->np = n;
   return QSS_OK;
 }

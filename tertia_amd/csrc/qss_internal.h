@@ -17,6 +17,8 @@ enum {
 
 // interleaved push field: (P+C)/2 of Ex,Ey,Ez,Bx,By,Bz per node, padded to 8
 constexpr int kPushStride = 8;
+// cells per tile edge of the tiled particle path
+constexpr int kTile = 16;
 
 // ---------------------------------------------------------------------------
 // transform GEMM (tertia_amd/csrc/gemm.hip)
@@ -39,7 +41,9 @@ struct GemmArgs {
   unsigned long long *resid;    // 6 slots: dif[3], max[3] as IEEE bit patterns
   GemmBatch b[kGemmMaxBatch];
 };
-hipError_t launch_gemm_f64(const GemmArgs &a, hipStream_t s);
+hipError_t launch_gemm_f64(const GemmArgs &a, hipStream_t s, int variant);
+int gemm_num_variants();
+const char *gemm_variant_name(int v);
 
 // ---------------------------------------------------------------------------
 // context
@@ -86,12 +90,25 @@ struct qss_ctx {
   int nblk = 0;
   int32_t *list = nullptr;
 
+  // tiled particle path (default when ny, nz are multiples of kTile and the
+  // sub-cycle restart quirk is kept): particles sorted by tile
+  bool tiled = false;
+  int ntj = 0, ntk = 0;
+  int32_t *tile_start[2] = {};   // ping-pong with the particle arrays
+  double *win = nullptr;         // ntiles x 441 x 4 deposit windows
+  int32_t *tcount = nullptr;     // ntiles x 9 full-step direction counts
+  int32_t *goff = nullptr;       // ntiles x 9 group offsets
+
   unsigned long long *d_resid = nullptr;   // 6 slots per pass
   double *d_res_hist = nullptr;            // residual of each layer step
   int res_count = 0;
 
   uint32_t hmask = 0;
   double *hist[QSS_NFIELD] = {};
+
+  // transform GEMM tile shape per (pass, batch): pass 0 = X.T_y, 1 = T_z.Y
+  int gemm_var[2][kGemmMaxBatch + 1] = {};
+  int gemm_force = -1;    // >= 0: use this variant for every launch
 
   // timing
   bool timing = false;
@@ -117,6 +134,10 @@ hipError_t launch_scatter(qss_ctx *c);
 hipError_t launch_tile_reduce(qss_ctx *c, int full_step);
 hipError_t launch_residual_final(qss_ctx *c, double *dst);
 hipError_t launch_push_field(qss_ctx *c);
+hipError_t launch_push_tile(qss_ctx *c, int full);
+hipError_t launch_combine(qss_ctx *c);
+hipError_t launch_tile_reorder(qss_ctx *c);
+hipError_t launch_seed_tiled(qss_ctx *c);
 
 // host helpers
 void qss_set_error(const std::string &m);

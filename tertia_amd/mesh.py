@@ -232,6 +232,30 @@ class Mesh:
         self.set_field("Psi", saved[1])
         return out
 
+    # -- transform tile shapes --------------------------------------------
+    def autotune(self, reps: int = 3) -> dict:
+        """Time every compiled transform tile shape for every launch shape
+        and keep the fastest (FFTW_MEASURE-style planning)."""
+        nv = self.gemm_variants()
+        tab = (ctypes.c_float * (2 * 8 * nv))()
+        check(self.L.qss_autotune(self.h, int(reps), tab), "qss_autotune")
+        out = self.gemm_choice()
+        out["table_us"] = [[[round(tab[(p * 8 + b) * nv + v] * 1e3, 1) for v in range(nv)]
+                            for b in range(8)] for p in range(2)]
+        return out
+
+    def gemm_choice(self) -> dict:
+        arr = (ctypes.c_int * 18)()
+        check(self.L.qss_gemm_choice(self.h, arr), "qss_gemm_choice")
+        return {"y_pass": list(arr[1:9]), "z_pass": list(arr[10:18])}
+
+    def set_gemm_variant(self, v: int):
+        check(self.L.qss_set_gemm_variant(self.h, int(v)), "qss_set_gemm_variant")
+
+    @staticmethod
+    def gemm_variants() -> int:
+        return _lib.load().qss_gemm_variants()
+
     # -- timing ------------------------------------------------------------
     def enable_timing(self, on: bool = True):
         check(self.L.qss_enable_timing(self.h, int(on)), "qss_enable_timing")

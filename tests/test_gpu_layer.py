@@ -5,8 +5,10 @@ Tolerances (FP64 both sides):
   * beam planes: max-rel 1e-14 (same sums, same order; exp may differ by 1 ulp)
   * Guess/Iterate: max-rel 1e-12 per spectral/real plane (inputs differ only
     by transform rounding, ~1e-15)
-  * push + deposit: max-rel 1e-14 on J, rho and the pushed particles (same
-    arithmetic in the same order, FMA contraction off on both sides)
+  * push + deposit: max-rel 1e-13 on J, rho (same arithmetic with FMA
+    contraction off on both sides; the sums over a node's contributions are
+    associated differently: the GPU adds per tile window, then windows) and
+    1e-14 on the pushed particles
 """
 import numpy as np
 import pytest
@@ -115,9 +117,11 @@ def test_guess_and_iterate(oracle_mod, nz, ny):
 
 @pytest.mark.parametrize("shape", [0, 1])
 @pytest.mark.parametrize("fix_sub", [0, 1])
-def test_push_deposit(oracle_mod, shape, fix_sub):
-    rng = np.random.default_rng(100 + 10 * shape + fix_sub)
-    ny, nz = 32, 24
+@pytest.mark.parametrize("ny,nz", [(32, 24), (48, 32), (16, 16)])
+def test_push_deposit(oracle_mod, shape, fix_sub, ny, nz):
+    """(48,32) and (16,16) with the kept sub-cycle quirk run the tiled LDS
+    deposit; (32,24) and fix_sub=1 run the global counting-sort deposit."""
+    rng = np.random.default_rng(100 + 10 * shape + fix_sub + ny)
     # a thick layer makes fast particles sub-cycle (|dy| > 1 cell)
     p = small(ny=ny, nz=nz, nslices=3, hx=0.4, shape=shape, fix_subcycle=fix_sub)
     m, o = make_pair(oracle_mod, p)
@@ -136,12 +140,15 @@ def test_push_deposit(oracle_mod, shape, fix_sub):
         m.MoveParticlesLayerSplit(0, 0, full)
         for f in ("Jx", "Jy", "Jz", "Rho"):
             ref, got = o.get_field(f), m.get_field(f)
-            assert max_rel(got, ref) <= 1e-14, (full, f)
+            assert max_rel(got, ref) <= 1e-13, (full, f)
     po, pg = o.particles(), m.particles()
-    assert np.array_equal(po["id"], pg["id"])
-    assert np.array_equal(po["cj"], pg["cj"]) and np.array_equal(po["ck"], pg["ck"])
+    oo, og = np.argsort(po["id"]), np.argsort(pg["id"])
+    assert np.array_equal(po["id"][oo], pg["id"][og])
+    assert np.array_equal(po["cj"][oo], pg["cj"][og])
+    assert np.array_equal(po["ck"][oo], pg["ck"][og])
     for k in ("fy", "fz", "px", "py", "pz"):
-        assert np.abs(po[k] - pg[k]).max() <= 1e-14 * max(1.0, np.abs(po[k]).max()), k
+        d = np.abs(po[k][oo] - pg[k][og]).max()
+        assert d <= 1e-14 * max(1.0, np.abs(po[k]).max()), k
     m.close()
 
 

@@ -71,3 +71,20 @@ def test_inverse_roundtrip(mesh_factory):
             y = m.transform2d(x, kz, ky)
             back = m.transform2d(y, kz, ky, scale=1.0 / (4.0 * nz * ny))
             assert max_rel(back, x) < 1e-13
+
+
+def test_every_tile_shape(mesh_factory, oracle_mod):
+    """Every compiled GEMM tile shape gives the same transform (to 1e-13),
+    including partial tiles (40 x 72) and the full 512 x 512 plane."""
+    from tertia_amd.mesh import Mesh
+    for nz, ny in ((40, 72), (512, 512)):
+        m = mesh_factory(nz, ny)
+        x = np.random.default_rng(11).standard_normal((nz, ny))
+        ref = oracle_mod.transform2d(x, 1, 0, use_fft=False, nthreads=4)
+        for v in range(Mesh.gemm_variants()):
+            m.set_gemm_variant(v)
+            assert max_rel(m.transform2d(x, 1, 0), ref) < 1e-13, (nz, ny, v)
+        m.set_gemm_variant(-1)
+        choice = m.autotune(2)
+        assert all(0 <= c < Mesh.gemm_variants() for c in choice["y_pass"] + choice["z_pass"])
+        assert max_rel(m.transform2d(x, 1, 0), ref) < 1e-13

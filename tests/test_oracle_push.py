@@ -1,0 +1,178 @@
+"""Pins the oracle's predictor-corrector push and deposit
+(movepart_splitlayer.cpp:880-1655; CIC shape movepart_layer_CIC.cpp:166-203,
+449-496) by invariants and closed forms:
+
+* zero fields: momenta unchanged, straight-line move by V*hx/h/(1-Vx);
+* pure magnetic field: the Boris rotation satisfies the implicit-midpoint
+  identity p+ - p- = (p+ + p-) x t with t = B q/m hx / (2 gamma (1-Vx)),
+  so |p| is conserved (a sign or factor error in the rotation breaks it);
+* pure uniform electric field: p gains E q/m hx / (1-Vx) exactly;
+* charge and current conservation of the deposit (CIC weights and NGP are a
+  partition of unity, sub-steps split the deposit into fractions summing
+  to one);
+* a uniform plasma at rest deposits Dens = -n0 on every node;
+* sub-cycling: with the quirk fixed the final position is the straight-line
+  one; the reference's literal restart (shadowed j_jump/k_jump) only matters
+  when |dy| > 1 cell.
+"""
+import numpy as np
+import pytest
+
+from tertia_amd.params import small
+
+EB = ["Ex", "Ey", "Ez", "Bx", "By", "Bz"]
+
+
+def make(oracle_mod, **kw):
+    p = small(ny=16, nz=12, nslices=2, **kw)
+    return p, oracle_mod.Oracle(p, history=())
+
+
+def set_uniform(o, p, vals):
+    for f, v in zip(EB, vals):
+        arr = np.full((p.nz, p.ny), float(v))
+        o.set_field(f, arr, which=0)
+        o.set_field(f, arr, which=1)
+
+
+def particles(rng, p, n, pmax):
+    cj = np.sort(rng.integers(0, p.ny, n)).astype(np.int32)
+    ck = rng.integers(0, p.nz, n).astype(np.int32)
+    o = np.lexsort((cj, ck))
+    cj, ck = cj[o], ck[o]
+    fy, fz = rng.uniform(0.05, 0.95, n), rng.uniform(0.05, 0.95, n)
+    pp = rng.uniform(-pmax, pmax, (3, n))
+    return cj, ck, fy, fz, pp[0], pp[1], pp[2], np.arange(n, dtype=np.int64)
+
+
+def by_id(d):
+    o = np.argsort(d["id"])
+    return {k: v[o] for k, v in d.items()}
+
+
+@pytest.mark.parametrize("shape", [0, 1])
+def test_zero_fields_straight_line(oracle_mod, shape):
+    p, o = make(oracle_mod, shape=shape, hx=0.05)
+    set_uniform(o, p, [0] * 6)
+    rng = np.random.default_rng(0)
+    parts = particles(rng, p, 200, 0.6)
+    o.set_particles(*parts)
+    assert o.push_deposit(1) == 0
+    d = by_id(o.particles())
+    cj, ck, fy, fz, px, py, pz, _ = parts
+    assert np.array_equal(d["px"], px) and np.array_equal(d["py"], py)
+    g = np.sqrt(1 + px ** 2 + py ** 2 + pz ** 2)
+    vx, vy, vz = px / g, py / g, pz / g
+    hy, hz = p.ly / p.ny, p.lz / p.nz
+    y = np.mod(cj + fy + vy * p.hx / hy / (1 - vx), p.ny)
+    z = np.mod(ck + fz + vz * p.hx / hz / (1 - vx), p.nz)
+    assert np.abs(d["y"] - y).max() < 1e-12
+    assert np.abs(d["z"] - z).max() < 1e-12
+
+
+def test_boris_rotation_pure_b(oracle_mod):
+    p, o = make(oracle_mod, shape=1, hx=0.05)
+    B = np.array([0.7, -1.3, 2.1])
+    set_uniform(o, p, [0, 0, 0, *B])
+    rng = np.random.default_rng(1)
+    parts = particles(rng, p, 300, 2.0)
+    o.set_particles(*parts)
+    o.push_deposit(1)
+    d = by_id(o.particles())
+    p0 = np.stack(parts[4:7], 1)
+    p1 = np.stack([d["px"], d["py"], d["pz"]], 1)
+    g = np.sqrt(1 + (p0 ** 2).sum(1))
+    vx = p0[:, 0] / g
+    t = B[None, :] * (-1.0) * p.hx / (2 * g * (1 - vx))[:, None]
+    assert np.abs(np.linalg.norm(p1, axis=1) - np.linalg.norm(p0, axis=1)).max() < 1e-13
+    lhs = p1 - p0
+    rhs = np.cross(p1 + p0, t)
+    assert np.abs(lhs - rhs).max() < 1e-13
+
+
+def test_uniform_e_kick(oracle_mod):
+    p, o = make(oracle_mod, shape=1, hx=0.05)
+    E = np.array([0.3, -0.8, 0.5])
+    set_uniform(o, p, [*E, 0, 0, 0])
+    rng = np.random.default_rng(2)
+    parts = particles(rng, p, 200, 1.0)
+    o.set_particles(*parts)
+    o.push_deposit(1)
+    d = by_id(o.particles())
+    p0 = np.stack(parts[4:7], 1)
+    g = np.sqrt(1 + (p0 ** 2).sum(1))
+    vx = p0[:, 0] / g
+    expect = p0 + (E[None, :] * (-1.0) * p.hx) / (1 - vx)[:, None]
+    got = np.stack([d["px"], d["py"], d["pz"]], 1)
+    assert np.abs(got - expect).max() < 1e-14
+
+
+@pytest.mark.parametrize("shape", [0, 1])
+@pytest.mark.parametrize("fix", [0, 1])
+def test_deposit_conservation(oracle_mod, shape, fix):
+    p, o = make(oracle_mod, shape=shape, fix_subcycle=fix, hx=0.6)
+    rng = np.random.default_rng(3)
+    for f in EB:
+        F = rng.uniform(-0.5, 0.5, (p.nz, p.ny))
+        o.set_field(f, F, which=0)
+        o.set_field(f, F, which=1)
+    parts = particles(rng, p, 400, 3.0)
+    o.set_particles(*parts)
+    o.push_deposit(0)
+    # independent charge sum: w * dRho0 / (1 - <Vx>), <Vx> from the new state
+    o2 = oracle_mod.Oracle(p, history=())
+    for f in EB:
+        o2.set_field(f, o.get_field(f, 0), which=0)
+        o2.set_field(f, o.get_field(f, 1), which=1)
+    o2.set_particles(*parts)
+    o2.push_deposit(1)
+    d = by_id(o2.particles())
+    p0 = np.stack(parts[4:7], 1)
+    p1 = np.stack([d["px"], d["py"], d["pz"]], 1)
+    v0 = p0 / np.sqrt(1 + (p0 ** 2).sum(1))[:, None]
+    v1 = p1 / np.sqrt(1 + (p1 ** 2).sum(1))[:, None]
+    vbar_x = 0.5 * (v0[:, 0] + v1[:, 0])
+    w = 0.25
+    rho = (w * -1.0 / (1 - vbar_x)).sum()
+    jx = (w * -1.0 * v1[:, 0] / (1 - vbar_x)).sum()
+    jy = (w * -1.0 * v1[:, 1] / (1 - vbar_x)).sum()
+    assert abs(o.get_field("Rho").sum() - rho) < 1e-11 * abs(rho)
+    assert abs(o.get_field("Jx").sum() - jx) < 1e-11 * max(1, abs(jx))
+    assert abs(o.get_field("Jy").sum() - jy) < 1e-11 * max(1, abs(jy))
+
+
+@pytest.mark.parametrize("shape", [0, 1])
+def test_uniform_plasma_at_rest(oracle_mod, shape):
+    p = small(ny=16, nz=12, nslices=2, shape=shape, jitter=0.0)
+    o = oracle_mod.Oracle(p, history=())
+    set_uniform(o, p, [0] * 6)
+    o.push_deposit(0)
+    assert np.abs(o.get_field("Rho") + 1.0).max() < 1e-15
+    for f in ("Jx", "Jy", "Jz"):
+        assert np.abs(o.get_field(f)).max() == 0.0
+
+
+def test_subcycle_fixed_is_straight_line(oracle_mod):
+    """|dy| of several cells: with the fix the sub-steps add up to the
+    straight-line move; the literal form restarts from the start cell."""
+    res = {}
+    for fix in (0, 1):
+        p, o = make(oracle_mod, shape=1, fix_subcycle=fix, hx=1.5)
+        set_uniform(o, p, [0] * 6)
+        cj = np.array([3], np.int32)
+        ck = np.array([4], np.int32)
+        parts = (cj, ck, np.array([0.3]), np.array([0.6]), np.array([1.2]),
+                 np.array([0.9]), np.array([-0.2]), np.array([0], np.int64))
+        o.set_particles(*parts)
+        o.push_deposit(1)
+        res[fix] = o.particles()
+        res[fix]["rho"] = o.get_field("Rho").sum()
+    g = np.sqrt(1 + 1.2 ** 2 + 0.9 ** 2 + 0.2 ** 2)
+    vx, vy = 1.2 / g, 0.9 / g
+    hy = p.ly / p.ny
+    dy = vy * p.hx / hy / (1 - vx)
+    assert dy > 2.0
+    assert abs(res[1]["y"][0] - np.mod(3.3 + dy, p.ny)) < 1e-12
+    assert abs(res[0]["y"][0] - np.mod(3.3 + dy, p.ny)) > 1e-3
+    for fix in (0, 1):   # the sub-steps split the deposit into fractions summing to one
+        assert abs(res[fix]["rho"] - 0.25 * -1.0 / (1 - vx)) < 1e-14
