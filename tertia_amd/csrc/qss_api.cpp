@@ -795,6 +795,63 @@ int qss_set_particles(qss_ctx *c, int64_t n, const int32_t *cell_j, const int32_
   return QSS_OK;
 }
 
+int qss_gemm_variants(void) { return gemm_num_variants(); }
+
+int qss_set_gemm_variant(qss_ctx *c, int variant) {
+  if (!c || variant < -1 || variant >= gemm_num_variants()) return QSS_E_ARG;
+  c->gemm_force = variant;
+  return QSS_OK;
+}
+
+int qss_gemm_choice(const qss_ctx *c, int *chosen) {
+  if (!c || !chosen) return QSS_E_ARG;
+  for (int p = 0; p < 2; ++p)
+    for (int b = 0; b <= kGemmMaxBatch; ++b)
+      chosen[p * (kGemmMaxBatch + 1) + b] = c->gemm_var[p][b];
+  return QSS_OK;
+}
+
+int qss_autotune(qss_ctx *c, int reps, float *table_ms) {
+  if (!c || reps < 1) return QSS_E_ARG;
+  HIPCHECK(hipSetDevice(c->device));
+  hipEvent_t e0, e1;
+  HIPCHECK(hipEventCreate(&e0));
+  HIPCHECK(hipEventCreate(&e1));
+  int rc = QSS_OK;
+  const int nv = gemm_num_variants();
+  for (int pass = 0; pass < 2 && rc == QSS_OK; ++pass)
+    for (int b = 1; b <= kGemmMaxBatch && rc == QSS_OK; ++b) {
+      GemmArgs g{};
+      g.M = c->nz; g.N = c->ny; g.K = pass == 0 ? c->ny : c->nz;
+      g.alpha = 1.0; g.nbatch = b; g.F = c->F; g.resid = c->d_resid;
+      for (int q = 0; q < b; ++q) {
+        // operands: any plane of the right size; outputs: the scratch planes
+        g.b[q].A = pass == 0 ? c->S[q] : c->d_T[2];
+        g.b[q].B = pass == 0 ? c->d_T[0] : c->S[q];
+        g.b[q].C = c->tmp[q];
+        g.b[q].P = nullptr; g.b[q].resid = -1; g.b[q].push = -1;
+      }
+      float best = 1e30f;
+      int bestv = 0;
+      for (int v = 0; v < nv; ++v) {
+        if (launch_gemm_f64(g, c->stream, v) != hipSuccess) { rc = QSS_E_HIP; break; }
+        (void)hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps; ++r) (void)launch_gemm_f64(g, c->stream, v);
+        (void)hipEventRecord(e1, c->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) { rc = QSS_E_HIP; break; }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (table_ms) table_ms[(pass * kGemmMaxBatch + (b - 1)) * nv + v] = ms / reps;
+        if (ms < best) { best = ms; bestv = v; }
+      }
+      c->gemm_var[pass][b] = bestv;
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != QSS_OK) qss_set_error("autotune launch failed");
+  return rc;
+}
+
 int qss_enable_timing(qss_ctx *c, int on) {
   if (!c) return QSS_E_ARG;
   c->timing = on != 0;

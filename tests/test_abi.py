@@ -1,0 +1,73 @@
+"""The C ABI: libqss.so loads without a GPU, exports every function declared in
+include/qss.h, and the ctypes mirror of qss_params has the C layout."""
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "qss.h")
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char)\s*\*?\s*(qss_\w+)\s*\(", txt,
+                                 re.M)))
+
+
+def test_header_declares_the_sweep_surface():
+    names = declared_functions()
+    for must in ("qss_create", "qss_destroy", "qss_sweep", "qss_layer_step", "qss_guess",
+                 "qss_iterate", "qss_push_deposit", "qss_transform2d", "qss_beam_layer"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from tertia_amd import _lib
+    L = _lib.load()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+    # and the Python binding declares a signature for each of them
+    assert set(declared_functions()) <= set(_lib.SIGNATURES), \
+        set(declared_functions()) - set(_lib.SIGNATURES)
+
+
+def test_create_without_gpu_fails_cleanly():
+    from tertia_amd import _lib
+    from tertia_amd.params import small
+    L = _lib.load()
+    c = small().to_c()
+    h = ctypes.c_void_p()
+    rc = L.qss_create(ctypes.byref(c), 0, 0, ctypes.byref(h))
+    if rc == 0:          # a GPU is present: nothing to check here
+        L.qss_destroy(h)
+        pytest.skip("GPU present")
+    assert rc in (-1, -2) and not h.value
+
+
+def test_params_layout_matches_c():
+    from tertia_amd.params import QssBunch, QssParams
+    fields = [f for f, _ in QssParams._fields_]
+    bfields = [f for f, _ in QssBunch._fields_]
+    src = ["#include <stdio.h>", "#include <stddef.h>", '#include "qss.h"', "int main(void){",
+           'printf("%zu %zu\\n", sizeof(qss_params), sizeof(qss_bunch));']
+    for f in fields:
+        src.append('printf("%%zu\\n", offsetof(qss_params, %s));' % f)
+    for f in bfields:
+        src.append('printf("%%zu\\n", offsetof(qss_bunch, %s));' % f)
+    src.append("return 0;}")
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "l.c")
+        exe = os.path.join(d, "l")
+        open(c, "w").write("\n".join(src))
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    vals = [int(v) for v in out]
+    assert vals[0] == ctypes.sizeof(QssParams) and vals[1] == ctypes.sizeof(QssBunch)
+    for i, f in enumerate(fields):
+        assert vals[2 + i] == getattr(QssParams, f).offset, f
+    for i, f in enumerate(bfields):
+        assert vals[2 + len(fields) + i] == getattr(QssBunch, f).offset, f
