@@ -10,13 +10,18 @@ defaults cover the whole 2048-slice sweep).  Everything is resident in HBM
 before the timed region: plasma particles are seeded on the device and the
 beam is deposited per slice on the device.
 
-value = particle-slice pushes/s = plasma particles per slice x timed slices /
-timed seconds (one push = one particle advanced across one slice with all
-nIter+1 = 3 passes).
+value = particle-slice pushes/s of the plain FP64 sweep = plasma particles
+per slice x timed slices / timed seconds (one push = one particle advanced
+across one slice with all nIter+1 = 3 passes).  The same timed window is run
+in each mixed-precision mode ("mixed": FP32 push with FP32-MFMA or
+split-BF16-MFMA transforms) and each mode's whole sweep is compared with the
+FP64 sweep: relative L2 over every 8th layer of Ex (the longitudinal field,
+E_z in the usual wakefield notation) and psi.
 
-roofline: the dominant kernel is the transform GEMM (FP64 MFMA); achieved =
-its algorithmic flops (2*M*N*K per GEMM pass and field) / its summed HIP-event
-time over the timed region, against MI355X's 78.6 TFLOP/s FP64 peak.
+roofline: the dominant kernel is the transform GEMM; achieved = its
+algorithmic flops (2*M*N*K per GEMM pass and field; 35 2-D transforms of 4N^3
+per slice) / its summed HIP-event time over the timed region (events on the
+context's stream), against MI355X's 78.6 TFLOP/s FP64 MFMA peak.
 
 cpu_baseline: the FP64 oracle (FFT-form DCT-IV, OpenMP) on the host cores,
 on the first slices of the same workload, reported in the same unit.
@@ -29,11 +34,18 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP64_MFMA_PEAK_TFLOPS = 78.6
-HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"fp64": 78.6, "mixed32": 157.3, "bf16x3": 2500.0, "bf16x6": 2500.0}
+MFMA_KIND = {"fp64": "FP64 MFMA", "mixed32": "FP32 MFMA", "bf16x3": "BF16 MFMA (3 products)",
+             "bf16x6": "BF16 MFMA (6 products)"}
+HW_FLOP_FACTOR = {"fp64": 1, "mixed32": 1, "bf16x3": 3, "bf16x6": 6}
+WORKLOAD = ("512x512 cells over 20x20 k_p^-1, 4 ppc (1,048,576 plasma electrons/slice), "
+            "dxi=0.01, 2048 slices, driver nb=4 sr=0.25 sxi=0.7 + witness nb=8 sr=0.1 "
+            "sxi=0.1, CIC, nIter=2")
 
 
 def parse():
@@ -44,6 +56,10 @@ def parse():
     ap.add_argument("--cpu-slices", type=int, default=6,
                     help="slices of the CPU baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--mixed", default="mixed32,bf16x6,bf16x3",
+                    help="comma list of mixed modes to time and compare ('' = none)")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the whole-sweep accuracy comparison of the mixed modes")
     ap.add_argument("--no-timing-events", action="store_true")
     return ap.parse_args()
 
@@ -64,69 +80,122 @@ def cpu_baseline(params, nslices, nthreads):
                       % (nslices, nthreads, dt)}
 
 
+def timed_run(mode, a):
+    """Autotune, sweep `warmup` slices, time `steps` slices."""
+    from tertia_amd.params import PRECISIONS, config2
+    from tertia_amd.mesh import Mesh
+    p = config2(precision=PRECISIONS[mode])
+    m = Mesh(p, history=())
+    try:
+        choice = m.autotune(3)
+        npart = m.n_particles
+        m.reset()
+        if a.warmup:
+            m.sweep_range(0, a.warmup)
+        if not a.no_timing_events:
+            m.enable_timing(True)
+        t0 = time.perf_counter()
+        res = m.sweep_range(a.warmup, a.steps)   # synchronises the stream at the end
+        dt = time.perf_counter() - t0
+        tm = m.timing() if not a.no_timing_events else None
+    finally:
+        m.close()
+    out = {"pushes_per_s": npart * a.steps / dt, "slices_per_s": a.steps / dt,
+           "ms_per_step": dt * 1e3 / a.steps, "max_residual": float(max(res)),
+           "gemm_tiles": choice}
+    if tm is not None:
+        gemm_ms = tm["ms"]["gemm"]
+        achieved = tm["gemm_flops"] / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+        out["gemm"] = {"achieved_tflops": achieved, "avg_launch_us":
+                       gemm_ms * 1e3 / max(1, tm["launches"]["gemm"]),
+                       "launches": tm["launches"]["gemm"], "flops": tm["gemm_flops"],
+                       "ms": gemm_ms}
+        out["kernel_ms"] = tm["ms"]
+        out["event_window_ms"] = tm["ms_total"]
+    return out
+
+
+def sampled_history(mode, every):
+    """Whole sweep (from the front) with Ex and psi stored; every `every`-th layer."""
+    from tertia_amd.params import PRECISIONS, config2
+    from tertia_amd.mesh import Mesh
+    p = config2(precision=PRECISIONS[mode])
+    m = Mesh(p, history=("Ex", "Psi"))
+    try:
+        m.MoveAllSplitLayers()
+        idx = range(0, p.nslices, every)
+        return {f: np.stack([m.history(f, s) for s in idx]) for f in ("Ex", "Psi")}
+    finally:
+        m.close()
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
-              "(the sweep is strictly sequential in xi, see DESIGN.md for sharding)" % a.gpus,
+              "(the xi sweep is sequential; sharding is described in DESIGN.md)" % a.gpus,
               file=sys.stderr)
         return 2
     from tertia_amd.params import config2
-    from tertia_amd.mesh import Mesh
-
-    p = config2()
-    if a.warmup + a.steps > p.nslices:
-        print("warmup + steps must be <= %d slices" % p.nslices, file=sys.stderr)
+    if a.warmup + a.steps > config2().nslices:
+        print("warmup + steps must be <= %d slices" % config2().nslices, file=sys.stderr)
         return 2
-    m = Mesh(p, history=())
-    choice = m.autotune(3)
-    npart = m.n_particles
-    m.reset()
-    if a.warmup:
-        m.sweep_range(0, a.warmup)
-    if not a.no_timing_events:
-        m.enable_timing(True)
-    t0 = time.perf_counter()
-    res = m.sweep_range(a.warmup, a.steps)   # synchronises the stream at the end
-    dt = time.perf_counter() - t0
-    tm = m.timing() if not a.no_timing_events else None
-    m.close()
+    modes = [s for s in a.mixed.split(",") if s]
 
-    value = npart * a.steps / dt
+    base = timed_run("fp64", a)
+    print("fp64: %.3f ms/slice" % base["ms_per_step"], file=sys.stderr, flush=True)
+    mixed = {}
+    for mode in modes:
+        mixed[mode] = timed_run(mode, a)
+        print("%s: %.3f ms/slice" % (mode, mixed[mode]["ms_per_step"]), file=sys.stderr,
+              flush=True)
+    if modes and not a.no_accuracy:
+        ref = sampled_history("fp64", 8)
+        for mode in modes:
+            h = sampled_history(mode, 8)
+            mixed[mode]["rel_l2_vs_fp64"] = {
+                f: float(np.linalg.norm(h[f] - ref[f]) / np.linalg.norm(ref[f]))
+                for f in ("Ex", "Psi")}
+            print("%s accuracy %s" % (mode, mixed[mode]["rel_l2_vs_fp64"]), file=sys.stderr,
+                  flush=True)
+    for mode, r in mixed.items():
+        r["speedup_vs_fp64"] = r["slices_per_s"] / base["slices_per_s"]
+        if "gemm" in r:
+            r["gemm"]["peak_tflops"] = PEAK_TFLOPS[mode]
+            r["gemm"]["mfma"] = MFMA_KIND[mode]
+            r["gemm"]["frac_of_peak_hw"] = (r["gemm"]["achieved_tflops"] *
+                                            HW_FLOP_FACTOR[mode] / PEAK_TFLOPS[mode])
+
     line = {
-        "metric": "particle-slice pushes/s at 512^2*4ppc (FP64)",
-        "value": value,
+        "metric": "particle-slice pushes/s at 512^2*4ppc, FP64 vs mixed",
+        "value": base["pushes_per_s"],
         "unit": "particle-slice pushes/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": dt * 1e3 / a.steps,
+        "ms_per_step": base["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded plasma lattice + frozen bi-Gaussian driver/witness)",
-        "config": {"workload": "512x512 cells over 20x20 k_p^-1, 4 ppc (1,048,576 plasma "
-                               "electrons/slice), dxi=0.01, 2048 slices, driver nb=4 "
-                               "sr=0.25 sxi=0.7 + witness nb=8 sr=0.1 sxi=0.1, CIC, nIter=2",
-                   "slices_per_s": a.steps / dt,
-                   "gemm_tiles": choice,
-                   "max_residual": float(max(res)) if len(res) else None},
+        "config": {"workload": WORKLOAD, "slices_per_s": base["slices_per_s"],
+                   "gemm_tiles": base["gemm_tiles"], "max_residual": base["max_residual"]},
     }
-    if tm is not None:
-        gemm_ms = tm["ms"]["gemm"]
-        achieved = tm["gemm_flops"] / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
-        line["roofline"] = {"bound": "mfma", "achieved": achieved,
-                            "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                            "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": None,
-                            "kernel": "k_gemm_f64 (transform GEMM)",
-                            "avg_launch_us": gemm_ms * 1e3 / max(1, tm["launches"]["gemm"]),
-                            "launches": tm["launches"]["gemm"]}
-        line["kernel_ms"] = tm["ms"]
-        line["event_window_ms"] = tm["ms_total"]
+    if "gemm" in base:
+        g = base["gemm"]
+        line["roofline"] = {"bound": "mfma", "achieved": g["achieved_tflops"],
+                            "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                            "frac": g["achieved_tflops"] / PEAK_TFLOPS["fp64"],
+                            "traffic": None, "kernel": "k_gemm<MathF64,...> (transform GEMM)",
+                            "avg_launch_us": g["avg_launch_us"], "launches": g["launches"]}
+        line["kernel_ms"] = base["kernel_ms"]
+        line["event_window_ms"] = base["event_window_ms"]
+    if mixed:
+        line["mixed"] = mixed
     if a.cpu_slices > 0:
         nthr = a.cpu_threads or min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(p, a.cpu_slices, nthr)
+        line["cpu_baseline"] = cpu_baseline(config2(), a.cpu_slices, nthr)
     print(json.dumps(line))
     return 0
 

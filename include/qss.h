@@ -50,10 +50,17 @@ extern "C" {
 #define QSS_SHAPE_NGP_REF 0
 #define QSS_SHAPE_CIC     1
 
-/* precision of the GPU path (the oracle is always FP64) */
-#define QSS_PREC_FP64     0   /* FP64 push, FP64 MFMA transforms             */
-#define QSS_PREC_MIXED32  1   /* FP32 push arithmetic, FP32 MFMA transforms;
-                                 FP64 particle state + FP64 residual          */
+/* precision of the GPU path (the oracle is always FP64).  In every mixed
+ * mode the plasma push runs in FP32 on an FP32 copy of the push field,
+ * while particle positions and momenta are accumulated across slices in
+ * FP64, deposits are summed in FP64, and the spectral state, the transform
+ * operands/results in HBM and the corrector residual stay FP64.  Mixed modes
+ * need ny, nz multiples of 16 and fix_subcycle = 0 (the tiled particle path). */
+#define QSS_PREC_FP64         0   /* FP64 push, FP64 MFMA transforms          */
+#define QSS_PREC_MIXED32      1   /* transforms on FP32 MFMA (fp32 operands)  */
+#define QSS_PREC_MIXED_BF16X3 2   /* transforms on BF16 MFMA, operands split
+                                     into 2 bf16 pieces, 3 products          */
+#define QSS_PREC_MIXED_BF16X6 3   /* 3 bf16 pieces, 6 products (~fp32)       */
 
 /* field ids for history / layer access */
 enum qss_field {
@@ -182,12 +189,13 @@ int  qss_transform2d_batched(qss_ctx *c, int n, const int *kind_z,
  * every shape on this context's buffers for every launch shape the sweep
  * uses (1..8 fields, both passes) and keeps the fastest.  reps >= 1.       */
 int  qss_autotune(qss_ctx *c, int reps, float *table_ms);
-/* table_ms (may be NULL): 2*8*qss_gemm_variants() floats, the mean time of
- * one launch for [pass][nbatch-1][variant].                                 */
+/* table_ms (may be NULL): 2*8*qss_gemm_variants(precision) floats, the mean
+ * time of one launch for [pass][nbatch-1][variant].                         */
 /* variant >= 0 forces one tile shape for every launch (testing); -1 returns
- * to the tuned/default choice.  qss_gemm_variants returns the count.       */
+ * to the tuned/default choice.  qss_gemm_variants returns the number of tile
+ * shapes compiled for a precision mode (QSS_PREC_*).                        */
 int  qss_set_gemm_variant(qss_ctx *c, int variant);
-int  qss_gemm_variants(void);
+int  qss_gemm_variants(int precision);
 /* chosen[pass*9 + nbatch] for pass 0 (X.T_y), 1 (T_z.Y), nbatch 1..8.      */
 int  qss_gemm_choice(const qss_ctx *c, int *chosen);
 

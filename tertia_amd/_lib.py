@@ -55,7 +55,7 @@ SIGNATURES = {
     "qss_enable_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "qss_autotune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
     "qss_set_gemm_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
-    "qss_gemm_variants": (ctypes.c_int, []),
+    "qss_gemm_variants": (ctypes.c_int, [ctypes.c_int]),
     "qss_gemm_choice": (ctypes.c_int, [_vp, _ip]),
     "qss_timing": (ctypes.c_int, [_vp, _dp, _dp, _i64p, _dp]),
 }

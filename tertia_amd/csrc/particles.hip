@@ -92,10 +92,9 @@ struct PushArgs {
   int32_t *cnt, *cnt_main;
 };
 
-__device__ inline void gather6(const double *F, long n, double *f) {
-  const double2 *q = reinterpret_cast<const double2 *>(F + n * kPushStride);
-  const double2 a = q[0], b = q[1], c = q[2];
-  f[0] = a.x; f[1] = a.y; f[2] = b.x; f[3] = b.y; f[4] = c.x; f[5] = c.y;
+__device__ inline void gather6(const double *F, long plane, long n, double *f) {
+#pragma unroll
+  for (int q = 0; q < kPushComp; ++q) f[q] = F[q * plane + n];
 }
 
 __global__ __launch_bounds__(NT) void k_push(PushArgs a) {
@@ -118,15 +117,15 @@ __global__ __launch_bounds__(NT) void k_push(PushArgs a) {
   const double Vzp = pz / gammap;
   double f[6];
   if (a.shape == QSS_SHAPE_NGP_REF) {
-    gather6(a.F, c0, f);
+    gather6(a.F, (long)ny * nz, c0, f);
   } else {
     const int j1 = (j + 1 == ny) ? 0 : j + 1;
     const int k1 = (k + 1 == nz) ? 0 : k + 1;
     double f00[6], f10[6], f01[6], f11[6];
-    gather6(a.F, c0, f00);
-    gather6(a.F, j1 + (long)ny * k, f10);
-    gather6(a.F, j + (long)ny * k1, f01);
-    gather6(a.F, j1 + (long)ny * k1, f11);
+    gather6(a.F, (long)ny * nz, c0, f00);
+    gather6(a.F, (long)ny * nz, j1 + (long)ny * k, f10);
+    gather6(a.F, (long)ny * nz, j + (long)ny * k1, f01);
+    gather6(a.F, (long)ny * nz, j1 + (long)ny * k1, f11);
     const double ayc = 1. - yp, ayp = yp, azc = 1. - zp, azp = zp;
     const double acc = ayc * azc, apc = ayp * azc, acp = ayc * azp, app = ayp * azp;
 #pragma unroll

@@ -51,6 +51,7 @@ struct TileArgs {
   const int32_t *tile_start;
   int ntj, ntk, ny, nz;
   const double *F;
+  const float *Ff;        // fp32 push field (mixed modes)
   double hx, hy, hz, weight;
   int shape, full;
   double *win;            // ntiles x NWIN x 4
@@ -92,8 +93,15 @@ __device__ int sort256(int key, int *sh) {
   return key;
 }
 
+// R: arithmetic of the push (double: the reference's; float: mixed modes).
+// FT: storage of the push field.  Positions and momenta are carried in
+// double in both: the float form computes each slice's kick, rotation and
+// displacement in float and adds them to the double state, and the deposit
+// weights are summed in double.  With R = double every expression below
+// evaluates exactly as the reference's (no contraction, same association).
+template <typename R, typename FT>
 __global__ __launch_bounds__(NT) void k_push_tile(TileArgs a) {
-  __shared__ double Fw[NF * 6];
+  __shared__ FT Fw[NF * 6];
   __shared__ int keys_sh[NT];
   __shared__ int sorted[NT];
   __shared__ double ev_y[NT], ev_z[NT];
@@ -117,10 +125,12 @@ __global__ __launch_bounds__(NT) void k_push_tile(TileArgs a) {
   for (int i = t; i < NF; i += NT) {
     const int lj = i % FW, lk = i / FW;
     const long g = wrapi(j0 + lj, ny) + (long)ny * wrapi(k0 + lk, nz);
-    const double2 *src = reinterpret_cast<const double2 *>(a.F + g * kPushStride);
-    const double2 u = src[0], v = src[1], w = src[2];
-    Fw[i * 6 + 0] = u.x; Fw[i * 6 + 1] = u.y; Fw[i * 6 + 2] = v.x;
-    Fw[i * 6 + 3] = v.y; Fw[i * 6 + 4] = w.x; Fw[i * 6 + 5] = w.y;
+    const long plane = (long)ny * nz;
+#pragma unroll
+    for (int q = 0; q < kPushComp; ++q) {
+      if constexpr (sizeof(FT) == 8) Fw[i * 6 + q] = a.F[q * plane + g];
+      else Fw[i * 6 + q] = a.Ff[q * plane + g];
+    }
   }
   if (t < 9) dir_total[t] = 0;
   double acc0[4] = {0, 0, 0, 0}, acc1[4] = {0, 0, 0, 0};
@@ -134,7 +144,7 @@ __global__ __launch_bounds__(NT) void k_push_tile(TileArgs a) {
     const bool active = c0 + t < np;
     int j = 0, k = 0;
     double yp = 0, zp = 0, px = 0, py = 0, pz = 0;
-    double djx = 0, djy = 0, djz = 0, drho = 0, dy = 0, dz = 0, partdx = 0, step = 1;
+    R djx = 0, djy = 0, djz = 0, drho = 0, dy = 0, dz = 0, partdx = 0, step = 1;
     if (active) {
       const int32_t c = a.cell[slot];
       j = c % ny;
@@ -143,9 +153,10 @@ __global__ __launch_bounds__(NT) void k_push_tile(TileArgs a) {
       zp = a.z[slot];
       if (!(yp >= 0 && yp <= 1 && zp >= 0 && zp <= 1)) atomicOr(&a.counters[1], ST_PARTICLE);
       px = a.px[slot]; py = a.py[slot]; pz = a.pz[slot];
-      const double gammap = sqrt(1. + px * px + py * py + pz * pz);
-      double Vx = px / gammap;
-      const double Vxp = Vx, Vyp = py / gammap, Vzp = pz / gammap;
+      const R rpx = (R)px, rpy = (R)py, rpz = (R)pz;
+      const R gammap = sqrt(R(1) + rpx * rpx + rpy * rpy + rpz * rpz);
+      R Vx = rpx / gammap;
+      const R Vxp = Vx, Vyp = rpy / gammap, Vzp = rpz / gammap;
       // gather from the staged window: local node (j - j0, k - k0)
       int lj = j - j0, lk = k - k0;
       if (lj < 0 || lj >= T || lk < 0 || lk >= T) {   // not in this tile: never expected
@@ -153,70 +164,74 @@ __global__ __launch_bounds__(NT) void k_push_tile(TileArgs a) {
         lj = min(max(lj, 0), T - 1);
         lk = min(max(lk, 0), T - 1);
       }
-      double f[6];
-      const double *f00 = &Fw[(lj + FW * lk) * 6];
+      R f[6];
+      const FT *f00 = &Fw[(lj + FW * lk) * 6];
       if (!cic) {
 #pragma unroll
-        for (int q = 0; q < 6; ++q) f[q] = f00[q];
+        for (int q = 0; q < 6; ++q) f[q] = (R)f00[q];
       } else {
-        const double *f10 = f00 + 6, *f01 = f00 + 6 * FW, *f11 = f01 + 6;
-        const double ayc = 1. - yp, ayp = yp, azc = 1. - zp, azp = zp;
-        const double acc = ayc * azc, apc = ayp * azc, acp = ayc * azp, app = ayp * azp;
+        const FT *f10 = f00 + 6, *f01 = f00 + 6 * FW, *f11 = f01 + 6;
+        const R ayc = R(1. - yp), ayp = R(yp), azc = R(1. - zp), azp = R(zp);
+        const R acc = ayc * azc, apc = ayp * azc, acp = ayc * azp, app = ayp * azp;
 #pragma unroll
         for (int q = 0; q < 6; ++q)
-          f[q] = acc * f00[q] + apc * f10[q] + acp * f01[q] + app * f11[q];
+          f[q] = acc * (R)f00[q] + apc * (R)f10[q] + acp * (R)f01[q] + app * (R)f11[q];
       }
-      double ex = f[0], ey = f[1], ez = f[2], bx = f[3], by = f[4], bz = f[5];
-      ex *= q2m * hx / 2.;
-      ey *= q2m * hx / 2.;
-      ez *= q2m * hx / 2.;
-      px += ex / (1. - Vx);
-      py += ey / (1. - Vx);
-      pz += ez / (1. - Vx);
-      double gamma = sqrt(1. + px * px + py * py + pz * pz);
-      const double gamma_r = 1. / gamma;
-      bx = bx * gamma_r * q2m / (1. - Vx) * hx / 2.;
-      by = by * gamma_r * q2m / (1. - Vx) * hx / 2.;
-      bz = bz * gamma_r * q2m / (1. - Vx) * hx / 2.;
-      const double co = 2. / (1. + (bx * bx) + (by * by) + (bz * bz));
-      double p3x = py * bz - pz * by + px;
-      double p3y = pz * bx - px * bz + py;
-      double p3z = px * by - py * bx + pz;
+      R ex = f[0], ey = f[1], ez = f[2], bx = f[3], by = f[4], bz = f[5];
+      ex *= R(q2m * hx / 2.);
+      ey *= R(q2m * hx / 2.);
+      ez *= R(q2m * hx / 2.);
+      // half kick, carried in the double state
+      const R kx = ex / (R(1) - Vx), ky = ey / (R(1) - Vx), kz = ez / (R(1) - Vx);
+      px += (double)kx;
+      py += (double)ky;
+      pz += (double)kz;
+      const R hpx = (R)px, hpy = (R)py, hpz = (R)pz;
+      R gamma = sqrt(R(1) + hpx * hpx + hpy * hpy + hpz * hpz);
+      const R gamma_r = R(1) / gamma;
+      bx = bx * gamma_r * R(q2m) / (R(1) - Vx) * R(hx) / R(2);
+      by = by * gamma_r * R(q2m) / (R(1) - Vx) * R(hx) / R(2);
+      bz = bz * gamma_r * R(q2m) / (R(1) - Vx) * R(hx) / R(2);
+      const R co = R(2) / (R(1) + (bx * bx) + (by * by) + (bz * bz));
+      R p3x = hpy * bz - hpz * by + hpx;
+      R p3y = hpz * bx - hpx * bz + hpy;
+      R p3z = hpx * by - hpy * bx + hpz;
       p3x *= co;
       p3y *= co;
       p3z *= co;
-      const double px_new = p3y * bz - p3z * by;
-      const double py_new = p3z * bx - p3x * bz;
-      const double pz_new = p3x * by - p3y * bx;
-      px += ex / (1. - Vx) + px_new;
-      py += ey / (1. - Vx) + py_new;
-      pz += ez / (1. - Vx) + pz_new;
-      gamma = sqrt(1. + px * px + py * py + pz * pz);
-      const double Vxm = px / gamma, Vym = py / gamma, Vzm = pz / gamma;
-      Vx = 0.5 * (Vxm + Vxp);
-      const double Vy = 0.5 * (Vym + Vyp), Vz = 0.5 * (Vzm + Vzp);
-      djx = weight * djx0 * Vxm;
-      djy = weight * djy0 * Vym;
-      djz = weight * djz0 * Vzm;
-      drho = weight * drho0;
-      djx *= 1. / (1. - Vx);
-      djy *= 1. / (1. - Vx);
-      djz *= 1. / (1. - Vx);
-      drho *= 1. / (1. - Vx);
-      dy = Vy * hx / hy;
-      dz = Vz * hx / hz;
-      dy = dy / (1. - Vx);
-      dz = dz / (1. - Vx);
-      if (fabs(dy) > 1. || fabs(dz) > 1.) {
+      const R px_new = p3y * bz - p3z * by;
+      const R py_new = p3z * bx - p3x * bz;
+      const R pz_new = p3x * by - p3y * bx;
+      px += (double)(kx + px_new);
+      py += (double)(ky + py_new);
+      pz += (double)(kz + pz_new);
+      const R npx = (R)px, npy = (R)py, npz = (R)pz;
+      gamma = sqrt(R(1) + npx * npx + npy * npy + npz * npz);
+      const R Vxm = npx / gamma, Vym = npy / gamma, Vzm = npz / gamma;
+      Vx = R(0.5) * (Vxm + Vxp);
+      const R Vy = R(0.5) * (Vym + Vyp), Vz = R(0.5) * (Vzm + Vzp);
+      djx = R(weight * djx0) * Vxm;
+      djy = R(weight * djy0) * Vym;
+      djz = R(weight * djz0) * Vzm;
+      drho = R(weight * drho0);
+      djx *= R(1) / (R(1) - Vx);
+      djy *= R(1) / (R(1) - Vx);
+      djz *= R(1) / (R(1) - Vx);
+      drho *= R(1) / (R(1) - Vx);
+      dy = Vy * R(hx) / R(hy);
+      dz = Vz * R(hx) / R(hz);
+      dy = dy / (R(1) - Vx);
+      dz = dz / (R(1) - Vx);
+      if (fabs(dy) > R(1) || fabs(dz) > R(1)) {
         if (fabs(dy) > fabs(dz)) step = partdx = fabs(dy);
         else step = partdx = fabs(dz);
       }
-      if (partdx < 1.) partdx = step = 1.;
-      if (!(step < (double)(kMaxSub - 1))) {
+      if (partdx < R(1)) partdx = step = R(1);
+      if (!(step < (R)(kMaxSub - 1))) {
         atomicOr(&a.counters[1], ST_SUBSTEPS);
-        partdx = step = 1.;
-        dy = 0.;
-        dz = 0.;
+        partdx = step = R(1);
+        dy = R(0);
+        dz = R(0);
       }
     }
     double ytmp = yp, ztmp = zp;
@@ -226,12 +241,12 @@ __global__ __launch_bounds__(NT) void k_push_tile(TileArgs a) {
       const bool have = active && partdx > 0.;
       int key = kInvalid;
       if (have) {
-        double part_step;
-        if (partdx > 1.) { partdx -= 1.; part_step = 1. / step; }
-        else { part_step = partdx / step; partdx = 0.; }
+        R part_step;
+        if (partdx > R(1)) { partdx -= R(1); part_step = R(1) / step; }
+        else { part_step = partdx / step; partdx = R(0); }
         // the reference restarts every sub-step from the start cell
-        ytmp += dy * part_step + j;
-        ztmp += dz * part_step + k;
+        ytmp += (double)(dy * part_step) + j;
+        ztmp += (double)(dz * part_step) + k;
         if (!(ytmp > -2.0 * ny && ytmp < 3.0 * ny && ztmp > -2.0 * nz && ztmp < 3.0 * nz)) {
           atomicOr(&a.counters[1], ST_PARTICLE);
           ytmp = 0.0;
@@ -258,8 +273,8 @@ __global__ __launch_bounds__(NT) void k_push_tile(TileArgs a) {
           key = ((lk * W + lj) << 8) | t;
           ev_y[t] = ytmp;
           ev_z[t] = ztmp;
-          ev_w[t] = make_double4(djx * part_step, djy * part_step, djz * part_step,
-                                 drho * part_step);
+          ev_w[t] = make_double4((double)(djx * part_step), (double)(djy * part_step),
+                                 (double)(djz * part_step), (double)(drho * part_step));
         } else {
           atomicOr(&a.counters[1], ST_FAR);
         }
@@ -516,6 +531,7 @@ TileArgs make_args(qss_ctx *c, int full) {
   a.tile_start = c->tile_start[b];
   a.ntj = c->ntj; a.ntk = c->ntk; a.ny = c->ny; a.nz = c->nz;
   a.F = c->F;
+  a.Ff = c->Ff;
   a.hx = c->hxs; a.hy = c->hy; a.hz = c->hz; a.weight = c->pweight;
   a.shape = c->p.shape; a.full = full;
   a.win = c->win;
@@ -532,7 +548,12 @@ TileArgs make_args(qss_ctx *c, int full) {
 
 hipError_t launch_push_tile(qss_ctx *c, int full) {
   const int nt = c->ntj * c->ntk;
-  hipLaunchKernelGGL(k_push_tile, dim3(nt), dim3(NT), 0, c->stream, make_args(c, full));
+  if (c->mixed)
+    hipLaunchKernelGGL((k_push_tile<float, float>), dim3(nt), dim3(NT), 0, c->stream,
+                       make_args(c, full));
+  else
+    hipLaunchKernelGGL((k_push_tile<double, double>), dim3(nt), dim3(NT), 0, c->stream,
+                       make_args(c, full));
   return hipGetLastError();
 }
 

@@ -122,8 +122,13 @@ int validate(const qss_params *p) {
     qss_set_error("unknown shape");
     return QSS_E_ARG;
   }
-  if (p->precision != QSS_PREC_FP64) {
-    qss_set_error("precision mode not available in this build");
+  if (p->precision < QSS_PREC_FP64 || p->precision > QSS_PREC_MIXED_BF16X6) {
+    qss_set_error("unknown precision mode");
+    return QSS_E_ARG;
+  }
+  if (p->precision != QSS_PREC_FP64 &&
+      (p->ny % kTile != 0 || p->nz % kTile != 0 || p->fix_subcycle)) {
+    qss_set_error("mixed precision needs ny, nz multiples of 16 and fix_subcycle = 0");
     return QSS_E_ARG;
   }
   if (p->nbins < 0 || (p->nbins > 0 && (!p->bins || !(p->bin_xi1 > p->bin_xi0)))) {
@@ -156,25 +161,40 @@ int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
   g.alpha = 1.0;
   g.nbatch = n;
   g.F = c->F;
+  g.Ff = c->Ff;
   g.resid = c->d_resid;
+  if (c->npc) {
+    // split-bf16: the inputs become npc bf16 planes; the y pass leaves its
+    // result transposed and split for the z pass
+    Timed t(c, KC_GEMM);
+    HIPCHECK(launch_split_bf16(in, c->xs, n, c->ncell, c->npc, c->stream));
+    g.tout = 1;
+  }
   for (int b = 0; b < n; ++b) {
     g.b[b].A = in[b];
     g.b[b].B = c->d_T[ky[b] == QSS_KIND_C ? 0 : 1];
     g.b[b].C = c->tmp[b];
     g.b[b].P = nullptr;
+    g.b[b].As = c->xs[b];
+    g.b[b].Bts = c->Ts[ky[b] == QSS_KIND_C ? 0 : 1];   // T_y is symmetric
+    g.b[b].Cts = c->ts[b];
     g.b[b].resid = -1;
     g.b[b].push = -1;
   }
   {
     Timed t(c, KC_GEMM);
-    HIPCHECK(launch_gemm_f64(g, c->stream, gemm_pick(c, 0, n)));
+    HIPCHECK(launch_gemm(g, c->stream, c->p.precision, gemm_pick(c, 0, n)));
   }
   if (c->timing) c->gemm_flops += 2.0 * g.M * g.N * g.K * n;
   g.M = c->nz; g.N = c->ny; g.K = c->nz;
   g.alpha = alpha;
+  g.tout = 0;
   for (int b = 0; b < n; ++b) {
     g.b[b].A = c->d_T[kz[b] == QSS_KIND_C ? 2 : 3];
     g.b[b].B = c->tmp[b];
+    g.b[b].As = c->Ts[kz[b] == QSS_KIND_C ? 2 : 3];
+    g.b[b].Bts = c->ts[b];
+    g.b[b].Cts = nullptr;
     g.b[b].C = out[b];
     g.b[b].resid = resid ? resid[b] : -1;
     g.b[b].push = push ? push[b] : -1;
@@ -182,7 +202,7 @@ int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
   }
   {
     Timed t(c, KC_GEMM);
-    HIPCHECK(launch_gemm_f64(g, c->stream, gemm_pick(c, 1, n)));
+    HIPCHECK(launch_gemm(g, c->stream, c->p.precision, gemm_pick(c, 1, n)));
   }
   if (c->timing) c->gemm_flops += 2.0 * g.M * g.N * g.K * n;
   return QSS_OK;
@@ -349,7 +369,10 @@ int do_reset(qss_ctx *c) {
     HIPCHECK(launch_fill_f64(c->P[f], c->ncell, 0.0, c->stream));
   }
   for (int s = 0; s < SP_N; ++s) HIPCHECK(launch_fill_f64(c->S[s], c->ncell, 0.0, c->stream));
-  HIPCHECK(launch_fill_f64(c->F, c->ncell * kPushStride, 0.0, c->stream));
+  HIPCHECK(launch_fill_f64(c->F, c->ncell * kPushComp, 0.0, c->stream));
+  if (c->Ff)   // +0.0f is the all-zero bit pattern
+    HIPCHECK(launch_fill_i32(reinterpret_cast<int32_t *>(c->Ff), c->ncell * kPushComp, 0,
+                             c->stream));
   HIPCHECK(launch_fill_i32(c->d_counters, 2, 0, c->stream));
   c->cur = 0;
   c->np = c->np_cap;
@@ -389,8 +412,9 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
   c->hz = p->lz / p->nz;
   c->pweight = (1.0 / (p->ppc_y * p->ppc_z)) * p->dens;
   c->nblk = (int)((c->ncell + 1023) / 1024);
+  c->mixed = p->precision != QSS_PREC_FP64;
   c->tiled = p->ny % kTile == 0 && p->nz % kTile == 0 && !p->fix_subcycle &&
-             std::getenv("QSS_GLOBAL_DEPOSIT") == nullptr;
+             (c->mixed || std::getenv("QSS_GLOBAL_DEPOSIT") == nullptr);
   c->ntj = p->ny / kTile;
   c->ntk = p->nz / kTile;
   int r = QSS_OK;
@@ -420,7 +444,24 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
     if ((r = dalloc(&c->S[s], nc)) != QSS_OK) return fail(r);
   for (int b = 0; b < kGemmMaxBatch; ++b)
     if ((r = dalloc(&c->tmp[b], nc)) != QSS_OK) return fail(r);
-  if ((r = dalloc(&c->F, nc * kPushStride)) != QSS_OK) return fail(r);
+  if ((r = dalloc(&c->F, nc * kPushComp)) != QSS_OK) return fail(r);
+  if (c->mixed && (r = dalloc(&c->Ff, nc * kPushComp)) != QSS_OK) return fail(r);
+  c->npc = gemm_pieces(p->precision);
+  if (c->npc) {
+    for (int i = 0; i < 4; ++i) {
+      const long n = i < 2 ? c->ny : c->nz;
+      if ((r = dalloc(&c->Ts[i], (size_t)c->npc * n * n)) != QSS_OK) return fail(r);
+      const double *src = c->d_T[i];
+      if (launch_split_bf16(&src, &c->Ts[i], 1, n * n, c->npc, c->stream) != hipSuccess) {
+        qss_set_error("table split failed");
+        return fail(QSS_E_HIP);
+      }
+    }
+    for (int b = 0; b < kGemmMaxBatch; ++b)
+      if ((r = dalloc(&c->xs[b], (size_t)c->npc * nc)) != QSS_OK ||
+          (r = dalloc(&c->ts[b], (size_t)c->npc * nc)) != QSS_OK)
+        return fail(r);
+  }
   if (!c->bins.empty()) {
     if ((r = dalloc(&c->d_bins, c->bins.size())) != QSS_OK) return fail(r);
     if (hipMemcpy(c->d_bins, c->bins.data(), c->bins.size() * 8, hipMemcpyHostToDevice) !=
@@ -478,7 +519,10 @@ int qss_destroy(qss_ctx *c) {
   for (int f = 0; f < QSS_NFIELD; ++f) { dfree(c->C[f]); dfree(c->P[f]); dfree(c->hist[f]); }
   for (auto &p : c->S) dfree(p);
   for (auto &p : c->tmp) dfree(p);
-  dfree(c->F); dfree(c->d_bins);
+  dfree(c->F); dfree(c->Ff); dfree(c->d_bins);
+  for (auto &p : c->Ts) dfree(p);
+  for (auto &p : c->xs) dfree(p);
+  for (auto &p : c->ts) dfree(p);
   for (int b = 0; b < 2; ++b) {
     dfree(c->py_[b]); dfree(c->pz_[b]); dfree(c->ppx[b]); dfree(c->ppy[b]);
     dfree(c->ppz[b]); dfree(c->pcell[b]); dfree(c->pid[b]);
@@ -795,10 +839,13 @@ int qss_set_particles(qss_ctx *c, int64_t n, const int32_t *cell_j, const int32_
   return QSS_OK;
 }
 
-int qss_gemm_variants(void) { return gemm_num_variants(); }
+int qss_gemm_variants(int precision) {
+  if (precision < QSS_PREC_FP64 || precision > QSS_PREC_MIXED_BF16X6) return QSS_E_ARG;
+  return gemm_num_variants(precision);
+}
 
 int qss_set_gemm_variant(qss_ctx *c, int variant) {
-  if (!c || variant < -1 || variant >= gemm_num_variants()) return QSS_E_ARG;
+  if (!c || variant < -1 || variant >= gemm_num_variants(c->p.precision)) return QSS_E_ARG;
   c->gemm_force = variant;
   return QSS_OK;
 }
@@ -818,25 +865,30 @@ int qss_autotune(qss_ctx *c, int reps, float *table_ms) {
   HIPCHECK(hipEventCreate(&e0));
   HIPCHECK(hipEventCreate(&e1));
   int rc = QSS_OK;
-  const int nv = gemm_num_variants();
+  const int prec = c->p.precision;
+  const int nv = gemm_num_variants(prec);
   for (int pass = 0; pass < 2 && rc == QSS_OK; ++pass)
     for (int b = 1; b <= kGemmMaxBatch && rc == QSS_OK; ++b) {
       GemmArgs g{};
       g.M = c->nz; g.N = c->ny; g.K = pass == 0 ? c->ny : c->nz;
-      g.alpha = 1.0; g.nbatch = b; g.F = c->F; g.resid = c->d_resid;
+      g.alpha = 1.0; g.nbatch = b; g.F = c->F; g.Ff = nullptr; g.resid = c->d_resid;
+      g.tout = (c->npc && pass == 0) ? 1 : 0;
       for (int q = 0; q < b; ++q) {
         // operands: any plane of the right size; outputs: the scratch planes
         g.b[q].A = pass == 0 ? c->S[q] : c->d_T[2];
         g.b[q].B = pass == 0 ? c->d_T[0] : c->S[q];
         g.b[q].C = c->tmp[q];
+        g.b[q].As = pass == 0 ? c->xs[q] : c->Ts[2];
+        g.b[q].Bts = pass == 0 ? c->Ts[0] : c->ts[q];
+        g.b[q].Cts = c->ts[q];
         g.b[q].P = nullptr; g.b[q].resid = -1; g.b[q].push = -1;
       }
       float best = 1e30f;
       int bestv = 0;
       for (int v = 0; v < nv; ++v) {
-        if (launch_gemm_f64(g, c->stream, v) != hipSuccess) { rc = QSS_E_HIP; break; }
+        if (launch_gemm(g, c->stream, prec, v) != hipSuccess) { rc = QSS_E_HIP; break; }
         (void)hipEventRecord(e0, c->stream);
-        for (int r = 0; r < reps; ++r) (void)launch_gemm_f64(g, c->stream, v);
+        for (int r = 0; r < reps; ++r) (void)launch_gemm(g, c->stream, prec, v);
         (void)hipEventRecord(e1, c->stream);
         if (hipEventSynchronize(e1) != hipSuccess) { rc = QSS_E_HIP; break; }
         float ms = 0.f;

@@ -15,35 +15,50 @@ enum {
   SP_RHOB, SP_JXB, SP_RHOBP, SP_JXBP, SP_PSI, SP_N
 };
 
-// interleaved push field: (P+C)/2 of Ex,Ey,Ez,Bx,By,Bz per node, padded to 8
-constexpr int kPushStride = 8;
+// push field: (P+C)/2 of Ex,Ey,Ez,Bx,By,Bz as 6 consecutive Ny*Nz planes
+constexpr int kPushComp = 6;
 // cells per tile edge of the tiled particle path
 constexpr int kTile = 16;
 
 // ---------------------------------------------------------------------------
 // transform GEMM (tertia_amd/csrc/gemm.hip)
-// C_b = alpha * A_b . B_b, all row-major, M x K . K x N, per-batch epilogue
+// C_b = alpha * A_b . B_b, all row-major, M x K . K x N, per-batch epilogue.
+// Split-bf16 modes take their operands pre-split into npc bf16 pieces (bit
+// patterns, piece p of an operand at +p*plane): A as [M][K], B transposed as
+// Bt[N][K], both k-contiguous; with tout the result is stored the same way,
+// transposed and split (Ct[N][M]), ready to be the next pass's Bt.
 // ---------------------------------------------------------------------------
 constexpr int kGemmMaxBatch = 8;
+constexpr int kMaxPieces = 3;
 struct GemmBatch {
   const double *A;
   const double *B;
   double *C;
   const double *P;   // P-layer plane for the push-field epilogue (or null)
+  const uint16_t *As;   // split operands (bf16 modes)
+  const uint16_t *Bts;
+  uint16_t *Cts;        // tout: transposed split result
   int resid;         // -1, or component 0..2: residual max|new-old|, max|new|
-  int push;          // -1, or component 0..5 of the interleaved push field
+  int push;          // -1, or component 0..5 of the push field
 };
 struct GemmArgs {
   int M, N, K;
   double alpha;
   int nbatch;
-  double *F;                    // interleaved push field (kPushStride doubles/node)
+  int tout;                     // bf16 modes: store Cts instead of C
+  double *F;                    // push field planes (kPushComp x M*N)
+  float *Ff;                    // its fp32 form (mixed modes; F unused when set)
   unsigned long long *resid;    // 6 slots: dif[3], max[3] as IEEE bit patterns
   GemmBatch b[kGemmMaxBatch];
 };
-hipError_t launch_gemm_f64(const GemmArgs &a, hipStream_t s, int variant);
-int gemm_num_variants();
-const char *gemm_variant_name(int v);
+// prec = QSS_PREC_*: selects the MFMA type and the variant table
+hipError_t launch_gemm(const GemmArgs &a, hipStream_t s, int prec, int variant);
+int gemm_num_variants(int prec);
+const char *gemm_variant_name(int prec, int v);
+int gemm_pieces(int prec);   // bf16 pieces per operand element (0: not split)
+// out[b] (npc planes of n bf16) = pieces of in[b] (n doubles), b < nb
+hipError_t launch_split_bf16(const double *const *in, uint16_t *const *out, int nb, long n,
+                             int npc, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // context
@@ -68,6 +83,12 @@ struct qss_ctx {
   double *S[SP_N] = {};
   double *tmp[kGemmMaxBatch] = {};   // transform intermediates
   double *F = nullptr;               // push field
+  float *Ff = nullptr;               // push field, fp32 (mixed modes)
+  bool mixed = false;                // precision != QSS_PREC_FP64
+  int npc = 0;                       // bf16 pieces per element (split modes)
+  uint16_t *Ts[4] = {};              // split tables (npc planes each)
+  uint16_t *xs[kGemmMaxBatch] = {};  // split transform inputs
+  uint16_t *ts[kGemmMaxBatch] = {};  // split, transposed y-pass results
   double *d_bins = nullptr;
 
   // particles (SoA), ping-pong for the full-step reorder

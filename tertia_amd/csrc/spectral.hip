@@ -142,23 +142,24 @@ __global__ void k_resid_final(const unsigned long long *r, double *dst) {
   *dst = err;
 }
 
-// F = (P + C)/2 interleaved, for layers edited from the host
+// F = (P + C)/2 as 6 planes, for layers edited from the host; fp32 copy in
+// the mixed modes
+template <typename FT>
 __global__ __launch_bounds__(NT) void k_push_field(const double *p0, const double *p1,
                                                   const double *p2, const double *p3,
                                                   const double *p4, const double *p5,
                                                   const double *c0, const double *c1,
                                                   const double *c2, const double *c3,
                                                   const double *c4, const double *c5,
-                                                  double *F, long n) {
+                                                  FT *F, long n) {
   const long i = (long)blockIdx.x * NT + threadIdx.x;
   if (i >= n) return;
-  double *f = F + i * kPushStride;
-  f[0] = 0.5 * (p0[i] + c0[i]);
-  f[1] = 0.5 * (p1[i] + c1[i]);
-  f[2] = 0.5 * (p2[i] + c2[i]);
-  f[3] = 0.5 * (p3[i] + c3[i]);
-  f[4] = 0.5 * (p4[i] + c4[i]);
-  f[5] = 0.5 * (p5[i] + c5[i]);
+  F[i] = (FT)(0.5 * (p0[i] + c0[i]));
+  F[n + i] = (FT)(0.5 * (p1[i] + c1[i]));
+  F[2 * n + i] = (FT)(0.5 * (p2[i] + c2[i]));
+  F[3 * n + i] = (FT)(0.5 * (p3[i] + c3[i]));
+  F[4 * n + i] = (FT)(0.5 * (p4[i] + c4[i]));
+  F[5 * n + i] = (FT)(0.5 * (p5[i] + c5[i]));
 }
 
 SpecArgs make_args(qss_ctx *c, int flag) {
@@ -210,8 +211,13 @@ hipError_t launch_residual_final(qss_ctx *c, double *dst) {
 }
 
 hipError_t launch_push_field(qss_ctx *c) {
-  hipLaunchKernelGGL(k_push_field, grid1(c->ncell), dim3(NT), 0, c->stream, c->P[0],
-                     c->P[1], c->P[2], c->P[3], c->P[4], c->P[5], c->C[0], c->C[1],
-                     c->C[2], c->C[3], c->C[4], c->C[5], c->F, c->ncell);
+  if (c->Ff)
+    hipLaunchKernelGGL(k_push_field<float>, grid1(c->ncell), dim3(NT), 0, c->stream, c->P[0],
+                       c->P[1], c->P[2], c->P[3], c->P[4], c->P[5], c->C[0], c->C[1],
+                       c->C[2], c->C[3], c->C[4], c->C[5], c->Ff, c->ncell);
+  else
+    hipLaunchKernelGGL(k_push_field<double>, grid1(c->ncell), dim3(NT), 0, c->stream, c->P[0],
+                       c->P[1], c->P[2], c->P[3], c->P[4], c->P[5], c->C[0], c->C[1],
+                       c->C[2], c->C[3], c->C[4], c->C[5], c->F, c->ncell);
   return hipGetLastError();
 }

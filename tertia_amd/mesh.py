@@ -236,7 +236,7 @@ class Mesh:
     def autotune(self, reps: int = 3) -> dict:
         """Time every compiled transform tile shape for every launch shape
         and keep the fastest (FFTW_MEASURE-style planning)."""
-        nv = self.gemm_variants()
+        nv = self.gemm_variants(self.params.precision)
         tab = (ctypes.c_float * (2 * 8 * nv))()
         check(self.L.qss_autotune(self.h, int(reps), tab), "qss_autotune")
         out = self.gemm_choice()
@@ -253,8 +253,12 @@ class Mesh:
         check(self.L.qss_set_gemm_variant(self.h, int(v)), "qss_set_gemm_variant")
 
     @staticmethod
-    def gemm_variants() -> int:
-        return _lib.load().qss_gemm_variants()
+    def gemm_variants(precision: int = 0) -> int:
+        """Number of transform tile shapes compiled for a precision mode."""
+        n = _lib.load().qss_gemm_variants(int(precision))
+        if n < 0:
+            raise QssError(n, "qss_gemm_variants", "unknown precision")
+        return n
 
     # -- timing ------------------------------------------------------------
     def enable_timing(self, on: bool = True):

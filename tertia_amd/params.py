@@ -23,6 +23,10 @@ QSS_SHAPE_NGP_REF = 0
 QSS_SHAPE_CIC = 1
 QSS_PREC_FP64 = 0
 QSS_PREC_MIXED32 = 1
+QSS_PREC_MIXED_BF16X3 = 2
+QSS_PREC_MIXED_BF16X6 = 3
+PRECISIONS = {"fp64": QSS_PREC_FP64, "mixed32": QSS_PREC_MIXED32,
+              "bf16x3": QSS_PREC_MIXED_BF16X3, "bf16x6": QSS_PREC_MIXED_BF16X6}
 QSS_MAX_BUNCH = 4
 
 # field ids (enum qss_field)

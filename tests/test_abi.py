@@ -48,6 +48,27 @@ def test_create_without_gpu_fails_cleanly():
     assert rc in (-1, -2) and not h.value
 
 
+@pytest.mark.parametrize("kw", [dict(precision=7), dict(precision=1, ny=24),
+                                dict(precision=2, fix_subcycle=1)])
+def test_create_rejects_bad_precision_before_touching_a_gpu(kw):
+    """Parameter validation runs first, so these fail with QSS_E_ARG on any host."""
+    from tertia_amd import _lib
+    from tertia_amd.params import small
+    L = _lib.load()
+    c = small(**kw).to_c()
+    h = ctypes.c_void_p()
+    assert L.qss_create(ctypes.byref(c), 0, 0, ctypes.byref(h)) == -1
+    assert b"precision" in L.qss_last_error()
+
+
+def test_variant_tables():
+    from tertia_amd import _lib
+    L = _lib.load()
+    for prec in range(4):
+        assert L.qss_gemm_variants(prec) > 0
+    assert L.qss_gemm_variants(9) == -1
+
+
 def test_params_layout_matches_c():
     from tertia_amd.params import QssBunch, QssParams
     fields = [f for f, _ in QssParams._fields_]
