@@ -228,6 +228,29 @@ int  qss_set_particles(qss_ctx *c, int64_t n, const int32_t *cell_j,
 int  qss_get_spectral(qss_ctx *c, int id, double *host);
 int  qss_set_spectral(qss_ctx *c, int id, const double *host);
 
+/* ---- differentiation: adjoint of the sweep ------------------------------ */
+/* The reference has no derivative of its sweep.  These entry points give the
+ * gradient of a scalar loss of the Ex planes (the longitudinal field, history
+ * of QSS_EX) with respect to the driver's piecewise-constant longitudinal
+ * profile (qss_params.nbins > 0), by a hand-written adjoint: reverse push
+ * with the transposed CIC gather and deposit, transposed spectral updates and
+ * transposed transform GEMMs, recomputing each segment of `every` layers from
+ * a checkpoint.  FP64, CIC, nx_split = 1, ny and nz multiples of 16.        */
+/* Replaces the profile values (nbins = the count given at create).          */
+int  qss_set_bins(qss_ctx *c, const double *bins, int nbins);
+/* Re-seeds and sweeps layers [0, nrun), storing the state at every
+ * `every`-th layer start (device memory, kept in the context) and the
+ * history planes of history_mask.  residuals: host, nrun values or NULL.   */
+int  qss_sweep_checkpointed(qss_ctx *c, int nrun, int every, double *residuals);
+/* Vector-Jacobian product of the preceding checkpointed sweep: ex_bar =
+ * dLoss/dEx of layers [0, nrun) (nrun*Ny*Nz doubles, layer-major; device
+ * memory, or host memory if on_host != 0; NULL = 0); grad_bins (host,
+ * nbins) = dLoss/dbins.  Leaves the layer state unspecified (call qss_reset
+ * or a sweep next).                                                          */
+int  qss_sweep_vjp(qss_ctx *c, const double *ex_bar, int on_host, double *grad_bins);
+/* Device-to-device copy of history planes [first, first+count) of a field. */
+int  qss_copy_history(qss_ctx *c, int field, int first, int count, double *dst_dev);
+
 /* ---- timing of the last qss_sweep/qss_sweep_range (HIP events) --------- */
 /* Kernel classes: 0 push, 1 transform GEMM, 2 spectral update, 3 beam,
  * 4 counting sort (fill+scan+scatter), 5 deposit tile reduce.              */

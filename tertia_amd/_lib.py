@@ -57,6 +57,11 @@ SIGNATURES = {
     "qss_set_gemm_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
     "qss_gemm_variants": (ctypes.c_int, [ctypes.c_int]),
     "qss_gemm_choice": (ctypes.c_int, [_vp, _ip]),
+    "qss_set_bins": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
+    "qss_sweep_checkpointed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "qss_sweep_vjp": (ctypes.c_int, [_vp, ctypes.c_void_p, ctypes.c_int, _dp]),
+    "qss_copy_history": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p]),
     "qss_timing": (ctypes.c_int, [_vp, _dp, _dp, _i64p, _dp]),
 }
 

@@ -140,7 +140,8 @@ __device__ inline void store_tiles(const GemmArgs &a, const GemmBatch &B, int r0
         const int col = c0 + j * 16 + li;
         if (!GUARD || (row < M && col < N)) {
           const long n = (long)row * N + col;
-          const double v = alpha * val(i, j, r);
+          double v = alpha * val(i, j, r);
+          if (a.beta != 0.0) v += a.beta * B.C[n];   // accumulate (adjoint sweep)
           if (B.resid >= 0) {
             // NaN -> +inf so the non-finite guard sees it (fmax drops NaN)
             const double old = B.C[n];

@@ -29,6 +29,7 @@
 // summation order, is a pure function of the input.
 
 #include "qss_internal.h"
+#include "tile_sort.h"
 
 namespace {
 
@@ -68,30 +69,7 @@ __device__ inline int wrapi(int a, int n) {
   return a < 0 ? a + n : a;
 }
 
-// bitonic sort of 256 int keys, one per thread: shuffles inside a wave,
-// LDS for the two cross-wave distances
-__device__ int sort256(int key, int *sh) {
-  const int t = threadIdx.x;
-  for (int k = 2; k <= NT; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      int other;
-      if (j >= 64) {
-        sh[t] = key;
-        __syncthreads();
-        other = sh[t ^ j];
-        __syncthreads();
-      } else {
-        other = __shfl_xor(key, j, 64);
-      }
-      const bool up = (t & k) == 0;
-      const bool lower = (t & j) == 0;
-      // lower element keeps min if ascending block, max otherwise
-      const int mn = min(key, other), mx = max(key, other);
-      key = (lower == up) ? mn : mx;
-    }
-  }
-  return key;
-}
+using qss_tile::sort256;
 
 // R: arithmetic of the push (double: the reference's; float: mixed modes).
 // FT: storage of the push field.  Positions and momenta are carried in

@@ -151,7 +151,8 @@ inline int gemm_pick(const qss_ctx *c, int pass, int n) {
 // out_b = alpha * T_kz[b] . in_b . T_ky[b]: y-pass into tmp, then z-pass
 int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
                     const double *const *in, double *const *out, double alpha,
-                    const int *resid, const int *push, const double *const *pplane) {
+                    const int *resid, const int *push, const double *const *pplane,
+                    double beta = 0.0) {
   if (n < 1 || n > kGemmMaxBatch) {
     qss_set_error("transform batch size");
     return QSS_E_ARG;
@@ -188,6 +189,7 @@ int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
   if (c->timing) c->gemm_flops += 2.0 * g.M * g.N * g.K * n;
   g.M = c->nz; g.N = c->ny; g.K = c->nz;
   g.alpha = alpha;
+  g.beta = beta;
   g.tout = 0;
   for (int b = 0; b < n; ++b) {
     g.b[b].A = c->d_T[kz[b] == QSS_KIND_C ? 2 : 3];
@@ -339,16 +341,252 @@ void swap_c_p(qss_ctx *c) {
   std::swap(c->C[QSS_PSI], c->P[QSS_PSI]);
 }
 
-int do_layer_step(qss_ctx *c, int slice, int isplit, double *res_dev) {
+// ---- particle snapshots (checkpoints and the adjoint tape) -------------------
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct PartView {
+  double *y, *z, *px, *py, *pz;
+  int32_t *cell;
+  int64_t *pid;
+  int32_t *tstart;
+};
+
+size_t part_bytes(const qss_ctx *c) {
+  const size_t np = c->np_cap, nt = (size_t)c->ntj * c->ntk + 1;
+  return 5 * align256(np * 8) + align256(np * 4) + align256(np * 8) + align256(nt * 4);
+}
+
+PartView part_view(const qss_ctx *c, char *base) {
+  const size_t np = c->np_cap;
+  PartView v;
+  size_t o = 0;
+  double **d[5] = {&v.y, &v.z, &v.px, &v.py, &v.pz};
+  for (auto p : d) { *p = reinterpret_cast<double *>(base + o); o += align256(np * 8); }
+  v.cell = reinterpret_cast<int32_t *>(base + o); o += align256(np * 4);
+  v.pid = reinterpret_cast<int64_t *>(base + o); o += align256(np * 8);
+  v.tstart = reinterpret_cast<int32_t *>(base + o);
+  return v;
+}
+
+// to = 0: context -> snapshot, 1: snapshot -> context
+int copy_particles(qss_ctx *c, const PartView &v, int to_ctx) {
+  const int b = c->cur;
+  const size_t n = c->np_cap, nt = (size_t)c->ntj * c->ntk + 1;
+  void *ctx[8] = {c->py_[b], c->pz_[b], c->ppx[b], c->ppy[b], c->ppz[b], c->pcell[b],
+                  c->pid[b], c->tile_start[b]};
+  void *snap[8] = {v.y, v.z, v.px, v.py, v.pz, v.cell, v.pid, v.tstart};
+  const size_t bytes[8] = {n * 8, n * 8, n * 8, n * 8, n * 8, n * 4, n * 8, nt * 4};
+  for (int i = 0; i < 8; ++i)
+    HIPCHECK(hipMemcpyAsync(to_ctx ? ctx[i] : snap[i], to_ctx ? snap[i] : ctx[i], bytes[i],
+                            hipMemcpyDeviceToDevice, c->stream));
+  return QSS_OK;
+}
+
+// tape slot of one layer: particles at the layer start + the push field of
+// each of the n_iter+1 passes
+size_t tape_bytes(const qss_ctx *c) {
+  return part_bytes(c) + (size_t)(c->p.n_iter + 1) * kPushComp * align256(c->ncell * 8);
+}
+double *tape_field(const qss_ctx *c, char *slot, int pass) {
+  return reinterpret_cast<double *>(slot + part_bytes(c) +
+                                    (size_t)pass * kPushComp * align256(c->ncell * 8));
+}
+
+// tape != null: record the layer's start particles and every pass's push
+// field (the reverse sweep replays the pushes from them)
+int do_layer_step(qss_ctx *c, int slice, int isplit, double *res_dev, char *tape = nullptr) {
+  if (tape) CHECK(copy_particles(c, part_view(c, tape), 0));
+  auto rec = [&](int pass) -> int {
+    if (tape)
+      HIPCHECK(hipMemcpyAsync(tape_field(c, tape, pass), c->F,
+                              kPushComp * c->ncell * sizeof(double), hipMemcpyDeviceToDevice,
+                              c->stream));
+    return QSS_OK;
+  };
   CHECK(do_beam(c, slice, isplit));
   CHECK(do_guess(c, slice == 0));
   for (int it = 0; it < c->p.n_iter; ++it) {
+    CHECK(rec(it));
     CHECK(do_push_deposit(c, 0));
     CHECK(do_iterate(c, 0, nullptr));
   }
+  CHECK(rec(c->p.n_iter));
   CHECK(do_push_deposit(c, 1));
   CHECK(do_iterate(c, 1, res_dev));
   swap_c_p(c);
+  return QSS_OK;
+}
+
+// checkpoint: particles + P planes + spectral state at a layer start
+size_t chk_bytes(const qss_ctx *c) {
+  return part_bytes(c) + (size_t)(QSS_NFIELD + SP_N) * align256(c->ncell * 8);
+}
+int checkpoint(qss_ctx *c, char *base, int restore) {
+  CHECK(copy_particles(c, part_view(c, base), restore));
+  const size_t pl = align256(c->ncell * 8), bytes = c->ncell * sizeof(double);
+  char *p = base + part_bytes(c);
+  for (int f = 0; f < QSS_NFIELD; ++f, p += pl)
+    HIPCHECK(hipMemcpyAsync(restore ? (void *)c->P[f] : (void *)p,
+                            restore ? (const void *)p : (const void *)c->P[f], bytes,
+                            hipMemcpyDeviceToDevice, c->stream));
+  for (int s = 0; s < SP_N; ++s, p += pl)
+    HIPCHECK(hipMemcpyAsync(restore ? (void *)c->S[s] : (void *)p,
+                            restore ? (const void *)p : (const void *)c->S[s], bytes,
+                            hipMemcpyDeviceToDevice, c->stream));
+  return QSS_OK;
+}
+
+void adj_free(qss_ctx *c) {
+  AdjState &a = c->adj;
+  for (auto &p : a.A) dfree(p);
+  dfree(a.pbarEB); dfree(a.fbar); dfree(a.jbar); dfree(a.bbar);
+  dfree(a.pbar[0]); dfree(a.pbar[1]); dfree(a.fwin); dfree(a.qbeam);
+  dfree(a.chk); dfree(a.tape); dfree(a.exbar);
+  a = AdjState();
+}
+
+int adj_check(const qss_ctx *c) {
+  if (!c->tiled || c->p.precision != QSS_PREC_FP64 || c->p.shape != QSS_SHAPE_CIC ||
+      c->p.nx_split != 1) {
+    qss_set_error("the adjoint sweep needs FP64, CIC, nx_split = 1 and ny, nz multiples of 16");
+    return QSS_E_ARG;
+  }
+  return QSS_OK;
+}
+
+int adj_alloc(qss_ctx *c, int nrun, int K) {
+  AdjState &a = c->adj;
+  const int nchk = (nrun + K - 1) / K;
+  if (a.chk && a.K == K && a.nrun == nrun) return QSS_OK;
+  adj_free(c);
+  const long nc = c->ncell, np = c->np_cap, nt = (long)c->ntj * c->ntk;
+  int r;
+  for (int s = 0; s < SP_N; ++s)
+    if ((r = dalloc(&a.A[s], nc)) != QSS_OK) return r;
+  a.chk_stride = chk_bytes(c);
+  a.tape_stride = tape_bytes(c);
+  if ((r = dalloc(&a.pbarEB, nc * kPushComp)) || (r = dalloc(&a.fbar, nc * kPushComp)) ||
+      (r = dalloc(&a.jbar, nc * 3)) || (r = dalloc(&a.bbar, nc * 2)) ||
+      (r = dalloc(&a.pbar[0], np * 5)) || (r = dalloc(&a.pbar[1], np * 5)) ||
+      (r = dalloc(&a.fwin, nt * 289 * 6)) || (r = dalloc(&a.qbeam, (long)c->p.nslices * 2)) ||
+      (r = dalloc(&a.chk, a.chk_stride * nchk)) || (r = dalloc(&a.tape, a.tape_stride * K)) ||
+      (r = dalloc(&a.exbar, (size_t)nrun * nc)))
+    return r;
+  // the zero fills ran on the null stream
+  HIPCHECK(hipDeviceSynchronize());
+  a.K = K;
+  a.nrun = nrun;
+  a.nchk = nchk;
+  return QSS_OK;
+}
+
+// One push pass of the reverse sweep: adjoint push, windows -> field adjoint,
+// P-layer share of it, and the C-layer share into the spectral E,B adjoints.
+int adj_push_pass(qss_ctx *c, char *slot, int pass, int full, int cur_adj) {
+  AdjState &s = c->adj;
+  const PartView v = part_view(c, slot);
+  AdjTileArgs g;
+  g.y = v.y; g.z = v.z; g.px = v.px; g.py = v.py; g.pz = v.pz;
+  g.cell = v.cell; g.pid = v.pid; g.tile_start = v.tstart;
+  g.ntj = c->ntj; g.ntk = c->ntk; g.ny = c->ny; g.nz = c->nz;
+  g.hx = c->hxs; g.hy = c->hy; g.hz = c->hz; g.weight = c->pweight;
+  g.F = tape_field(c, slot, pass);
+  g.Jb = s.jbar;
+  g.full = full;
+  g.pin = s.pbar[cur_adj];
+  g.pout = s.pbar[cur_adj ^ 1];
+  g.np = c->np_cap;
+  g.fwin = s.fwin;
+  g.counters = c->d_counters;
+  {
+    Timed t(c, KC_PUSH);
+    HIPCHECK(launch_push_tile_adj(c, g));
+  }
+  {
+    Timed t(c, KC_REDUCE);
+    HIPCHECK(launch_fbar_combine(c, s.fwin, s.fbar));
+    HIPCHECK(launch_axpy(s.pbarEB, s.fbar, 0.5, c->ncell * kPushComp, c->stream));
+  }
+  const int kz[6] = {QSS_KIND_C, QSS_KIND_C, QSS_KIND_S, QSS_KIND_S, QSS_KIND_S, QSS_KIND_C};
+  const int ky[6] = {QSS_KIND_C, QSS_KIND_S, QSS_KIND_C, QSS_KIND_S, QSS_KIND_C, QSS_KIND_S};
+  const double *in[6];
+  double *out[6];
+  for (int q = 0; q < 6; ++q) {
+    in[q] = s.fbar + q * c->ncell;
+    out[q] = s.A[SP_EX + q];
+  }
+  return transform_batch(c, 6, kz, ky, in, out, 0.5 / (4.0 * c->ny * c->nz), nullptr,
+                         nullptr, nullptr, 1.0);
+}
+
+// spectral J adjoints -> real-space deposit adjoints; the J spectra were
+// overwritten by the forward transform, so their adjoints restart at zero
+int adj_deposit_transform(qss_ctx *c) {
+  AdjState &s = c->adj;
+  const int kz[3] = {QSS_KIND_C, QSS_KIND_C, QSS_KIND_S};
+  const int ky[3] = {QSS_KIND_C, QSS_KIND_S, QSS_KIND_C};
+  const double *in[3] = {s.A[SP_JX], s.A[SP_JY], s.A[SP_JZ]};
+  double *out[3] = {s.jbar, s.jbar + c->ncell, s.jbar + 2 * c->ncell};
+  CHECK(transform_batch(c, 3, kz, ky, in, out, 1.0, nullptr, nullptr, nullptr));
+  Timed t(c, KC_SPEC);
+  for (int q = 0; q < 3; ++q) HIPCHECK(launch_fill_f64(s.A[SP_JX + q], c->ncell, 0.0, c->stream));
+  return QSS_OK;
+}
+
+// Reverse of do_layer_step for layer `slice` (tape slot), the loss adjoint of
+// the layer's final Ex plane being exbar (device, may be null).
+int adj_layer_step(qss_ctx *c, int slice, char *slot, const double *exbar, int &cur_adj) {
+  AdjState &s = c->adj;
+  const long nc = c->ncell;
+  const double sinv = 1.0 / (4.0 * c->ny * c->nz);
+  // final fields of the layer: loss + the next layer's P-layer gathers
+  if (exbar) HIPCHECK(launch_axpy(s.pbarEB, exbar, 1.0, nc, c->stream));
+  {
+    const int kz[6] = {QSS_KIND_C, QSS_KIND_C, QSS_KIND_S, QSS_KIND_S, QSS_KIND_S, QSS_KIND_C};
+    const int ky[6] = {QSS_KIND_C, QSS_KIND_S, QSS_KIND_C, QSS_KIND_S, QSS_KIND_C, QSS_KIND_S};
+    const double *in[6];
+    double *out[6];
+    for (int q = 0; q < 6; ++q) {
+      in[q] = s.pbarEB + q * nc;
+      out[q] = s.A[SP_EX + q];
+    }
+    CHECK(transform_batch(c, 6, kz, ky, in, out, sinv, nullptr, nullptr, nullptr, 1.0));
+    HIPCHECK(launch_fill_f64(s.pbarEB, nc * kPushComp, 0.0, c->stream));
+  }
+  // final Iterate, full-step push
+  {
+    Timed t(c, KC_SPEC);
+    HIPCHECK(launch_iterate_adj(c, s.A, 1));
+  }
+  CHECK(adj_deposit_transform(c));
+  CHECK(adj_push_pass(c, slot, c->p.n_iter, 1, cur_adj));
+  // corrector passes in reverse
+  for (int it = c->p.n_iter - 1; it >= 0; --it) {
+    {
+      Timed t(c, KC_SPEC);
+      HIPCHECK(launch_iterate_adj(c, s.A, 0));
+    }
+    CHECK(adj_deposit_transform(c));
+    CHECK(adj_push_pass(c, slot, it, 0, cur_adj));
+  }
+  {
+    Timed t(c, KC_SPEC);
+    HIPCHECK(launch_guess_adj(c, s.A, slice == 0));
+  }
+  // beam spectra -> beam planes -> profile derivative
+  {
+    const int kz[2] = {QSS_KIND_C, QSS_KIND_C}, ky[2] = {QSS_KIND_C, QSS_KIND_C};
+    const double *in[2] = {s.A[SP_RHOB], s.A[SP_JXB]};
+    double *out[2] = {s.bbar, s.bbar + nc};
+    CHECK(transform_batch(c, 2, kz, ky, in, out, 1.0, nullptr, nullptr, nullptr));
+    Timed t(c, KC_BEAM);
+    HIPCHECK(launch_beam_adj(c, slice, s.bbar, s.bbar + nc, s.qbeam));
+    HIPCHECK(launch_fill_f64(s.A[SP_RHOB], nc, 0.0, c->stream));
+    HIPCHECK(launch_fill_f64(s.A[SP_JXB], nc, 0.0, c->stream));
+  }
+  // this layer's beam P spectrum is the previous layer's beam spectrum
+  std::swap(s.A[SP_RHOB], s.A[SP_RHOBP]);
+  cur_adj ^= 1;
   return QSS_OK;
 }
 
@@ -451,11 +689,6 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
     for (int i = 0; i < 4; ++i) {
       const long n = i < 2 ? c->ny : c->nz;
       if ((r = dalloc(&c->Ts[i], (size_t)c->npc * n * n)) != QSS_OK) return fail(r);
-      const double *src = c->d_T[i];
-      if (launch_split_bf16(&src, &c->Ts[i], 1, n * n, c->npc, c->stream) != hipSuccess) {
-        qss_set_error("table split failed");
-        return fail(QSS_E_HIP);
-      }
     }
     for (int b = 0; b < kGemmMaxBatch; ++b)
       if ((r = dalloc(&c->xs[b], (size_t)c->npc * nc)) != QSS_OK ||
@@ -502,6 +735,20 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
   for (int f = 0; f < QSS_NFIELD; ++f)
     if (history_mask & (1u << f))
       if ((r = dalloc(&c->hist[f], (size_t)nc * p->nslices)) != QSS_OK) return fail(r);
+  // the zero fills of the allocations ran on the null stream, which does not
+  // order with the context's non-blocking stream: wait for them
+  if (hipDeviceSynchronize() != hipSuccess) {
+    qss_set_error("allocation fill failed");
+    return fail(QSS_E_HIP);
+  }
+  for (int i = 0; c->npc && i < 4; ++i) {
+    const long n = i < 2 ? c->ny : c->nz;
+    const double *src = c->d_T[i];
+    if (launch_split_bf16(&src, &c->Ts[i], 1, n * n, c->npc, c->stream) != hipSuccess) {
+      qss_set_error("table split failed");
+      return fail(QSS_E_HIP);
+    }
+  }
   if ((r = do_reset(c)) != QSS_OK) return fail(r);
   if (hipStreamSynchronize(c->stream) != hipSuccess) {
     qss_set_error("seeding failed");
@@ -515,6 +762,7 @@ int qss_destroy(qss_ctx *c) {
   if (!c) return QSS_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  adj_free(c);
   for (auto &p : c->d_T) dfree(p);
   for (int f = 0; f < QSS_NFIELD; ++f) { dfree(c->C[f]); dfree(c->P[f]); dfree(c->hist[f]); }
   for (auto &p : c->S) dfree(p);
@@ -600,6 +848,122 @@ int qss_sweep(qss_ctx *c, double *residuals) {
   HIPCHECK(hipSetDevice(c->device));
   CHECK(do_reset(c));
   return qss_sweep_range(c, 0, c->p.nslices, residuals);
+}
+
+int qss_set_bins(qss_ctx *c, const double *bins, int nbins) {
+  if (!c || !bins || nbins <= 0 || nbins != c->p.nbins) {
+    qss_set_error("bins: the count must equal the nbins given at create");
+    return QSS_E_ARG;
+  }
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->bins.assign(bins, bins + nbins);
+  HIPCHECK(hipMemcpy(c->d_bins, bins, (size_t)nbins * sizeof(double), hipMemcpyHostToDevice));
+  return QSS_OK;
+}
+
+int qss_copy_history(qss_ctx *c, int field, int first, int count, double *dst_dev) {
+  if (!c || field < 0 || field >= QSS_NFIELD || !c->hist[field] || first < 0 || count < 0 ||
+      first + count > c->p.nslices || (!dst_dev && count > 0)) {
+    qss_set_error("copy_history: bad field/range or field not stored");
+    return QSS_E_ARG;
+  }
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipMemcpyAsync(dst_dev, c->hist[field] + (size_t)first * c->ncell,
+                          (size_t)count * c->ncell * sizeof(double), hipMemcpyDeviceToDevice,
+                          c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return QSS_OK;
+}
+
+int qss_sweep_checkpointed(qss_ctx *c, int nrun, int every, double *residuals) {
+  if (!c || nrun < 1 || nrun > c->p.nslices || every < 1) {
+    qss_set_error("checkpointed sweep: bad layer count or interval");
+    return QSS_E_ARG;
+  }
+  CHECK(adj_check(c));
+  HIPCHECK(hipSetDevice(c->device));
+  CHECK(adj_alloc(c, nrun, every));
+  for (auto &u : c->ev_used) u = 0;
+  c->gemm_flops = 0;
+  if (!c->t0) { HIPCHECK(hipEventCreate(&c->t0)); HIPCHECK(hipEventCreate(&c->t1)); }
+  HIPCHECK(hipEventRecord(c->t0, c->stream));
+  CHECK(do_reset(c));
+  AdjState &a = c->adj;
+  for (int s = 0; s < nrun; ++s) {
+    if (s % every == 0) CHECK(checkpoint(c, a.chk + (size_t)(s / every) * a.chk_stride, 0));
+    CHECK(do_layer_step(c, s, 0, c->d_res_hist + s));
+    for (int f = 0; f < QSS_NFIELD; ++f)
+      if (c->hist[f])
+        HIPCHECK(hipMemcpyAsync(c->hist[f] + (size_t)s * c->ncell, c->P[f],
+                                c->ncell * sizeof(double), hipMemcpyDeviceToDevice,
+                                c->stream));
+  }
+  HIPCHECK(hipEventRecord(c->t1, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  if (residuals)
+    HIPCHECK(hipMemcpy(residuals, c->d_res_hist, (size_t)nrun * sizeof(double),
+                       hipMemcpyDeviceToHost));
+  return check_status(c);
+}
+
+int qss_sweep_vjp(qss_ctx *c, const double *ex_bar, int on_host, double *grad_bins) {
+  if (!c || !grad_bins) return QSS_E_ARG;
+  AdjState &a = c->adj;
+  if (!a.chk || a.nrun < 1) {
+    qss_set_error("sweep_vjp needs a preceding qss_sweep_checkpointed");
+    return QSS_E_STATE;
+  }
+  const double *ex_bar_dev = ex_bar;
+  if (ex_bar && on_host) {
+    HIPCHECK(hipMemcpy(a.exbar, ex_bar, (size_t)a.nrun * c->ncell * sizeof(double),
+                       hipMemcpyHostToDevice));
+    ex_bar_dev = a.exbar;
+  }
+  if (c->p.nbins < 1) {
+    qss_set_error("sweep_vjp: the driver has no piecewise-constant profile (nbins = 0)");
+    return QSS_E_ARG;
+  }
+  HIPCHECK(hipSetDevice(c->device));
+  for (auto &u : c->ev_used) u = 0;
+  c->gemm_flops = 0;
+  if (!c->t0) { HIPCHECK(hipEventCreate(&c->t0)); HIPCHECK(hipEventCreate(&c->t1)); }
+  HIPCHECK(hipEventRecord(c->t0, c->stream));
+  const long nc = c->ncell;
+  for (int s = 0; s < SP_N; ++s) HIPCHECK(launch_fill_f64(a.A[s], nc, 0.0, c->stream));
+  HIPCHECK(launch_fill_f64(a.pbarEB, nc * kPushComp, 0.0, c->stream));
+  HIPCHECK(launch_fill_f64(a.pbar[0], c->np_cap * 5, 0.0, c->stream));
+  HIPCHECK(launch_fill_f64(a.pbar[1], c->np_cap * 5, 0.0, c->stream));
+  HIPCHECK(launch_fill_f64(a.qbeam, (long)c->p.nslices * 2, 0.0, c->stream));
+  int cur_adj = 0;
+  for (int seg = a.nchk - 1; seg >= 0; --seg) {
+    const int s0 = seg * a.K, s1 = std::min(a.nrun, s0 + a.K);
+    // recompute the segment from its checkpoint, recording the tape
+    CHECK(checkpoint(c, a.chk + (size_t)seg * a.chk_stride, 1));
+    for (int s = s0; s < s1; ++s)
+      CHECK(do_layer_step(c, s, 0, c->d_res_hist + s, a.tape + (size_t)(s - s0) * a.tape_stride));
+    for (int s = s1 - 1; s >= s0; --s)
+      CHECK(adj_layer_step(c, s, a.tape + (size_t)(s - s0) * a.tape_stride,
+                           ex_bar_dev ? ex_bar_dev + (size_t)s * nc : nullptr, cur_adj));
+  }
+  HIPCHECK(hipEventRecord(c->t1, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  CHECK(check_status(c));
+  std::vector<double> q((size_t)a.nrun * 2);
+  HIPCHECK(hipMemcpy(q.data(), a.qbeam, q.size() * sizeof(double), hipMemcpyDeviceToHost));
+  // layer s receives the profile at xi = (s+1) hx and (s+2) hx (beam.hip)
+  const int nb = c->p.nbins, L = c->p.nslices;
+  for (int b = 0; b < nb; ++b) grad_bins[b] = 0.0;
+  for (int s = 0; s < a.nrun; ++s)
+    for (int side = 0; side < 2; ++side) {
+      const int ci = (L - 1 - s) - side;
+      if (ci < 0 || ci > L) continue;
+      const double xi = (double)(L - ci) * c->p.hx;
+      const double t = (xi - c->p.bin_xi0) / (c->p.bin_xi1 - c->p.bin_xi0) * nb;
+      if (!(t >= 0.0) || t >= nb) continue;
+      grad_bins[(int)t] += q[(size_t)s * 2 + side];
+    }
+  return QSS_OK;
 }
 
 int qss_layer_step(qss_ctx *c, int slice, int isplit, double *residual) {

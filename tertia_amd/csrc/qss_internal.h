@@ -44,6 +44,7 @@ struct GemmBatch {
 struct GemmArgs {
   int M, N, K;
   double alpha;
+  double beta;                  // C = alpha A.B + beta C (fp64 store)
   int nbatch;
   int tout;                     // bf16 modes: store Cts instead of C
   double *F;                    // push field planes (kPushComp x M*N)
@@ -59,6 +60,44 @@ int gemm_pieces(int prec);   // bf16 pieces per operand element (0: not split)
 // out[b] (npc planes of n bf16) = pieces of in[b] (n doubles), b < nb
 hipError_t launch_split_bf16(const double *const *in, uint16_t *const *out, int nb, long n,
                              int npc, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// adjoint sweep (tertia_amd/csrc/adjoint.hip, qss_sweep_vjp)
+// ---------------------------------------------------------------------------
+struct AdjTileArgs {
+  const double *y, *z, *px, *py, *pz;   // particles at the layer start (taped)
+  const int32_t *cell;
+  const int64_t *pid;
+  const int32_t *tile_start;
+  int ntj, ntk, ny, nz;
+  double hx, hy, hz, weight;
+  const double *F;      // the pass's push field (6 planes, taped)
+  const double *Jb;     // deposit adjoint planes of Jx, Jy, Jz
+  int full;
+  const double *pin;    // full step: adjoint of the new state by id (5 x np)
+  double *pout;         // adjoint of the start state by id (5 x np); full: =, else +=
+  long np;
+  double *fwin;         // per tile 17x17 nodes x 6 field adjoints
+  int32_t *counters;
+};
+
+// buffers of the checkpointed forward and of the reverse sweep
+struct AdjState {
+  int K = 0, nrun = 0, nchk = 0;
+  double *A[SP_N] = {};        // adjoint of the spectral state
+  double *pbarEB = nullptr;    // 6 planes: adjoint of the final E,B of a layer
+  double *fbar = nullptr;      // 6 planes: adjoint of a push field
+  double *jbar = nullptr;      // 3 planes: adjoint of the deposited Jx,Jy,Jz
+  double *bbar = nullptr;      // 2 planes: adjoint of the beam rho, Jx planes
+  double *pbar[2] = {};        // particle-state adjoints by id (5 x np_cap)
+  double *fwin = nullptr;      // ntiles x 289 x 6
+  double *qbeam = nullptr;     // per layer and x-cell side: d(loss)/d(profile)
+  char *chk = nullptr;         // checkpoints (state at layer starts)
+  size_t chk_stride = 0;
+  char *tape = nullptr;        // per layer of a segment: particles + push fields
+  size_t tape_stride = 0;
+  double *exbar = nullptr;     // loss adjoint of the Ex history given on the host
+};
 
 // ---------------------------------------------------------------------------
 // context
@@ -127,6 +166,8 @@ struct qss_ctx {
   uint32_t hmask = 0;
   double *hist[QSS_NFIELD] = {};
 
+  AdjState adj;
+
   // transform GEMM tile shape per (pass, batch): pass 0 = X.T_y, 1 = T_z.Y
   int gemm_var[2][kGemmMaxBatch + 1] = {};
   int gemm_force = -1;    // >= 0: use this variant for every launch
@@ -159,6 +200,15 @@ hipError_t launch_push_tile(qss_ctx *c, int full);
 hipError_t launch_combine(qss_ctx *c);
 hipError_t launch_tile_reorder(qss_ctx *c);
 hipError_t launch_seed_tiled(qss_ctx *c);
+hipError_t launch_axpy(double *y, const double *x, double a, long n, hipStream_t s);
+// adjoints of the spectral updates on the adjoint state A (flag: first layer
+// for the Guess, final pass for the Iterate)
+hipError_t launch_guess_adj(qss_ctx *c, double *const *A, int first_layer);
+hipError_t launch_iterate_adj(qss_ctx *c, double *const *A, int final_pass);
+hipError_t launch_push_tile_adj(qss_ctx *c, const AdjTileArgs &a);
+hipError_t launch_fbar_combine(qss_ctx *c, const double *fwin, double *out);
+hipError_t launch_beam_adj(qss_ctx *c, int slice, const double *rbar, const double *jbar,
+                           double *q);
 
 // host helpers
 void qss_set_error(const std::string &m);

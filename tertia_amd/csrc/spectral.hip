@@ -13,6 +13,7 @@
 // 7 transforms per layer with identical results up to rounding.
 
 #include "qss_internal.h"
+#include "spectral_mode.h"
 
 namespace {
 
@@ -45,87 +46,166 @@ __device__ inline void wavenumbers(const SpecArgs &a, int j, int k, double &aky,
   ak2 = aky * aky + akz * akz;
 }
 
+__device__ inline qss_spec::Mode mode_of(const SpecArgs &a, int j, int k) {
+  qss_spec::Mode m;
+  wavenumbers(a, j, k, m.aky, m.akz, m.ak2);
+  m.dens = a.dens;
+  m.hx = a.hx;
+  return m;
+}
+
 __global__ __launch_bounds__(NT) void k_guess(SpecArgs a) {
+  using namespace qss_spec;
   const long n1 = (long)blockIdx.x * NT + threadIdx.x;
   if (n1 >= (long)a.ny * a.nz) return;
-  const int j = (int)(n1 % a.ny), k = (int)(n1 / a.ny);
-  double aky, akz, ak2;
-  wavenumbers(a, j, k, aky, akz, ak2);
+  const Mode m = mode_of(a, (int)(n1 % a.ny), (int)(n1 / a.ny));
   double **S = a.S;
-  const double dens = a.dens, hx = a.hx;
   // P spectra = the spectra left by the previous Iterate (or zero after a
   // reset); the beam P spectrum is the previous beam spectrum (:637-646)
-  const double rp = S[SP_RHO][n1];
-  const double jxp = S[SP_JX][n1];
-  const double jyp = S[SP_JY][n1];
-  const double jzp = S[SP_JZ][n1];
-  const double exp_ = S[SP_EX][n1];
-  const double eyp = S[SP_EY][n1];
-  const double ezp = S[SP_EZ][n1];
-  S[SP_RHOP][n1] = rp;
-  S[SP_JXP][n1] = jxp;
-  S[SP_JYP][n1] = jyp;
-  S[SP_JZP][n1] = jzp;
-  S[SP_EXP][n1] = exp_;
-  S[SP_EYP][n1] = eyp;
-  S[SP_EZP][n1] = ezp;
-  const double rb = S[SP_RHOB][n1];
-  const double rbp = S[SP_RHOBP][n1];
-  const double jxb = S[SP_JXB][n1];
-  const double propagator = (4. - dens * hx * hx) / (4. + dens * hx * hx);
-  const double denominator = 4. + dens * hx * hx;
-  const double rho = propagator * rp - (rb + rbp) * dens * hx * hx / denominator +
-                     4. * exp_ * hx * (ak2 + dens) / denominator;
-  const double ex = propagator * exp_ -
-                    2. * (2. * rp + rb + rbp) * dens * hx / ((ak2 + dens) * denominator);
-  const double ey = -eyp + aky * (rb + rbp + rp + rho) / (dens + ak2);
-  const double ez = -ezp + akz * (rb + rbp + rp + rho) / (dens + ak2);
-  const double jy = jyp + hx * dens * (ey + eyp) / 2.;
-  const double jz = jzp + hx * dens * (ez + ezp) / 2.;
-  const double jx = jxp + hx * dens * (ex + exp_) / 2.;
-  S[SP_RHO][n1] = rho;
-  S[SP_EX][n1] = ex;
-  S[SP_EY][n1] = ey;
-  S[SP_EZ][n1] = ez;
-  S[SP_JX][n1] = jx;
-  S[SP_JY][n1] = jy;
-  S[SP_JZ][n1] = jz;
-  S[SP_BX][n1] = -aky / ak2 * jz + akz / ak2 * jy;
-  S[SP_BY][n1] = (-akz * (jx + jxb) + dens * ez) / ak2;
-  S[SP_BZ][n1] = (aky * (jx + jxb) - dens * ey) / ak2;
+  double in[GI_N], out[GO_N];
+  in[GI_RP] = S[SP_RHO][n1];
+  in[GI_JXP] = S[SP_JX][n1];
+  in[GI_JYP] = S[SP_JY][n1];
+  in[GI_JZP] = S[SP_JZ][n1];
+  in[GI_EXP] = S[SP_EX][n1];
+  in[GI_EYP] = S[SP_EY][n1];
+  in[GI_EZP] = S[SP_EZ][n1];
+  in[GI_RB] = S[SP_RHOB][n1];
+  in[GI_RBP] = S[SP_RHOBP][n1];
+  in[GI_JXB] = S[SP_JXB][n1];
+  S[SP_RHOP][n1] = in[GI_RP];
+  S[SP_JXP][n1] = in[GI_JXP];
+  S[SP_JYP][n1] = in[GI_JYP];
+  S[SP_JZP][n1] = in[GI_JZP];
+  S[SP_EXP][n1] = in[GI_EXP];
+  S[SP_EYP][n1] = in[GI_EYP];
+  S[SP_EZP][n1] = in[GI_EZP];
+  guess(m, in, out);
+  S[SP_RHO][n1] = out[GO_RHO];
+  S[SP_EX][n1] = out[GO_EX];
+  S[SP_EY][n1] = out[GO_EY];
+  S[SP_EZ][n1] = out[GO_EZ];
+  S[SP_JX][n1] = out[GO_JX];
+  S[SP_JY][n1] = out[GO_JY];
+  S[SP_JZ][n1] = out[GO_JZ];
+  S[SP_BX][n1] = out[GO_BX];
+  S[SP_BY][n1] = out[GO_BY];
+  S[SP_BZ][n1] = out[GO_BZ];
 }
 
 __global__ __launch_bounds__(NT) void k_iterate(SpecArgs a) {
+  using namespace qss_spec;
   const long n1 = (long)blockIdx.x * NT + threadIdx.x;
   if (n1 >= (long)a.ny * a.nz) return;
-  const int j = (int)(n1 % a.ny), k = (int)(n1 / a.ny);
-  double aky, akz, ak2;
-  wavenumbers(a, j, k, aky, akz, ak2);
+  const Mode m = mode_of(a, (int)(n1 % a.ny), (int)(n1 / a.ny));
   double **S = a.S;
-  const double dens = a.dens, hx = a.hx;
-  const double rp = S[SP_RHOP][n1];
-  const double jx = S[SP_JX][n1];
-  const double jy = S[SP_JY][n1];
-  const double jz = S[SP_JZ][n1];
-  const double rho = rp + jx - S[SP_JXP][n1] -
-                     hx * (aky * (jy + S[SP_JYP][n1]) + akz * (jz + S[SP_JZP][n1])) / 2.;
-  const double rb = S[SP_RHOB][n1];
-  const double rbp = S[SP_RHOBP][n1];
-  const double eyp = S[SP_EYP][n1];
-  const double ezp = S[SP_EZP][n1];
-  const double rho_old = S[SP_RHO][n1];
-  const double newEx = -(aky * jy + akz * jz) / ak2;
-  const double newEy = -eyp + aky * (rb + rbp + rp + rho_old) / (dens + ak2);
-  const double newEz = -ezp + akz * (rb + rbp + rp + rho_old) / (dens + ak2);
-  const double jxb = S[SP_JXB][n1];
-  S[SP_EY][n1] = newEy;
-  S[SP_EZ][n1] = newEz;
-  S[SP_RHO][n1] = rho;
-  S[SP_EX][n1] = newEx;
-  S[SP_BX][n1] = -aky / ak2 * jz + akz / ak2 * jy;
-  S[SP_BY][n1] = (-akz * (jx + jxb) + dens * newEz) / ak2;
-  S[SP_BZ][n1] = (aky * (jx + jxb) - dens * newEy) / ak2;
-  if (a.flag) S[SP_PSI][n1] = (rho - jx) / ak2;
+  double in[II_N], out[IO_N];
+  in[II_RP] = S[SP_RHOP][n1];
+  in[II_JX] = S[SP_JX][n1];
+  in[II_JY] = S[SP_JY][n1];
+  in[II_JZ] = S[SP_JZ][n1];
+  in[II_JXP] = S[SP_JXP][n1];
+  in[II_JYP] = S[SP_JYP][n1];
+  in[II_JZP] = S[SP_JZP][n1];
+  in[II_RB] = S[SP_RHOB][n1];
+  in[II_RBP] = S[SP_RHOBP][n1];
+  in[II_EYP] = S[SP_EYP][n1];
+  in[II_EZP] = S[SP_EZP][n1];
+  in[II_RHO_OLD] = S[SP_RHO][n1];
+  in[II_JXB] = S[SP_JXB][n1];
+  iterate(m, in, out);
+  S[SP_EY][n1] = out[IO_EY];
+  S[SP_EZ][n1] = out[IO_EZ];
+  S[SP_RHO][n1] = out[IO_RHO];
+  S[SP_EX][n1] = out[IO_EX];
+  S[SP_BX][n1] = out[IO_BX];
+  S[SP_BY][n1] = out[IO_BY];
+  S[SP_BZ][n1] = out[IO_BZ];
+  if (a.flag) S[SP_PSI][n1] = out[IO_PSI];
+}
+
+// ---- adjoints (A = adjoint of the spectral state, same ids) ---------------
+// Reverse of k_guess: the written entries' adjoints are consumed (zeroed)
+// and M^T of them is added to the read entries; an entry both read and
+// written (RHO, JX, ..., EZ) ends with the adjoint of its value before the
+// Guess, the P copies' adjoints included.
+__global__ __launch_bounds__(NT) void k_guess_adj(SpecArgs a) {
+  using namespace qss_spec;
+  const long n1 = (long)blockIdx.x * NT + threadIdx.x;
+  if (n1 >= (long)a.ny * a.nz) return;
+  const Mode m = mode_of(a, (int)(n1 % a.ny), (int)(n1 / a.ny));
+  double **A = a.S;
+  double ob[GO_N], ib[GI_N];
+  ob[GO_RHO] = A[SP_RHO][n1];
+  ob[GO_EX] = A[SP_EX][n1];
+  ob[GO_EY] = A[SP_EY][n1];
+  ob[GO_EZ] = A[SP_EZ][n1];
+  ob[GO_JX] = A[SP_JX][n1];
+  ob[GO_JY] = A[SP_JY][n1];
+  ob[GO_JZ] = A[SP_JZ][n1];
+  ob[GO_BX] = A[SP_BX][n1];
+  ob[GO_BY] = A[SP_BY][n1];
+  ob[GO_BZ] = A[SP_BZ][n1];
+  guess_adj(m, ob, ib);
+  A[SP_RHO][n1] = ib[GI_RP] + A[SP_RHOP][n1];
+  A[SP_JX][n1] = ib[GI_JXP] + A[SP_JXP][n1];
+  A[SP_JY][n1] = ib[GI_JYP] + A[SP_JYP][n1];
+  A[SP_JZ][n1] = ib[GI_JZP] + A[SP_JZP][n1];
+  A[SP_EX][n1] = ib[GI_EXP] + A[SP_EXP][n1];
+  A[SP_EY][n1] = ib[GI_EYP] + A[SP_EYP][n1];
+  A[SP_EZ][n1] = ib[GI_EZP] + A[SP_EZP][n1];
+  A[SP_RHOB][n1] += ib[GI_RB];
+  A[SP_RHOBP][n1] = a.flag ? 0.0 : A[SP_RHOBP][n1] + ib[GI_RBP];   // flag: first layer
+  A[SP_JXB][n1] += ib[GI_JXB];
+  A[SP_RHOP][n1] = 0.0;
+  A[SP_JXP][n1] = 0.0;
+  A[SP_JYP][n1] = 0.0;
+  A[SP_JZP][n1] = 0.0;
+  A[SP_EXP][n1] = 0.0;
+  A[SP_EYP][n1] = 0.0;
+  A[SP_EZP][n1] = 0.0;
+  A[SP_BX][n1] = 0.0;
+  A[SP_BY][n1] = 0.0;
+  A[SP_BZ][n1] = 0.0;
+}
+
+// Reverse of k_iterate (flag: final pass, psi written)
+__global__ __launch_bounds__(NT) void k_iterate_adj(SpecArgs a) {
+  using namespace qss_spec;
+  const long n1 = (long)blockIdx.x * NT + threadIdx.x;
+  if (n1 >= (long)a.ny * a.nz) return;
+  const Mode m = mode_of(a, (int)(n1 % a.ny), (int)(n1 / a.ny));
+  double **A = a.S;
+  double ob[IO_N], ib[II_N];
+  ob[IO_RHO] = A[SP_RHO][n1];
+  ob[IO_EX] = A[SP_EX][n1];
+  ob[IO_EY] = A[SP_EY][n1];
+  ob[IO_EZ] = A[SP_EZ][n1];
+  ob[IO_BX] = A[SP_BX][n1];
+  ob[IO_BY] = A[SP_BY][n1];
+  ob[IO_BZ] = A[SP_BZ][n1];
+  ob[IO_PSI] = a.flag ? A[SP_PSI][n1] : 0.0;
+  iterate_adj(m, ob, ib);
+  A[SP_EX][n1] = 0.0;
+  A[SP_EY][n1] = 0.0;
+  A[SP_EZ][n1] = 0.0;
+  A[SP_BX][n1] = 0.0;
+  A[SP_BY][n1] = 0.0;
+  A[SP_BZ][n1] = 0.0;
+  if (a.flag) A[SP_PSI][n1] = 0.0;
+  A[SP_RHO][n1] = ib[II_RHO_OLD];
+  A[SP_RHOP][n1] += ib[II_RP];
+  A[SP_JX][n1] += ib[II_JX];
+  A[SP_JY][n1] += ib[II_JY];
+  A[SP_JZ][n1] += ib[II_JZ];
+  A[SP_JXP][n1] += ib[II_JXP];
+  A[SP_JYP][n1] += ib[II_JYP];
+  A[SP_JZP][n1] += ib[II_JZP];
+  A[SP_RHOB][n1] += ib[II_RB];
+  A[SP_RHOBP][n1] += ib[II_RBP];
+  A[SP_EYP][n1] += ib[II_EYP];
+  A[SP_EZP][n1] += ib[II_EZP];
+  A[SP_JXB][n1] += ib[II_JXB];
 }
 
 // Eerr = max_c dif_c / max(1e-5, max_c)  (:1541, :1584-1621, :1641-1643)
@@ -172,9 +252,38 @@ SpecArgs make_args(qss_ctx *c, int flag) {
   return a;
 }
 
+SpecArgs make_adj_args(qss_ctx *c, double *const *A, int flag) {
+  SpecArgs a = make_args(c, flag);
+  for (int s = 0; s < SP_N; ++s) a.S[s] = A[s];
+  return a;
+}
+
+__global__ void k_axpy(double *y, const double *x, double alpha, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += alpha * x[i];
+}
+
 inline dim3 grid1(long n) { return dim3((unsigned)((n + NT - 1) / NT)); }
 
 }  // namespace
+
+hipError_t launch_axpy(double *y, const double *x, double a, long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_axpy, grid1(n), dim3(NT), 0, s, y, x, a, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_guess_adj(qss_ctx *c, double *const *A, int first_layer) {
+  hipLaunchKernelGGL(k_guess_adj, grid1(c->ncell), dim3(NT), 0, c->stream,
+                     make_adj_args(c, A, first_layer));
+  return hipGetLastError();
+}
+
+hipError_t launch_iterate_adj(qss_ctx *c, double *const *A, int final_pass) {
+  hipLaunchKernelGGL(k_iterate_adj, grid1(c->ncell), dim3(NT), 0, c->stream,
+                     make_adj_args(c, A, final_pass));
+  return hipGetLastError();
+}
 
 hipError_t launch_fill_i32(int32_t *p, long n, int32_t v, hipStream_t s) {
   if (n <= 0) return hipSuccess;

@@ -260,6 +260,39 @@ class Mesh:
             raise QssError(n, "qss_gemm_variants", "unknown precision")
         return n
 
+    # -- differentiation ---------------------------------------------------
+    def set_bins(self, bins) -> None:
+        """Replace the driver's piecewise-constant longitudinal profile."""
+        b = np.ascontiguousarray(bins, dtype=np.float64)
+        check(self.L.qss_set_bins(self.h, _p(b), int(b.size)), "qss_set_bins")
+
+    def sweep_checkpointed(self, nrun: int, every: int = 32) -> np.ndarray:
+        """Sweep layers [0, nrun) from a fresh front, keeping a checkpoint every
+        `every` layers for sweep_vjp.  Returns the layer residuals."""
+        res = np.zeros(nrun)
+        check(self.L.qss_sweep_checkpointed(self.h, int(nrun), int(every), _p(res)),
+              "qss_sweep_checkpointed")
+        return res
+
+    def sweep_vjp(self, ex_bar=None, device_ptr: Optional[int] = None) -> np.ndarray:
+        """dLoss/dbins of the preceding checkpointed sweep, given dLoss/dEx of
+        its layers (host array (nrun, nz, ny), or a device pointer)."""
+        g = np.zeros(self.params.nbins if self.params.bins is not None else 0)
+        if device_ptr is not None:
+            rc = self.L.qss_sweep_vjp(self.h, ctypes.c_void_p(device_ptr), 0, _p(g))
+        elif ex_bar is not None:
+            e = np.ascontiguousarray(ex_bar, dtype=np.float64)
+            rc = self.L.qss_sweep_vjp(self.h, e.ctypes.data_as(ctypes.c_void_p), 1, _p(g))
+        else:
+            rc = self.L.qss_sweep_vjp(self.h, None, 0, _p(g))
+        check(rc, "qss_sweep_vjp")
+        return g
+
+    def copy_history(self, name: str, first: int, count: int, dst_ptr: int) -> None:
+        """Device-to-device copy of history planes into memory at dst_ptr."""
+        check(self.L.qss_copy_history(self.h, FIELD_ID[name], int(first), int(count),
+                                      ctypes.c_void_p(dst_ptr)), "qss_copy_history")
+
     # -- timing ------------------------------------------------------------
     def enable_timing(self, on: bool = True):
         check(self.L.qss_enable_timing(self.h, int(on)), "qss_enable_timing")
