@@ -61,6 +61,14 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the whole-sweep accuracy comparison of the mixed modes")
     ap.add_argument("--no-timing-events", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint (gradient) timing on the 256^2 workload")
+    ap.add_argument("--fd-bins", type=int, default=16,
+                    help="bins checked against central differences in the adjoint section")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched parameter scan (64 wakes on 4 grid sizes)")
+    ap.add_argument("--batch-slices", type=int, default=512)
+    ap.add_argument("--batch-wakes", type=int, default=16, help="wakes per grid size")
     return ap.parse_args()
 
 
@@ -129,6 +137,139 @@ def sampled_history(mode, every):
         m.close()
 
 
+def adjoint_run(fd_bins: int):
+    """256x256 x 4 ppc (262,144 electrons/slice), FP64, 256 profile bins over
+    xi in [0,4]; loss = -mean Ex + 1e-3 var Ex over xi in [6,6.5], r < 0.2,
+    which only depends on the first 650 layers.  Times the plain forward
+    sweep of those layers and the gradient (checkpointed forward, every 32
+    layers, + reverse sweep), and checks the gradient against central
+    differences (h = 1e-6) on `fd_bins` random bins."""
+    from tertia_amd.loss import witness_loss_np
+    from tertia_amd.mesh import Mesh
+    from tertia_amd.params import config4
+    p = config4()
+    nrun, every = 650, 32
+    win = dict(xi0=6.0, xi1=6.5, rmax=0.2)
+    m = Mesh(p, history=("Ex",))
+    try:
+        m.autotune(2)
+        used0, _ = Mesh.device_memory()
+
+        def ex_hist():
+            return np.stack([m.history("Ex", s) for s in range(nrun)])
+
+        m.reset()
+        m.sweep_range(0, 5)                      # warm up
+        m.reset()
+        t0 = time.perf_counter()
+        m.sweep_range(0, nrun)
+        t_fwd = time.perf_counter() - t0
+        loss, _ = witness_loss_np(ex_hist(), p, **win)
+        t0 = time.perf_counter()
+        m.sweep_checkpointed(nrun, every)
+        t_ck = time.perf_counter() - t0
+        loss2, gex = witness_loss_np(ex_hist(), p, **win)
+        t0 = time.perf_counter()
+        g = m.sweep_vjp(gex)
+        t_vjp = time.perf_counter() - t0
+        # the same reverse sweep with a HIP event pair per launch: breakdown
+        m.sweep_checkpointed(nrun, every)
+        m.enable_timing(True)
+        m.sweep_vjp(gex)
+        tm = m.timing()
+        m.enable_timing(False)
+        used1, _ = Mesh.device_memory()
+        fd = None
+        if fd_bins > 0:
+            rng = np.random.default_rng(2026)
+            idx = sorted(rng.choice(np.arange(len(p.bins)), fd_bins, replace=False).tolist())
+            base = np.array(p.bins)
+            gf = []
+            for i in idx:
+                h = 1e-6 * max(1.0, abs(base[i]))
+                vals = []
+                for sgn in (1, -1):
+                    b = base.copy()
+                    b[i] += sgn * h
+                    m.set_bins(b)
+                    m.reset()
+                    m.sweep_range(0, nrun)
+                    vals.append(witness_loss_np(ex_hist(), p, **win)[0])
+                gf.append((vals[0] - vals[1]) / (2 * h))
+            m.set_bins(base)
+            gf = np.array(gf)
+            scale = float(np.abs(g).max())
+            fd = {"bins": idx, "max_rel_err": float(np.abs(g[idx] - gf).max() / scale),
+                  "grad_max": scale, "h": 1e-6}
+    finally:
+        m.close()
+    return {"workload": "256x256 over 16x16 k_p^-1, 4 ppc (262,144 electrons/slice), dxi=0.01,"
+                        " FP64, 256 profile bins on xi in [0,4], loss on xi in [6,6.5] r<0.2 "
+                        "(first 650 layers)",
+            "forward_s": t_fwd, "checkpointed_forward_s": t_ck, "reverse_s": t_vjp,
+            "adjoint_over_forward": t_vjp / t_fwd,
+            "gradient_over_forward": (t_ck + t_vjp) / t_fwd,
+            "note": "adjoint = the reverse sweep (segment recomputation from the checkpoints "
+                    "included); gradient = checkpointed forward + reverse sweep",
+            "reverse_kernel_ms": tm["ms"], "reverse_launches": tm["launches"],
+            "checkpoint_every": every,
+            "loss": loss, "loss_checkpointed": loss2,
+            "device_bytes_in_use": used1, "device_bytes_before": used0,
+            "fd_check": fd}
+
+
+def batch_run(nslices: int, per_size: int):
+    """Parameter scan: per_size wakes on each of 128^2, 192^2, 256^2, 384^2
+    (4 ppc, nslices layers, random driver parameters).  Batched: one context
+    per grid size advancing all its wakes with every launch; sequential: the
+    same wakes one context at a time (same tile-shape tuning per size)."""
+    from tertia_amd.mesh import Mesh
+    from tertia_amd.params import Params, config3
+    groups = config3(per_size=per_size, nslices=nslices)
+    out = {"workload": "%d wakes: %d each on 128^2, 192^2, 256^2, 384^2 (h = 1/16), 4 ppc, "
+                       "%d layers of dxi = 0.02, FP64; sigma_r~U[0.2,0.8], "
+                       "sigma_xi~U[0.5,1.5], nb~U[0.5,8]" % (4 * per_size, per_size, nslices),
+           "groups": []}
+    t_b = t_s = 0.0
+    for p in groups:
+        m = Mesh(p, history=())
+        try:
+            m.autotune(2)
+            m.reset()
+            m.sweep_range(0, 2)
+            m.reset()
+            t0 = time.perf_counter()
+            m.sweep_range(0, nslices)
+            tb = time.perf_counter() - t0
+        finally:
+            m.close()
+        ts = 0.0
+        choice = None
+        for w in p.wakes:
+            ps = Params(ny=p.ny, nz=p.nz, ly=p.ly, lz=p.lz, hx=p.hx, nslices=nslices,
+                        bunches=list(w))
+            m = Mesh(ps, history=())
+            try:
+                if choice is None:
+                    choice = m.autotune(2)
+                else:
+                    m.set_gemm_choice(choice)
+                m.reset()
+                t0 = time.perf_counter()
+                m.sweep_range(0, nslices)
+                ts += time.perf_counter() - t0
+            finally:
+                m.close()
+        out["groups"].append({"n": p.ny, "wakes": len(p.wakes), "batched_s": tb,
+                              "sequential_s": ts})
+        t_b += tb
+        t_s += ts
+    total = 4 * per_size * nslices
+    out.update({"batched_slices_per_s": total / t_b, "sequential_slices_per_s": total / t_s,
+                "speedup": t_s / t_b})
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -193,6 +334,13 @@ def main():
         line["event_window_ms"] = base["event_window_ms"]
     if mixed:
         line["mixed"] = mixed
+    if not a.no_batch:
+        line["batch"] = batch_run(a.batch_slices, a.batch_wakes)
+        print("batch speedup %.2f" % line["batch"]["speedup"], file=sys.stderr, flush=True)
+    if not a.no_adjoint:
+        line["adjoint"] = adjoint_run(a.fd_bins)
+        print("adjoint/forward %.2f" % line["adjoint"]["adjoint_over_forward"], file=sys.stderr,
+              flush=True)
     if a.cpu_slices > 0:
         nthr = a.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(config2(), a.cpu_slices, nthr)

@@ -114,6 +114,15 @@ typedef struct qss_params {
   int32_t nbins;
   double  bin_xi0, bin_xi1;
   const double *bins;    /* host pointer, nbins values (copied at create)   */
+  /* batch of independent wakes on this grid, advanced together: every
+   * kernel launch of a layer step covers all of them (0 or 1: one wake).
+   * wake_bunch (host, nwake*nbunch, copied at create; NULL = every wake uses
+   * `bunch`) gives wake w its bunches at [w*nbunch, (w+1)*nbunch).  The
+   * plasma, profile bins and grid are shared.  nwake > 1 needs the tiled
+   * particle path (ny, nz multiples of 16, fix_subcycle = 0) and precision
+   * FP64 or MIXED32; the adjoint works on single-wake contexts.            */
+  int32_t nwake;
+  const qss_bunch *wake_bunch;
 } qss_params;
 
 /* ---- context ---------------------------------------------------------- */
@@ -128,8 +137,12 @@ int  qss_create(const qss_params *p, int device, uint32_t history_mask,
                 qss_ctx **out);
 int  qss_destroy(qss_ctx *c);
 const char *qss_last_error(void);
-/* Returns the compiled-in sizes: particles per layer etc. */
+/* Returns the compiled-in sizes: particles per layer etc. (per wake) */
 int  qss_info(const qss_ctx *c, int64_t *n_particles, int64_t *n_cells);
+/* Batched contexts: the wake that the data-access calls below (layer
+ * fields, spectra, history, particles, pointers) refer to; default 0.
+ * Sweep residuals of a batch are returned wake-major.                      */
+int  qss_select_wake(qss_ctx *c, int wake);
 /* The HIP stream all work of this context runs on (hipStream_t). */
 void *qss_stream(qss_ctx *c);
 
@@ -142,6 +155,12 @@ int  qss_sweep(qss_ctx *c, double *residuals);
  * layer behind the front); used for timing windows and checkpoints.       */
 int  qss_sweep_range(qss_ctx *c, int first, int count, double *residuals);
 int  qss_reset(qss_ctx *c);   /* re-seed particles, zero fields/spectra    */
+
+/* on != 0: qss_sweep/qss_sweep_range replay pairs of layer steps from one
+ * captured hipGraph (layers >= 1, nx_split = 1, timing off) instead of
+ * launching every kernel from the host.  Results are bitwise those of the
+ * eager launches.  Off by default.                                         */
+int  qss_enable_graph(qss_ctx *c, int on);
 
 /* ---- one layer step: Mesh::MoveSplitLayer (movepart_splitlayer.cpp:561) */
 /* slice: 0-based position behind the front; isplit < nx_split.
@@ -198,6 +217,9 @@ int  qss_set_gemm_variant(qss_ctx *c, int variant);
 int  qss_gemm_variants(int precision);
 /* chosen[pass*9 + nbatch] for pass 0 (X.T_y), 1 (T_z.Y), nbatch 1..8.      */
 int  qss_gemm_choice(const qss_ctx *c, int *chosen);
+/* Installs a choice table (same layout), e.g. one tuned on another context
+ * of the same grid and precision.                                           */
+int  qss_set_gemm_choice(qss_ctx *c, const int *chosen);
 
 /* ---- data access (host buffers, Ny*Nz doubles, synchronous) ------------ */
 /* which: 0 = C layer, 1 = P layer.  field: QSS_EX..QSS_PSI. */
@@ -250,6 +272,8 @@ int  qss_sweep_checkpointed(qss_ctx *c, int nrun, int every, double *residuals);
 int  qss_sweep_vjp(qss_ctx *c, const double *ex_bar, int on_host, double *grad_bins);
 /* Device-to-device copy of history planes [first, first+count) of a field. */
 int  qss_copy_history(qss_ctx *c, int field, int first, int count, double *dst_dev);
+/* Bytes of device memory in use on `device` (all processes) and in total.  */
+int  qss_device_memory(int device, int64_t *used, int64_t *total);
 
 /* ---- timing of the last qss_sweep/qss_sweep_range (HIP events) --------- */
 /* Kernel classes: 0 push, 1 transform GEMM, 2 spectral update, 3 beam,

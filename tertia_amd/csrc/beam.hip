@@ -23,10 +23,12 @@ struct BeamArgs {
   double hx, hy, hz, ly, lz, bin_xi0, bin_xi1;
   uint64_t seed;
   qss_bunch b[QSS_MAX_BUNCH];
+  const qss_bunch *wb;   // batched wakes: QSS_MAX_BUNCH bunches per wake (or null)
   const double *bins;
   double *rho, *jx;   // C-layer beam planes
   int layer;          // reference layer index i = nslices-1-slice
   int nsplit, isplit;
+  const int *slice_dev;   // graph replay: the slice is read from device memory
 };
 
 __device__ inline uint64_t mix64(uint64_t x) {
@@ -40,9 +42,9 @@ __device__ inline double unif_pm1(uint64_t seed, uint64_t a, uint64_t b) {
   return (double)(h >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
 }
 
-__device__ double bunch_density(const BeamArgs &a, int b, double xi, double yc,
-                                double zc) {
-  const qss_bunch &B = a.b[b];
+__device__ double bunch_density(const BeamArgs &a, const qss_bunch *bun, int b, double xi,
+                                double yc, double zc) {
+  const qss_bunch &B = bun[b];
   const double Ry = sqrt(2.0) * B.sigma_r;
   const double dy = yc - B.y0, dz = zc - B.z0;
   if (b == 0 && a.nbins > 0) {
@@ -61,8 +63,8 @@ __device__ double bunch_density(const BeamArgs &a, int b, double xi, double yc,
 
 // value of reference layer i at node (nj, nk); sums in the order
 // x-cell (i, i-1), bunch, ck, cj, seed
-__device__ void node_value(const BeamArgs &a, int i, int nj, int nk, double &rho,
-                           double &jx) {
+__device__ void node_value(const BeamArgs &a, const qss_bunch *bun, int i, int nj, int nk,
+                           double &rho, double &jx) {
   rho = 0.0;
   jx = 0.0;
   for (int side = 0; side < 2; ++side) {
@@ -71,7 +73,7 @@ __device__ void node_value(const BeamArgs &a, int i, int nj, int nk, double &rho
     const double ax = 0.5;
     const double xi = (double)(a.nslices - ci) * a.hx;
     for (int b = 0; b < a.nbunch; ++b) {
-      const qss_bunch &B = a.b[b];
+      const qss_bunch &B = bun[b];
       for (int ck = nk - 1; ck <= nk; ++ck) {
         if (ck < 0) continue;
         const int dz_ = nk - ck;
@@ -80,7 +82,7 @@ __device__ void node_value(const BeamArgs &a, int i, int nj, int nk, double &rho
           const int dy_ = nj - cj;
           const double yc = a.hy * cj - a.ly / 2.0;
           const double zc = a.hz * ck - a.lz / 2.0;
-          const double dn = bunch_density(a, b, xi, yc, zc);
+          const double dn = bunch_density(a, bun, b, xi, yc, zc);
           if (!(dn > 0.0)) continue;
           for (int is = 0; is < 4; ++is) {
             const double yq = 1.0 / 3.0 * (1.0 + is % 2);
@@ -107,20 +109,25 @@ __device__ void node_value(const BeamArgs &a, int i, int nj, int nk, double &rho
 
 __global__ __launch_bounds__(NT) void k_beam_plane(BeamArgs a) {
   const long n = (long)blockIdx.x * NT + threadIdx.x;
-  if (n >= (long)a.ny * a.nz) return;
+  const long nc = (long)a.ny * a.nz;
+  if (n >= nc) return;
+  const int w = blockIdx.y;   // wake of a batch
+  const qss_bunch *bun = a.wb ? a.wb + (long)w * QSS_MAX_BUNCH : a.b;
+  double *rho = a.rho + w * nc, *jx = a.jx + w * nc;
   const int nj = (int)(n % a.ny), nk = (int)(n / a.ny);
+  const int layer = a.slice_dev ? a.nslices - 1 - *a.slice_dev : a.layer;
   double rc, jc;
-  node_value(a, a.layer, nj, nk, rc, jc);
+  node_value(a, bun, layer, nj, nk, rc, jc);
   if (a.nsplit == 1) {
-    a.rho[n] = rc;
-    a.jx[n] = jc;
+    rho[n] = rc;
+    jx[n] = jc;
     return;
   }
   double rp, jp;
-  node_value(a, a.layer + 1, nj, nk, rp, jp);
+  node_value(a, bun, layer + 1, nj, nk, rp, jp);
   const int S = a.nsplit, s = a.isplit;
-  a.rho[n] = (rc * (S - s) + rp * s) / S;
-  a.jx[n] = (jc * (S - s) + jp * s) / S;
+  rho[n] = (rc * (S - s) + rp * s) / S;
+  jx[n] = (jc * (S - s) + jp * s) / S;
 }
 
 }  // namespace
@@ -133,13 +140,15 @@ hipError_t launch_beam_plane(qss_ctx *c, int slice, int isplit) {
   a.bin_xi0 = c->p.bin_xi0; a.bin_xi1 = c->p.bin_xi1;
   a.seed = c->p.seed;
   for (int b = 0; b < QSS_MAX_BUNCH; ++b) a.b[b] = c->p.bunch[b];
+  a.wb = c->nw > 1 ? c->d_wbunch : nullptr;
   a.bins = c->d_bins;
   a.rho = c->C[QSS_RHOBEAM];
   a.jx = c->C[QSS_JXBEAM];
   a.layer = c->p.nslices - 1 - slice;
   a.nsplit = c->p.nx_split;
   a.isplit = isplit;
-  hipLaunchKernelGGL(k_beam_plane, dim3((unsigned)((c->ncell + NT - 1) / NT)), dim3(NT), 0,
-                     c->stream, a);
+  a.slice_dev = c->capturing ? c->d_slice : nullptr;
+  hipLaunchKernelGGL(k_beam_plane, dim3((unsigned)((c->ncell + NT - 1) / NT), c->nw), dim3(NT),
+                     0, c->stream, a);
   return hipGetLastError();
 }

@@ -49,10 +49,10 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
-// work item of this block: field f, tile origin (m0, n0).  Hardware sends
-// block b to XCD b % 8; each XCD gets a contiguous range of items.
+// work item of this block: wake w, field f, tile origin (m0, n0).  Hardware
+// sends block b to XCD b % 8; each XCD gets a contiguous range of items.
 struct Item {
-  int f, m0, n0;
+  int w, f, m0, n0;
 };
 template <int BM, int BN>
 __device__ inline Item block_item(const GemmArgs &a) {
@@ -62,8 +62,10 @@ __device__ inline Item block_item(const GemmArgs &a) {
   int bid = blockIdx.x;
   if (nblk % kXcd == 0) bid = (bid % kXcd) * (nblk / kXcd) + bid / kXcd;
   Item it;
-  it.f = bid / per_field;
-  const int tt = bid - it.f * per_field;
+  const int fw = bid / per_field;
+  it.w = fw / a.nbatch;
+  it.f = fw - it.w * a.nbatch;
+  const int tt = bid - fw * per_field;
   const int tm = tt / tiles_n;
   it.m0 = tm * BM;
   it.n0 = (tt - tm * tiles_n) * BN;
@@ -124,12 +126,17 @@ __device__ inline void k_loop(int K, L0 load0, L1 load1, S0 store0, S1 store1, C
 // tile (i, j), register r; ROW(lk, r) the tile row that register r of a lane
 // in row group lk = lane/16 holds (the column is lane % 16).
 template <bool GUARD, int TI, int TJ, class ROW, class V>
-__device__ inline void store_tiles(const GemmArgs &a, const GemmBatch &B, int r0, int c0,
+__device__ inline void store_tiles(const GemmArgs &a, const GemmBatch &B, int w, int r0, int c0,
                                    int lane, V val) {
   const int M = a.M, N = a.N;
   const int li = lane & 15, lk = lane >> 4;
   double dmax = 0.0, vmax = 0.0;
   const double alpha = a.alpha;
+  double *const Cw = B.C + w * a.wC;
+  const double *const Pw = B.P ? B.P + w * a.wP : nullptr;
+  double *const Fw = a.F ? a.F + w * a.wF : nullptr;
+  float *const Ffw = a.Ff ? a.Ff + w * a.wF : nullptr;
+  unsigned long long *const Rw = a.resid + w * a.wR;
 #pragma unroll
   for (int i = 0; i < TI; ++i)
 #pragma unroll
@@ -141,20 +148,20 @@ __device__ inline void store_tiles(const GemmArgs &a, const GemmBatch &B, int r0
         if (!GUARD || (row < M && col < N)) {
           const long n = (long)row * N + col;
           double v = alpha * val(i, j, r);
-          if (a.beta != 0.0) v += a.beta * B.C[n];   // accumulate (adjoint sweep)
+          if (a.beta != 0.0) v += a.beta * Cw[n];   // accumulate (adjoint sweep)
           if (B.resid >= 0) {
             // NaN -> +inf so the non-finite guard sees it (fmax drops NaN)
-            const double old = B.C[n];
+            const double old = Cw[n];
             const double d = fabs(old - v), m = fabs(v);
             dmax = fmax(dmax, d == d ? d : INFINITY);
             vmax = fmax(vmax, m == m ? m : INFINITY);
           }
-          B.C[n] = v;
+          Cw[n] = v;
           if (B.push >= 0) {
-            const double pv = 0.5 * (B.P[n] + v);
+            const double pv = 0.5 * (Pw[n] + v);
             const long pn = (long)B.push * M * N + n;
-            if (a.Ff) a.Ff[pn] = (float)pv;
-            else a.F[pn] = pv;
+            if (Ffw) Ffw[pn] = (float)pv;
+            else Fw[pn] = pv;
           }
         }
       }
@@ -162,8 +169,8 @@ __device__ inline void store_tiles(const GemmArgs &a, const GemmBatch &B, int r0
     dmax = wave_max(dmax);
     vmax = wave_max(vmax);
     if (lane == 0) {
-      atomic_max_nonneg(&a.resid[B.resid], dmax);
-      atomic_max_nonneg(&a.resid[3 + B.resid], vmax);
+      atomic_max_nonneg(&Rw[B.resid], dmax);
+      atomic_max_nonneg(&Rw[3 + B.resid], vmax);
     }
   }
 }
@@ -230,8 +237,8 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_gemm(GemmArgs a) {
   const int ks = w / C::NWT, wl = w - ks * C::NWT;
   const int wm = wl / WGN, wn = wl - wm * WGN;
   const int li = lane & 15, lk = lane >> 4;
-  const double *Ap = B.A;
-  const double *Bp = B.B;
+  const double *Ap = B.A + it.w * a.wA;
+  const double *Bp = B.B + it.w * a.wB;
 
   double ra[2][C::A_PER_T], rb[2][C::B_PER_T];
   auto load = [&](int k0, double *ra_, double *rb_) {
@@ -329,7 +336,7 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_gemm(GemmArgs a) {
         for (int j = 0; j < C::TJ; ++j) acc[i][j] += red[q][wl][i * C::TJ + j][lane];
   }
 
-  store_tiles<GUARD, C::TI, C::TJ, MT>(a, B, m0 + wm * C::WTM, n0 + wn * C::WTN, lane,
+  store_tiles<GUARD, C::TI, C::TJ, MT>(a, B, it.w, m0 + wm * C::WTM, n0 + wn * C::WTN, lane,
                                        [&](int i, int j, int r) { return (double)acc[i][j][r]; });
 }
 
@@ -472,7 +479,7 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_gemm_bf16(GemmArgs a) {
 
   if constexpr (!TOUT) {
     store_tiles<GUARD, TI, TJ, RowBF16>(
-        a, B, m0 + wm * WTM, n0 + wn * WTN, lane,
+        a, B, 0, m0 + wm * WTM, n0 + wn * WTN, lane,
         [&](int i, int j, int r) { return (double)hi[i][j][r] + (double)lo[i][j][r]; });
   } else {
     // transposed split store: Ct[col][row], the 4 rows of a lane's registers
@@ -647,10 +654,13 @@ hipError_t launch_gemm(const GemmArgs &a, hipStream_t s, int prec, int variant) 
   const Variant &v = tb.v[variant];
   const int tout = a.tout ? 1 : 0;
   if (tout && !gemm_pieces(prec)) return hipErrorInvalidValue;
+  const int nw = a.nw > 1 ? a.nw : 1;
+  if (nw > 1 && gemm_pieces(prec)) return hipErrorInvalidValue;   // split modes: one wake
   const int tiles = ((a.M + v.bm - 1) / v.bm) * ((a.N + v.bn - 1) / v.bn);
   const bool exact = a.M % v.bm == 0 && a.N % v.bn == 0 && a.K % (2 * v.bk) == 0 &&
                      (!tout || a.M % 4 == 0);
-  hipLaunchKernelGGL(v.fn[exact ? 0 : 1][tout], dim3(tiles * a.nbatch), dim3(v.nt), 0, s, a);
+  hipLaunchKernelGGL(v.fn[exact ? 0 : 1][tout], dim3(tiles * a.nbatch * nw), dim3(v.nt), 0, s,
+                     a);
   return hipGetLastError();
 }
 

@@ -62,11 +62,30 @@ struct TileArgs {
   int64_t *spid;
   int32_t *tcount;         // ntiles x 9
   int32_t *counters;
+  long npw;                // particle slots per wake (batched contexts)
 };
 
 __device__ inline int wrapi(int a, int n) {
   a %= n;
   return a < 0 ? a + n : a;
+}
+
+// the arguments of wake w of a batch: every per-wake array is stored
+// wake-major with a fixed stride
+__host__ __device__ inline TileArgs wake_view(const TileArgs &a0, int w) {
+  TileArgs a = a0;
+  const long pw = (long)w * a0.npw, nt = (long)a0.ntj * a0.ntk;
+  const long plane = (long)a0.ny * a0.nz;
+  a.y += pw; a.z += pw; a.px += pw; a.py += pw; a.pz += pw;
+  a.cell += pw; a.pid += pw;
+  a.tile_start += w * (nt + 1);
+  if (a.F) a.F += w * kPushComp * plane;
+  if (a.Ff) a.Ff += w * kPushComp * plane;
+  a.win += w * nt * NWIN * 4;
+  a.sy += pw; a.sz += pw; a.spx += pw; a.spy += pw; a.spz += pw;
+  a.scell += pw; a.sdest += pw; a.spid += pw;
+  a.tcount += w * nt * 9;
+  return a;
 }
 
 using qss_tile::sort256;
@@ -78,7 +97,8 @@ using qss_tile::sort256;
 // weights are summed in double.  With R = double every expression below
 // evaluates exactly as the reference's (no contraction, same association).
 template <typename R, typename FT>
-__global__ __launch_bounds__(NT) void k_push_tile(TileArgs a) {
+__global__ __launch_bounds__(NT) void k_push_tile(TileArgs a0) {
+  const TileArgs a = wake_view(a0, blockIdx.y);   // wake of a batch
   __shared__ FT Fw[NF * 6];
   __shared__ int keys_sh[NT];
   __shared__ int sorted[NT];
@@ -363,11 +383,14 @@ __global__ __launch_bounds__(NT) void k_push_tile(TileArgs a) {
 
 // sum of the windows covering each node, fixed (dtk, dtj) order
 __global__ __launch_bounds__(NT) void k_combine(const double *win, int ntj, int ntk, int ny,
-                                                int nz, double *J0, double *J1, double *J2,
-                                                double *J3) {
-  const long n = (long)blockIdx.x * NT + threadIdx.x;
-  if (n >= (long)ny * nz) return;
-  const int nj = (int)(n % ny), nk = (int)(n / ny);
+                                                int nz, int nw, double *J0, double *J1,
+                                                double *J2, double *J3) {
+  const long n = (long)blockIdx.x * NT + threadIdx.x;   // wake-major node index
+  const long plane = (long)ny * nz;
+  if (n >= plane * nw) return;
+  win += (n / plane) * ntj * ntk * NWIN * 4;   // the wake's windows
+  const long n1 = n % plane;
+  const int nj = (int)(n1 % ny), nk = (int)(n1 / ny);
   const int tj = nj / T, tk = nk / T;
   double s[4] = {0, 0, 0, 0};
   for (int dtk = -1; dtk <= 1; ++dtk) {
@@ -394,6 +417,10 @@ __global__ __launch_bounds__(NT) void k_tile_scan(const int32_t *tcount, int ntj
                                                   int32_t *tile_start, int32_t *goff) {
   __shared__ int sh[NT];
   const int nt = ntj * ntk;
+  const int w = blockIdx.x;   // one work-group per wake
+  tcount += (long)w * nt * 9;
+  goff += (long)w * nt * 9;
+  tile_start += (long)w * (nt + 1);
   int carry = 0;
   for (int b0 = 0; b0 < nt; b0 += NT) {
     const int u = b0 + threadIdx.x;
@@ -434,11 +461,18 @@ __global__ __launch_bounds__(NT) void k_tile_scan(const int32_t *tcount, int ntj
   if (threadIdx.x == 0) tile_start[nt] = carry;
 }
 
-__global__ __launch_bounds__(NT) void k_permute(TileArgs a, const int32_t *goff, long np,
-                                                double *y, double *z, double *px, double *py,
-                                                double *pz, int32_t *cell, int64_t *pid) {
-  const long s = (long)blockIdx.x * NT + threadIdx.x;
-  if (s >= np) return;
+__global__ __launch_bounds__(NT) void k_permute(TileArgs a0, const int32_t *goff, long np,
+                                                int nw, double *y, double *z, double *px,
+                                                double *py, double *pz, int32_t *cell,
+                                                int64_t *pid) {
+  const long sg = (long)blockIdx.x * NT + threadIdx.x;   // wake-major slot
+  if (sg >= np * nw) return;
+  const int w = (int)(sg / np);
+  const long s = sg - (long)w * np;
+  const TileArgs a = wake_view(a0, w);
+  goff += (long)w * a.ntj * a.ntk * 9;
+  const long pw = (long)w * a0.npw;
+  y += pw; z += pw; px += pw; py += pw; pz += pw; cell += pw; pid += pw;
   const int32_t c = a.scell[s];
   const int dst_tile = ((c / a.ny) / T) * a.ntj + (c % a.ny) / T;
   const int sd = a.sdest[s];
@@ -454,10 +488,14 @@ __global__ __launch_bounds__(NT) void k_permute(TileArgs a, const int32_t *goff,
 
 // seeding in tile-major slot order: tile, local cell (j fastest), seed
 __global__ void k_seed_tiled(double *y, double *z, double *px, double *py, double *pz,
-                             int32_t *cell, int64_t *pid, long np, int ny, int ntj,
+                             int32_t *cell, int64_t *pid, long np, int nw, int ny, int ntj,
                              int ppc_y, int ppc_z, double jitter, uint64_t seed) {
-  const long n = (long)blockIdx.x * NT + threadIdx.x;
-  if (n >= np) return;
+  const long ng = (long)blockIdx.x * NT + threadIdx.x;   // wake-major slot
+  if (ng >= np * nw) return;
+  // every wake of a batch starts from the same plasma
+  const long n = ng % np;
+  y += ng - n; z += ng - n; px += ng - n; py += ng - n; pz += ng - n;
+  cell += ng - n; pid += ng - n;
   const int ppc = ppc_y * ppc_z;
   const long per_tile = (long)T * T * ppc;
   const long tile = n / per_tile;
@@ -494,9 +532,9 @@ __global__ void k_seed_tiled(double *y, double *z, double *px, double *py, doubl
   pid[n] = id;
 }
 
-__global__ void k_uniform_tile_start(int32_t *ts, int nt, int per_tile) {
-  const int i = blockIdx.x * NT + threadIdx.x;
-  if (i <= nt) ts[i] = i * per_tile;
+__global__ void k_uniform_tile_start(int32_t *ts, int nt, int nw, int per_tile) {
+  const int i = blockIdx.x * NT + threadIdx.x;   // wake-major
+  if (i < (nt + 1) * nw) ts[i] = (i % (nt + 1)) * per_tile;
 }
 
 inline dim3 grid1(long n) { return dim3((unsigned)((n + NT - 1) / NT)); }
@@ -513,12 +551,14 @@ TileArgs make_args(qss_ctx *c, int full) {
   a.hx = c->hxs; a.hy = c->hy; a.hz = c->hz; a.weight = c->pweight;
   a.shape = c->p.shape; a.full = full;
   a.win = c->win;
-  a.sy = c->ev_y; a.sz = c->ev_z; a.spx = c->ev_p; a.spy = c->ev_p + c->np_cap;
-  a.spz = c->ev_p + 2 * c->np_cap;
+  const long npall = c->np_cap * c->nw;
+  a.sy = c->ev_y; a.sz = c->ev_z; a.spx = c->ev_p; a.spy = c->ev_p + npall;
+  a.spz = c->ev_p + 2 * npall;
   a.scell = c->ev_cell; a.sdest = c->ev_rank;
   a.spid = reinterpret_cast<int64_t *>(c->ov_key);   // ov_cap >= np_cap
   a.tcount = c->tcount;
   a.counters = c->d_counters;
+  a.npw = c->np_cap;
   return a;
 }
 
@@ -527,42 +567,43 @@ TileArgs make_args(qss_ctx *c, int full) {
 hipError_t launch_push_tile(qss_ctx *c, int full) {
   const int nt = c->ntj * c->ntk;
   if (c->mixed)
-    hipLaunchKernelGGL((k_push_tile<float, float>), dim3(nt), dim3(NT), 0, c->stream,
+    hipLaunchKernelGGL((k_push_tile<float, float>), dim3(nt, c->nw), dim3(NT), 0, c->stream,
                        make_args(c, full));
   else
-    hipLaunchKernelGGL((k_push_tile<double, double>), dim3(nt), dim3(NT), 0, c->stream,
+    hipLaunchKernelGGL((k_push_tile<double, double>), dim3(nt, c->nw), dim3(NT), 0, c->stream,
                        make_args(c, full));
   return hipGetLastError();
 }
 
 hipError_t launch_combine(qss_ctx *c) {
-  hipLaunchKernelGGL(k_combine, grid1(c->ncell), dim3(NT), 0, c->stream, c->win, c->ntj,
-                     c->ntk, c->ny, c->nz, c->C[QSS_JX], c->C[QSS_JY], c->C[QSS_JZ],
-                     c->C[QSS_RHO]);
+  hipLaunchKernelGGL(k_combine, grid1(c->ncell * c->nw), dim3(NT), 0, c->stream, c->win,
+                     c->ntj, c->ntk, c->ny, c->nz, c->nw, c->C[QSS_JX], c->C[QSS_JY],
+                     c->C[QSS_JZ], c->C[QSS_RHO]);
   return hipGetLastError();
 }
 
 hipError_t launch_tile_reorder(qss_ctx *c) {
   const int nb = c->cur ^ 1;
-  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(NT), 0, c->stream, c->tcount, c->ntj, c->ntk,
-                     c->tile_start[nb], c->goff);
+  hipLaunchKernelGGL(k_tile_scan, dim3(c->nw), dim3(NT), 0, c->stream, c->tcount, c->ntj,
+                     c->ntk, c->tile_start[nb], c->goff);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_permute, grid1(c->np), dim3(NT), 0, c->stream, make_args(c, 1),
-                     c->goff, c->np, c->py_[nb], c->pz_[nb], c->ppx[nb], c->ppy[nb],
+  hipLaunchKernelGGL(k_permute, grid1(c->np * c->nw), dim3(NT), 0, c->stream, make_args(c, 1),
+                     c->goff, c->np, c->nw, c->py_[nb], c->pz_[nb], c->ppx[nb], c->ppy[nb],
                      c->ppz[nb], c->pcell[nb], c->pid[nb]);
   return hipGetLastError();
 }
 
 hipError_t launch_seed_tiled(qss_ctx *c) {
   const int b = c->cur;
-  hipLaunchKernelGGL(k_seed_tiled, grid1(c->np), dim3(NT), 0, c->stream, c->py_[b],
+  hipLaunchKernelGGL(k_seed_tiled, grid1(c->np * c->nw), dim3(NT), 0, c->stream, c->py_[b],
                      c->pz_[b], c->ppx[b], c->ppy[b], c->ppz[b], c->pcell[b], c->pid[b], c->np,
-                     c->ny, c->ntj, c->p.ppc_y, c->p.ppc_z, c->p.jitter, (uint64_t)c->p.seed);
+                     c->nw, c->ny, c->ntj, c->p.ppc_y, c->p.ppc_z, c->p.jitter,
+                     (uint64_t)c->p.seed);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int nt = c->ntj * c->ntk;
-  hipLaunchKernelGGL(k_uniform_tile_start, grid1(nt + 1), dim3(NT), 0, c->stream,
-                     c->tile_start[b], nt, T * T * c->p.ppc_y * c->p.ppc_z);
+  hipLaunchKernelGGL(k_uniform_tile_start, grid1((long)(nt + 1) * c->nw), dim3(NT), 0,
+                     c->stream, c->tile_start[b], nt, c->nw, T * T * c->p.ppc_y * c->p.ppc_z);
   return hipGetLastError();
 }

@@ -135,7 +135,17 @@ int validate(const qss_params *p) {
     qss_set_error("bad piecewise profile");
     return QSS_E_ARG;
   }
-  long long np = (long long)p->ny * p->nz * p->ppc_y * p->ppc_z;
+  if (p->nwake < 0 || p->nwake > 1024) {
+    qss_set_error("nwake must be in [0, 1024]");
+    return QSS_E_ARG;
+  }
+  if (p->nwake > 1 && (p->ny % kTile != 0 || p->nz % kTile != 0 || p->fix_subcycle ||
+                       (p->precision != QSS_PREC_FP64 && p->precision != QSS_PREC_MIXED32))) {
+    qss_set_error("a batch of wakes needs ny, nz multiples of 16, fix_subcycle = 0 and "
+                  "precision FP64 or MIXED32");
+    return QSS_E_ARG;
+  }
+  long long np = (long long)p->ny * p->nz * p->ppc_y * p->ppc_z * (p->nwake > 1 ? p->nwake : 1);
   if (np > (1LL << 30)) {
     qss_set_error("too many particles per layer");
     return QSS_E_ARG;
@@ -152,7 +162,7 @@ inline int gemm_pick(const qss_ctx *c, int pass, int n) {
 int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
                     const double *const *in, double *const *out, double alpha,
                     const int *resid, const int *push, const double *const *pplane,
-                    double beta = 0.0) {
+                    double beta = 0.0, bool one_wake = false) {
   if (n < 1 || n > kGemmMaxBatch) {
     qss_set_error("transform batch size");
     return QSS_E_ARG;
@@ -161,6 +171,10 @@ int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
   g.M = c->nz; g.N = c->ny; g.K = c->ny;
   g.alpha = 1.0;
   g.nbatch = n;
+  // batched wakes: every plane operand repeats per wake with the plane stride
+  const long nc = c->ncell;
+  g.nw = one_wake ? 1 : c->nw;
+  g.wA = nc; g.wB = 0; g.wC = nc; g.wP = 0; g.wF = 0; g.wR = 0;
   g.F = c->F;
   g.Ff = c->Ff;
   g.resid = c->d_resid;
@@ -186,11 +200,12 @@ int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
     Timed t(c, KC_GEMM);
     HIPCHECK(launch_gemm(g, c->stream, c->p.precision, gemm_pick(c, 0, n)));
   }
-  if (c->timing) c->gemm_flops += 2.0 * g.M * g.N * g.K * n;
+  if (c->timing) c->gemm_flops += 2.0 * g.M * g.N * g.K * n * g.nw;
   g.M = c->nz; g.N = c->ny; g.K = c->nz;
   g.alpha = alpha;
   g.beta = beta;
   g.tout = 0;
+  g.wA = 0; g.wB = nc; g.wC = nc; g.wP = nc; g.wF = nc * kPushComp; g.wR = 6;
   for (int b = 0; b < n; ++b) {
     g.b[b].A = c->d_T[kz[b] == QSS_KIND_C ? 2 : 3];
     g.b[b].B = c->tmp[b];
@@ -206,7 +221,7 @@ int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
     Timed t(c, KC_GEMM);
     HIPCHECK(launch_gemm(g, c->stream, c->p.precision, gemm_pick(c, 1, n)));
   }
-  if (c->timing) c->gemm_flops += 2.0 * g.M * g.N * g.K * n;
+  if (c->timing) c->gemm_flops += 2.0 * g.M * g.N * g.K * n * g.nw;
   return QSS_OK;
 }
 
@@ -231,8 +246,8 @@ int do_guess(qss_ctx *c, int first) {
     // spectra here are the context's state: zero after a reset, the split-0
     // result for later splits of the first layer.
     Timed t(c, KC_SPEC);
-    HIPCHECK(launch_fill_f64(c->S[SP_RHOB], c->ncell, 0.0, c->stream));
-    HIPCHECK(launch_fill_f64(c->S[SP_JXB], c->ncell, 0.0, c->stream));
+    HIPCHECK(launch_fill_f64(c->S[SP_RHOB], c->ncell * c->nw, 0.0, c->stream));
+    HIPCHECK(launch_fill_f64(c->S[SP_JXB], c->ncell * c->nw, 0.0, c->stream));
   }
   // beam P spectrum = previous beam spectrum (:637-646): swap buffers
   std::swap(c->S[SP_RHOBP], c->S[SP_RHOB]);
@@ -309,7 +324,7 @@ int do_iterate(qss_ctx *c, int final_pass, double *res_dev) {
   {
     Timed t(c, KC_SPEC);
     HIPCHECK(launch_iterate_update(c, final_pass));
-    HIPCHECK(launch_fill_u64(c->d_resid, 6, 0ull, c->stream));
+    HIPCHECK(launch_fill_u64(c->d_resid, 6 * c->nw, 0ull, c->stream));
   }
   // E (residual + push field), B (push field); final pass: rho and psi too
   int kz[8] = {QSS_KIND_C, QSS_KIND_C, QSS_KIND_S, QSS_KIND_S, QSS_KIND_S, QSS_KIND_C,
@@ -447,8 +462,9 @@ void adj_free(qss_ctx *c) {
 
 int adj_check(const qss_ctx *c) {
   if (!c->tiled || c->p.precision != QSS_PREC_FP64 || c->p.shape != QSS_SHAPE_CIC ||
-      c->p.nx_split != 1) {
-    qss_set_error("the adjoint sweep needs FP64, CIC, nx_split = 1 and ny, nz multiples of 16");
+      c->p.nx_split != 1 || c->nw != 1) {
+    qss_set_error("the adjoint sweep needs one wake, FP64, CIC, nx_split = 1 and ny, nz "
+                  "multiples of 16");
     return QSS_E_ARG;
   }
   return QSS_OK;
@@ -590,6 +606,71 @@ int adj_layer_step(qss_ctx *c, int slice, char *slot, const double *exbar, int &
   return QSS_OK;
 }
 
+// the finished layer into the history (movepart_splitlayer.cpp:467-473);
+// wake w of a batch at hist + (w * nslices + s) * ncell
+int store_history(qss_ctx *c, int s) {
+  const size_t plane = c->ncell * sizeof(double);
+  for (int f = 0; f < QSS_NFIELD; ++f)
+    if (c->hist[f])
+      HIPCHECK(hipMemcpy2DAsync(c->hist[f] + (size_t)s * c->ncell, plane * c->p.nslices,
+                                c->P[f], plane, plane, c->nw, hipMemcpyDeviceToDevice,
+                                c->stream));
+  return QSS_OK;
+}
+
+void graph_free(qss_ctx *c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+}
+
+// Layers s and s+1 by one graph launch.  The graph holds two layer steps
+// captured from the current pointer state; the layer index is read from
+// d_slice by the beam, residual and history kernels and advanced inside the
+// graph.  A replay from the other pointer parity runs one layer eagerly
+// first (returns 1 then: one layer done).
+int sweep_graph_pair(qss_ctx *c, int s, int *done) {
+  const bool match = c->gexec && c->g_key_c0 == c->C[0] && c->g_key_cur == c->cur &&
+                     c->g_key_rb == c->S[SP_RHOB];
+  if (c->gexec && !match) {
+    CHECK(do_layer_step(c, s, 0, c->d_res_hist + s));
+    CHECK(store_history(c, s));
+    *done = 1;
+    return QSS_OK;
+  }
+  if (!c->gexec) {
+    c->g_key_c0 = c->C[0];
+    c->g_key_cur = c->cur;
+    c->g_key_rb = c->S[SP_RHOB];
+    HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    int r = QSS_OK;
+    for (int q = 0; q < 2 && r == QSS_OK; ++q) {
+      // not the first layer; residual slot of layer 0 (+ d_slice at replay)
+      r = do_layer_step(c, 1, 0, c->d_res_hist);
+      for (int f = 0; f < QSS_NFIELD && r == QSS_OK; ++f)
+        if (c->hist[f] && launch_store_hist(c, c->P[f], c->hist[f]) != hipSuccess)
+          r = QSS_E_HIP;
+      if (r == QSS_OK && launch_add_i32(c->d_slice, 1, c->stream) != hipSuccess) r = QSS_E_HIP;
+    }
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (r != QSS_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (r == QSS_OK) qss_set_error(std::string("graph capture: ") + hipGetErrorString(e));
+      return r != QSS_OK ? r : QSS_E_HIP;
+    }
+    c->graph = g;
+    HIPCHECK(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+  }
+  HIPCHECK(launch_fill_i32(c->d_slice, 1, s, c->stream));
+  HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
+  *done = 2;
+  return QSS_OK;
+}
+
 int check_status(qss_ctx *c) {
   int32_t h[2] = {0, 0};
   HIPCHECK(hipMemcpyAsync(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -603,13 +684,14 @@ int check_status(qss_ctx *c) {
 
 int do_reset(qss_ctx *c) {
   for (int f = 0; f < QSS_NFIELD; ++f) {
-    HIPCHECK(launch_fill_f64(c->C[f], c->ncell, 0.0, c->stream));
-    HIPCHECK(launch_fill_f64(c->P[f], c->ncell, 0.0, c->stream));
+    HIPCHECK(launch_fill_f64(c->C[f], c->ncell * c->nw, 0.0, c->stream));
+    HIPCHECK(launch_fill_f64(c->P[f], c->ncell * c->nw, 0.0, c->stream));
   }
-  for (int s = 0; s < SP_N; ++s) HIPCHECK(launch_fill_f64(c->S[s], c->ncell, 0.0, c->stream));
-  HIPCHECK(launch_fill_f64(c->F, c->ncell * kPushComp, 0.0, c->stream));
+  for (int s = 0; s < SP_N; ++s)
+    HIPCHECK(launch_fill_f64(c->S[s], c->ncell * c->nw, 0.0, c->stream));
+  HIPCHECK(launch_fill_f64(c->F, c->ncell * kPushComp * c->nw, 0.0, c->stream));
   if (c->Ff)   // +0.0f is the all-zero bit pattern
-    HIPCHECK(launch_fill_i32(reinterpret_cast<int32_t *>(c->Ff), c->ncell * kPushComp, 0,
+    HIPCHECK(launch_fill_i32(reinterpret_cast<int32_t *>(c->Ff), c->ncell * kPushComp * c->nw, 0,
                              c->stream));
   HIPCHECK(launch_fill_i32(c->d_counters, 2, 0, c->stream));
   c->cur = 0;
@@ -639,6 +721,14 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
   c->p = *p;
   if (p->nbins > 0) c->bins.assign(p->bins, p->bins + p->nbins);
   c->p.bins = nullptr;
+  c->nw = p->nwake > 1 ? p->nwake : 1;
+  c->wbunch.resize((size_t)c->nw * QSS_MAX_BUNCH);
+  for (int w = 0; w < c->nw; ++w)
+    for (int b = 0; b < QSS_MAX_BUNCH; ++b)
+      c->wbunch[(size_t)w * QSS_MAX_BUNCH + b] =
+          (p->wake_bunch && b < p->nbunch) ? p->wake_bunch[(size_t)w * p->nbunch + b]
+                                            : p->bunch[b];
+  c->p.wake_bunch = nullptr;
   c->device = device;
   c->ny = p->ny; c->nz = p->nz;
   c->ncell = (long)p->ny * p->nz;
@@ -661,7 +751,7 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
     qss_set_error("hipStreamCreate failed");
     return fail(QSS_E_HIP);
   }
-  const long nc = c->ncell;
+  const long nc = c->ncell, nw = c->nw, ncw = nc * nw;
   for (int i = 0; i < 2; ++i) {
     const int n = i == 0 ? c->ny : c->nz;
     if ((r = dalloc(&c->d_T[2 * i], (size_t)n * n)) != QSS_OK) return fail(r);
@@ -675,15 +765,23 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
     }
   }
   for (int f = 0; f < QSS_NFIELD; ++f) {
-    if ((r = dalloc(&c->C[f], nc)) != QSS_OK) return fail(r);
-    if ((r = dalloc(&c->P[f], nc)) != QSS_OK) return fail(r);
+    if ((r = dalloc(&c->C[f], ncw)) != QSS_OK) return fail(r);
+    if ((r = dalloc(&c->P[f], ncw)) != QSS_OK) return fail(r);
   }
   for (int s = 0; s < SP_N; ++s)
-    if ((r = dalloc(&c->S[s], nc)) != QSS_OK) return fail(r);
+    if ((r = dalloc(&c->S[s], ncw)) != QSS_OK) return fail(r);
   for (int b = 0; b < kGemmMaxBatch; ++b)
-    if ((r = dalloc(&c->tmp[b], nc)) != QSS_OK) return fail(r);
-  if ((r = dalloc(&c->F, nc * kPushComp)) != QSS_OK) return fail(r);
-  if (c->mixed && (r = dalloc(&c->Ff, nc * kPushComp)) != QSS_OK) return fail(r);
+    if ((r = dalloc(&c->tmp[b], ncw)) != QSS_OK) return fail(r);
+  if ((r = dalloc(&c->F, ncw * kPushComp)) != QSS_OK) return fail(r);
+  if (c->mixed && (r = dalloc(&c->Ff, ncw * kPushComp)) != QSS_OK) return fail(r);
+  if (nw > 1) {
+    if ((r = dalloc(&c->d_wbunch, c->wbunch.size())) != QSS_OK) return fail(r);
+    if (hipMemcpy(c->d_wbunch, c->wbunch.data(), c->wbunch.size() * sizeof(qss_bunch),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+      qss_set_error("bunch upload failed");
+      return fail(QSS_E_HIP);
+    }
+  }
   c->npc = gemm_pieces(p->precision);
   if (c->npc) {
     for (int i = 0; i < 4; ++i) {
@@ -703,7 +801,7 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
       return fail(QSS_E_HIP);
     }
   }
-  const long np = c->np_cap, ov = c->ov_cap;
+  const long np = c->np_cap * nw, ov = c->ov_cap * nw;
   for (int b = 0; b < 2; ++b) {
     if ((r = dalloc(&c->py_[b], np)) || (r = dalloc(&c->pz_[b], np)) ||
         (r = dalloc(&c->ppx[b], np)) || (r = dalloc(&c->ppy[b], np)) ||
@@ -721,20 +819,22 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
       (r = dalloc(&c->cnt, nc)) || (r = dalloc(&c->cnt_main, nc)) ||
       (r = dalloc(&c->off, nc)) || (r = dalloc(&c->off_main, nc)) ||
       (r = dalloc(&c->blk, c->nblk)) || (r = dalloc(&c->blk_main, c->nblk)) ||
-      (r = dalloc(&c->list, np + ov)) || (r = dalloc(&c->d_resid, 6)) ||
-      (r = dalloc(&c->d_res_hist, (size_t)p->nslices * p->nx_split)))
+      (r = dalloc(&c->list, np + ov)) || (r = dalloc(&c->d_resid, 6 * nw)) ||
+      (r = dalloc(&c->d_slice, 1)) ||
+      (r = dalloc(&c->d_res_hist, (size_t)p->nslices * p->nx_split * nw)))
     return fail(r);
   if (c->tiled) {
     const long nt = (long)c->ntj * c->ntk;
-    if ((r = dalloc(&c->tile_start[0], nt + 1)) || (r = dalloc(&c->tile_start[1], nt + 1)) ||
-        (r = dalloc(&c->win, nt * 441 * 4)) || (r = dalloc(&c->tcount, nt * 9)) ||
-        (r = dalloc(&c->goff, nt * 9)))
+    if ((r = dalloc(&c->tile_start[0], (nt + 1) * nw)) ||
+        (r = dalloc(&c->tile_start[1], (nt + 1) * nw)) ||
+        (r = dalloc(&c->win, nt * 441 * 4 * nw)) || (r = dalloc(&c->tcount, nt * 9 * nw)) ||
+        (r = dalloc(&c->goff, nt * 9 * nw)))
       return fail(r);
   }
   c->hmask = history_mask;
   for (int f = 0; f < QSS_NFIELD; ++f)
     if (history_mask & (1u << f))
-      if ((r = dalloc(&c->hist[f], (size_t)nc * p->nslices)) != QSS_OK) return fail(r);
+      if ((r = dalloc(&c->hist[f], (size_t)ncw * p->nslices)) != QSS_OK) return fail(r);
   // the zero fills of the allocations ran on the null stream, which does not
   // order with the context's non-blocking stream: wait for them
   if (hipDeviceSynchronize() != hipSuccess) {
@@ -762,12 +862,14 @@ int qss_destroy(qss_ctx *c) {
   if (!c) return QSS_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  graph_free(c);
+  dfree(c->d_slice);
   adj_free(c);
   for (auto &p : c->d_T) dfree(p);
   for (int f = 0; f < QSS_NFIELD; ++f) { dfree(c->C[f]); dfree(c->P[f]); dfree(c->hist[f]); }
   for (auto &p : c->S) dfree(p);
   for (auto &p : c->tmp) dfree(p);
-  dfree(c->F); dfree(c->Ff); dfree(c->d_bins);
+  dfree(c->F); dfree(c->Ff); dfree(c->d_bins); dfree(c->d_wbunch);
   for (auto &p : c->Ts) dfree(p);
   for (auto &p : c->xs) dfree(p);
   for (auto &p : c->ts) dfree(p);
@@ -801,6 +903,15 @@ int qss_info(const qss_ctx *c, int64_t *n_particles, int64_t *n_cells) {
 
 void *qss_stream(qss_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
+int qss_select_wake(qss_ctx *c, int wake) {
+  if (!c || wake < 0 || wake >= c->nw) {
+    qss_set_error("wake index out of range");
+    return QSS_E_ARG;
+  }
+  c->sel = wake;
+  return QSS_OK;
+}
+
 int qss_reset(qss_ctx *c) {
   if (!c) return QSS_E_ARG;
   HIPCHECK(hipSetDevice(c->device));
@@ -820,22 +931,27 @@ int qss_sweep_range(qss_ctx *c, int first, int count, double *residuals) {
   if (!c->t0) { HIPCHECK(hipEventCreate(&c->t0)); HIPCHECK(hipEventCreate(&c->t1)); }
   HIPCHECK(hipEventRecord(c->t0, c->stream));
   const int S = c->p.nx_split;
-  for (int s = first; s < first + count; ++s) {
+  for (int s = first; s < first + count;) {
+    if (c->use_graph && !c->timing && S == 1 && s >= 1 && first + count - s >= 2) {
+      int done = 0;
+      CHECK(sweep_graph_pair(c, s, &done));
+      s += done;
+      continue;
+    }
     for (int is = 0; is < S; ++is)
       CHECK(do_layer_step(c, s, is, c->d_res_hist + (size_t)s * S + is));
-    // the finished layer: movepart_splitlayer.cpp:467-473
-    for (int f = 0; f < QSS_NFIELD; ++f)
-      if (c->hist[f])
-        HIPCHECK(hipMemcpyAsync(c->hist[f] + (size_t)s * c->ncell, c->P[f],
-                                c->ncell * sizeof(double), hipMemcpyDeviceToDevice,
-                                c->stream));
+    CHECK(store_history(c, s));
+    ++s;
   }
   HIPCHECK(hipEventRecord(c->t1, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
-  std::vector<double> res((size_t)count * S);
+  // wake-major: wake w's residuals at [w*count*S, (w+1)*count*S)
+  std::vector<double> res((size_t)count * S * c->nw);
+  const size_t row = (size_t)count * S * sizeof(double);
   if (count > 0)
-    HIPCHECK(hipMemcpy(res.data(), c->d_res_hist + (size_t)first * S,
-                       res.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy2D(res.data(), row, c->d_res_hist + (size_t)first * S,
+                         (size_t)c->p.nslices * S * sizeof(double), row, c->nw,
+                         hipMemcpyDeviceToHost));
   if (residuals) std::copy(res.begin(), res.end(), residuals);
   CHECK(check_status(c));
   for (double v : res)
@@ -869,10 +985,23 @@ int qss_copy_history(qss_ctx *c, int field, int first, int count, double *dst_de
     return QSS_E_ARG;
   }
   HIPCHECK(hipSetDevice(c->device));
-  HIPCHECK(hipMemcpyAsync(dst_dev, c->hist[field] + (size_t)first * c->ncell,
-                          (size_t)count * c->ncell * sizeof(double), hipMemcpyDeviceToDevice,
-                          c->stream));
+  HIPCHECK(hipMemcpyAsync(
+      dst_dev, c->hist[field] + ((size_t)c->sel * c->p.nslices + first) * c->ncell,
+      (size_t)count * c->ncell * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
+  return QSS_OK;
+}
+
+int qss_device_memory(int device, int64_t *used, int64_t *total) {
+  int cur = 0;
+  HIPCHECK(hipGetDevice(&cur));
+  HIPCHECK(hipSetDevice(device));
+  size_t fr = 0, tot = 0;
+  const hipError_t e = hipMemGetInfo(&fr, &tot);
+  (void)hipSetDevice(cur);
+  HIPCHECK(e);
+  if (used) *used = (int64_t)(tot - fr);
+  if (total) *total = (int64_t)tot;
   return QSS_OK;
 }
 
@@ -893,11 +1022,7 @@ int qss_sweep_checkpointed(qss_ctx *c, int nrun, int every, double *residuals) {
   for (int s = 0; s < nrun; ++s) {
     if (s % every == 0) CHECK(checkpoint(c, a.chk + (size_t)(s / every) * a.chk_stride, 0));
     CHECK(do_layer_step(c, s, 0, c->d_res_hist + s));
-    for (int f = 0; f < QSS_NFIELD; ++f)
-      if (c->hist[f])
-        HIPCHECK(hipMemcpyAsync(c->hist[f] + (size_t)s * c->ncell, c->P[f],
-                                c->ncell * sizeof(double), hipMemcpyDeviceToDevice,
-                                c->stream));
+    CHECK(store_history(c, s));
   }
   HIPCHECK(hipEventRecord(c->t1, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1052,7 +1177,7 @@ int qss_transform2d_batched(qss_ctx *c, int n, const int *kind_z, const int *kin
     }
   HIPCHECK(hipSetDevice(c->device));
   CHECK(transform_batch(c, n, kind_z, kind_y, in_dev, out_dev, scale, nullptr, nullptr,
-                        nullptr));
+                        nullptr, 0.0, true));
   HIPCHECK(hipStreamSynchronize(c->stream));
   return QSS_OK;
 }
@@ -1061,7 +1186,7 @@ int qss_get_layer_field(qss_ctx *c, int which, int field, double *host) {
   if (!c || !host || field < 0 || field >= QSS_NFIELD || (which != 0 && which != 1))
     return QSS_E_ARG;
   HIPCHECK(hipSetDevice(c->device));
-  double *src = which ? c->P[field] : c->C[field];
+  double *src = (which ? c->P[field] : c->C[field]) + (size_t)c->sel * c->ncell;
   HIPCHECK(hipMemcpyAsync(host, src, c->ncell * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1072,7 +1197,7 @@ int qss_set_layer_field(qss_ctx *c, int which, int field, const double *host) {
   if (!c || !host || field < 0 || field >= QSS_NFIELD || (which != 0 && which != 1))
     return QSS_E_ARG;
   HIPCHECK(hipSetDevice(c->device));
-  double *dst = which ? c->P[field] : c->C[field];
+  double *dst = (which ? c->P[field] : c->C[field]) + (size_t)c->sel * c->ncell;
   HIPCHECK(hipMemcpyAsync(dst, host, c->ncell * sizeof(double), hipMemcpyHostToDevice,
                           c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1085,7 +1210,8 @@ int qss_get_history(qss_ctx *c, int field, int slice, double *host) {
       slice >= c->p.nslices)
     return QSS_E_ARG;
   HIPCHECK(hipSetDevice(c->device));
-  HIPCHECK(hipMemcpy(host, c->hist[field] + (size_t)slice * c->ncell,
+  HIPCHECK(hipMemcpy(host,
+                     c->hist[field] + ((size_t)c->sel * c->p.nslices + slice) * c->ncell,
                      c->ncell * sizeof(double), hipMemcpyDeviceToHost));
   return QSS_OK;
 }
@@ -1093,20 +1219,21 @@ int qss_get_history(qss_ctx *c, int field, int slice, double *host) {
 int qss_layer_ptr(qss_ctx *c, int which, int field, double **dev) {
   if (!c || !dev || field < 0 || field >= QSS_NFIELD || (which != 0 && which != 1))
     return QSS_E_ARG;
-  *dev = which ? c->P[field] : c->C[field];
+  *dev = (which ? c->P[field] : c->C[field]) + (size_t)c->sel * c->ncell;
   return QSS_OK;
 }
 
 int qss_history_ptr(qss_ctx *c, int field, double **dev) {
   if (!c || !dev || field < 0 || field >= QSS_NFIELD || !c->hist[field]) return QSS_E_ARG;
-  *dev = c->hist[field];
+  *dev = c->hist[field] + (size_t)c->sel * c->p.nslices * c->ncell;
   return QSS_OK;
 }
 
 int qss_get_spectral(qss_ctx *c, int id, double *host) {
   if (!c || !host || id < 0 || id >= QSS_NSPECTRAL) return QSS_E_ARG;
   HIPCHECK(hipSetDevice(c->device));
-  HIPCHECK(hipMemcpyAsync(host, c->S[id], c->ncell * sizeof(double), hipMemcpyDeviceToHost,
+  HIPCHECK(hipMemcpyAsync(host, c->S[id] + (size_t)c->sel * c->ncell,
+                          c->ncell * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   return QSS_OK;
@@ -1115,7 +1242,8 @@ int qss_get_spectral(qss_ctx *c, int id, double *host) {
 int qss_set_spectral(qss_ctx *c, int id, const double *host) {
   if (!c || !host || id < 0 || id >= QSS_NSPECTRAL) return QSS_E_ARG;
   HIPCHECK(hipSetDevice(c->device));
-  HIPCHECK(hipMemcpyAsync(c->S[id], host, c->ncell * sizeof(double), hipMemcpyHostToDevice,
+  HIPCHECK(hipMemcpyAsync(c->S[id] + (size_t)c->sel * c->ncell, host,
+                          c->ncell * sizeof(double), hipMemcpyHostToDevice,
                           c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   return QSS_OK;
@@ -1126,20 +1254,20 @@ int qss_get_particles(qss_ctx *c, int32_t *cell_j, int32_t *cell_k, double *fy, 
   if (!c) return QSS_E_ARG;
   HIPCHECK(hipSetDevice(c->device));
   const int b = c->cur;
-  const size_t n = (size_t)c->np;
+  const size_t n = (size_t)c->np, o = (size_t)c->sel * c->np_cap;
   auto get = [&](void *dst, const void *src, size_t bytes) -> int {
     if (!dst) return QSS_OK;
     HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     return QSS_OK;
   };
   std::vector<int32_t> cell(n);
-  CHECK(get(cell.data(), c->pcell[b], n * 4));
-  CHECK(get(fy, c->py_[b], n * 8));
-  CHECK(get(fz, c->pz_[b], n * 8));
-  CHECK(get(px, c->ppx[b], n * 8));
-  CHECK(get(py, c->ppy[b], n * 8));
-  CHECK(get(pz, c->ppz[b], n * 8));
-  CHECK(get(id, c->pid[b], n * 8));
+  CHECK(get(cell.data(), c->pcell[b] + o, n * 4));
+  CHECK(get(fy, c->py_[b] + o, n * 8));
+  CHECK(get(fz, c->pz_[b] + o, n * 8));
+  CHECK(get(px, c->ppx[b] + o, n * 8));
+  CHECK(get(py, c->ppy[b] + o, n * 8));
+  CHECK(get(pz, c->ppz[b] + o, n * 8));
+  CHECK(get(id, c->pid[b] + o, n * 8));
   HIPCHECK(hipStreamSynchronize(c->stream));
   for (size_t i = 0; i < n; ++i) {
     if (cell_j) cell_j[i] = cell[i] % c->ny;
@@ -1154,6 +1282,11 @@ int qss_set_particles(qss_ctx *c, int64_t n, const int32_t *cell_j, const int32_
   if (!c || n < 0 || n > c->np_cap || !cell_j || !cell_k || !fy || !fz || !px || !py ||
       !pz || !id)
     return QSS_E_ARG;
+  if (c->nw > 1 && n != c->np) {
+    qss_set_error("a wake of a batch keeps its particle count");
+    return QSS_E_ARG;
+  }
+  graph_free(c);   // launch sizes follow the particle count
   std::vector<int32_t> cell((size_t)n);
   for (int64_t i = 0; i < n; ++i) {
     if (cell_j[i] < 0 || cell_j[i] >= c->ny || cell_k[i] < 0 || cell_k[i] >= c->nz) {
@@ -1189,16 +1322,17 @@ int qss_set_particles(qss_ctx *c, int64_t n, const int32_t *cell_j, const int32_
   }
   HIPCHECK(hipSetDevice(c->device));
   const int b = c->cur;
-  HIPCHECK(hipMemcpy(c->pcell[b], ocell.data(), n * 4, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(c->py_[b], ofy.data(), n * 8, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(c->pz_[b], ofz.data(), n * 8, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(c->ppx[b], opx.data(), n * 8, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(c->ppy[b], opy.data(), n * 8, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(c->ppz[b], opz.data(), n * 8, hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(c->pid[b], oid.data(), n * 8, hipMemcpyHostToDevice));
+  const size_t o = (size_t)c->sel * c->np_cap;
+  HIPCHECK(hipMemcpy(c->pcell[b] + o, ocell.data(), n * 4, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->py_[b] + o, ofy.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->pz_[b] + o, ofz.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->ppx[b] + o, opx.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->ppy[b] + o, opy.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->ppz[b] + o, opz.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(c->pid[b] + o, oid.data(), n * 8, hipMemcpyHostToDevice));
   if (c->tiled)
-    HIPCHECK(hipMemcpy(c->tile_start[b], tstart.data(), tstart.size() * 4,
-                       hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->tile_start[b] + (size_t)c->sel * (c->ntj * c->ntk + 1),
+                       tstart.data(), tstart.size() * 4, hipMemcpyHostToDevice));
   c->np = n;
   return QSS_OK;
 }
@@ -1210,6 +1344,7 @@ int qss_gemm_variants(int precision) {
 
 int qss_set_gemm_variant(qss_ctx *c, int variant) {
   if (!c || variant < -1 || variant >= gemm_num_variants(c->p.precision)) return QSS_E_ARG;
+  graph_free(c);
   c->gemm_force = variant;
   return QSS_OK;
 }
@@ -1222,8 +1357,31 @@ int qss_gemm_choice(const qss_ctx *c, int *chosen) {
   return QSS_OK;
 }
 
+int qss_enable_graph(qss_ctx *c, int on) {
+  if (!c) return QSS_E_ARG;
+  c->use_graph = on != 0;
+  if (!c->use_graph) graph_free(c);
+  return QSS_OK;
+}
+
+int qss_set_gemm_choice(qss_ctx *c, const int *chosen) {
+  if (!c || !chosen) return QSS_E_ARG;
+  graph_free(c);
+  const int nv = gemm_num_variants(c->p.precision);
+  for (int i = 0; i < 2 * (kGemmMaxBatch + 1); ++i)
+    if (chosen[i] < 0 || chosen[i] >= nv) {
+      qss_set_error("gemm choice out of range");
+      return QSS_E_ARG;
+    }
+  for (int p = 0; p < 2; ++p)
+    for (int b = 0; b <= kGemmMaxBatch; ++b)
+      c->gemm_var[p][b] = chosen[p * (kGemmMaxBatch + 1) + b];
+  return QSS_OK;
+}
+
 int qss_autotune(qss_ctx *c, int reps, float *table_ms) {
   if (!c || reps < 1) return QSS_E_ARG;
+  graph_free(c);
   HIPCHECK(hipSetDevice(c->device));
   hipEvent_t e0, e1;
   HIPCHECK(hipEventCreate(&e0));
@@ -1236,6 +1394,11 @@ int qss_autotune(qss_ctx *c, int reps, float *table_ms) {
       GemmArgs g{};
       g.M = c->nz; g.N = c->ny; g.K = pass == 0 ? c->ny : c->nz;
       g.alpha = 1.0; g.nbatch = b; g.F = c->F; g.Ff = nullptr; g.resid = c->d_resid;
+      // time the launch shape the sweep uses: all wakes of the batch
+      g.nw = c->nw;
+      g.wA = pass == 0 ? c->ncell : 0;
+      g.wB = pass == 0 ? 0 : c->ncell;
+      g.wC = c->ncell;
       g.tout = (c->npc && pass == 0) ? 1 : 0;
       for (int q = 0; q < b; ++q) {
         // operands: any plane of the right size; outputs: the scratch planes

@@ -51,6 +51,10 @@ struct GemmArgs {
   float *Ff;                    // its fp32 form (mixed modes; F unused when set)
   unsigned long long *resid;    // 6 slots: dif[3], max[3] as IEEE bit patterns
   GemmBatch b[kGemmMaxBatch];
+  // batched wakes (dense kernels): the launch covers nw copies of every
+  // field; copy w uses operand + w * stride (stride 0 = shared operand)
+  int nw;
+  long wA, wB, wC, wP, wF, wR;
 };
 // prec = QSS_PREC_*: selects the MFMA type and the variant table
 hipError_t launch_gemm(const GemmArgs &a, hipStream_t s, int prec, int variant);
@@ -105,6 +109,11 @@ struct AdjState {
 struct qss_ctx {
   qss_params p{};
   std::vector<double> bins;
+  std::vector<qss_bunch> wbunch;   // nw * QSS_MAX_BUNCH (per-wake bunches)
+  qss_bunch *d_wbunch = nullptr;
+  int nw = 1;                      // wakes in the batch; per-wake arrays are
+                                   // stored wake-major with a fixed stride
+  int sel = 0;                     // wake of the data-access calls
   int device = 0;
   hipStream_t stream = nullptr;
   int ny = 0, nz = 0;
@@ -168,6 +177,18 @@ struct qss_ctx {
 
   AdjState adj;
 
+  // hipGraph replay of two layer steps (the C/P and particle ping-pong
+  // pointers return to the same state every two layers); the layer index
+  // of a replay is read by the kernels from d_slice
+  bool use_graph = false;
+  bool capturing = false;
+  int *d_slice = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  const double *g_key_c0 = nullptr;   // pointer state the graph was captured in
+  int g_key_cur = -1;
+  const double *g_key_rb = nullptr;
+
   // transform GEMM tile shape per (pass, batch): pass 0 = X.T_y, 1 = T_z.Y
   int gemm_var[2][kGemmMaxBatch + 1] = {};
   int gemm_force = -1;    // >= 0: use this variant for every launch
@@ -201,6 +222,9 @@ hipError_t launch_combine(qss_ctx *c);
 hipError_t launch_tile_reorder(qss_ctx *c);
 hipError_t launch_seed_tiled(qss_ctx *c);
 hipError_t launch_axpy(double *y, const double *x, double a, long n, hipStream_t s);
+// graph replay helpers: history store at the device-side slice, slice += v
+hipError_t launch_store_hist(qss_ctx *c, const double *src, double *hist);
+hipError_t launch_add_i32(int *p, int v, hipStream_t s);
 // adjoints of the spectral updates on the adjoint state A (flag: first layer
 // for the Guess, final pass for the Iterate)
 hipError_t launch_guess_adj(qss_ctx *c, double *const *A, int first_layer);

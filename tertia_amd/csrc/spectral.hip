@@ -34,7 +34,7 @@ __global__ void k_fill_u64(unsigned long long *p, long n, unsigned long long v) 
 
 struct SpecArgs {
   double *S[SP_N];
-  int ny, nz;
+  int ny, nz, nw;
   double ly, lz, hx, dens;
   int flag;  // iterate: final pass
 };
@@ -56,9 +56,11 @@ __device__ inline qss_spec::Mode mode_of(const SpecArgs &a, int j, int k) {
 
 __global__ __launch_bounds__(NT) void k_guess(SpecArgs a) {
   using namespace qss_spec;
-  const long n1 = (long)blockIdx.x * NT + threadIdx.x;
-  if (n1 >= (long)a.ny * a.nz) return;
-  const Mode m = mode_of(a, (int)(n1 % a.ny), (int)(n1 / a.ny));
+  const long n1 = (long)blockIdx.x * NT + threadIdx.x;   // wake-major mode index
+  const long nc = (long)a.ny * a.nz;
+  if (n1 >= nc * a.nw) return;
+  const long mi = n1 % nc;
+  const Mode m = mode_of(a, (int)(mi % a.ny), (int)(mi / a.ny));
   double **S = a.S;
   // P spectra = the spectra left by the previous Iterate (or zero after a
   // reset); the beam P spectrum is the previous beam spectrum (:637-646)
@@ -95,9 +97,11 @@ __global__ __launch_bounds__(NT) void k_guess(SpecArgs a) {
 
 __global__ __launch_bounds__(NT) void k_iterate(SpecArgs a) {
   using namespace qss_spec;
-  const long n1 = (long)blockIdx.x * NT + threadIdx.x;
-  if (n1 >= (long)a.ny * a.nz) return;
-  const Mode m = mode_of(a, (int)(n1 % a.ny), (int)(n1 / a.ny));
+  const long n1 = (long)blockIdx.x * NT + threadIdx.x;   // wake-major mode index
+  const long nc = (long)a.ny * a.nz;
+  if (n1 >= nc * a.nw) return;
+  const long mi = n1 % nc;
+  const Mode m = mode_of(a, (int)(mi % a.ny), (int)(mi / a.ny));
   double **S = a.S;
   double in[II_N], out[IO_N];
   in[II_RP] = S[SP_RHOP][n1];
@@ -131,9 +135,11 @@ __global__ __launch_bounds__(NT) void k_iterate(SpecArgs a) {
 // Guess, the P copies' adjoints included.
 __global__ __launch_bounds__(NT) void k_guess_adj(SpecArgs a) {
   using namespace qss_spec;
-  const long n1 = (long)blockIdx.x * NT + threadIdx.x;
-  if (n1 >= (long)a.ny * a.nz) return;
-  const Mode m = mode_of(a, (int)(n1 % a.ny), (int)(n1 / a.ny));
+  const long n1 = (long)blockIdx.x * NT + threadIdx.x;   // wake-major mode index
+  const long nc = (long)a.ny * a.nz;
+  if (n1 >= nc * a.nw) return;
+  const long mi = n1 % nc;
+  const Mode m = mode_of(a, (int)(mi % a.ny), (int)(mi / a.ny));
   double **A = a.S;
   double ob[GO_N], ib[GI_N];
   ob[GO_RHO] = A[SP_RHO][n1];
@@ -172,9 +178,11 @@ __global__ __launch_bounds__(NT) void k_guess_adj(SpecArgs a) {
 // Reverse of k_iterate (flag: final pass, psi written)
 __global__ __launch_bounds__(NT) void k_iterate_adj(SpecArgs a) {
   using namespace qss_spec;
-  const long n1 = (long)blockIdx.x * NT + threadIdx.x;
-  if (n1 >= (long)a.ny * a.nz) return;
-  const Mode m = mode_of(a, (int)(n1 % a.ny), (int)(n1 / a.ny));
+  const long n1 = (long)blockIdx.x * NT + threadIdx.x;   // wake-major mode index
+  const long nc = (long)a.ny * a.nz;
+  if (n1 >= nc * a.nw) return;
+  const long mi = n1 % nc;
+  const Mode m = mode_of(a, (int)(mi % a.ny), (int)(mi / a.ny));
   double **A = a.S;
   double ob[IO_N], ib[II_N];
   ob[IO_RHO] = A[SP_RHO][n1];
@@ -209,8 +217,15 @@ __global__ __launch_bounds__(NT) void k_iterate_adj(SpecArgs a) {
 }
 
 // Eerr = max_c dif_c / max(1e-5, max_c)  (:1541, :1584-1621, :1641-1643)
-__global__ void k_resid_final(const unsigned long long *r, double *dst) {
-  if (threadIdx.x != 0) return;
+// one thread per wake: slots r[6w..6w+5] -> dst[w * stride]; during a graph
+// replay the layer slot comes from device memory: dst += *slice_dev * per_slice
+__global__ void k_resid_final(const unsigned long long *r, double *dst, int nw, long stride,
+                              const int *slice_dev, int per_slice) {
+  const int w = threadIdx.x;
+  if (w >= nw) return;
+  r += 6 * w;
+  dst += w * stride;
+  if (slice_dev) dst += (long)(*slice_dev) * per_slice;
   double err = 0.0;
   for (int c = 0; c < 3; ++c) {
     double dif = __longlong_as_double((long long)r[c]);
@@ -231,21 +246,22 @@ __global__ __launch_bounds__(NT) void k_push_field(const double *p0, const doubl
                                                   const double *c0, const double *c1,
                                                   const double *c2, const double *c3,
                                                   const double *c4, const double *c5,
-                                                  FT *F, long n) {
-  const long i = (long)blockIdx.x * NT + threadIdx.x;
-  if (i >= n) return;
-  F[i] = (FT)(0.5 * (p0[i] + c0[i]));
-  F[n + i] = (FT)(0.5 * (p1[i] + c1[i]));
-  F[2 * n + i] = (FT)(0.5 * (p2[i] + c2[i]));
-  F[3 * n + i] = (FT)(0.5 * (p3[i] + c3[i]));
-  F[4 * n + i] = (FT)(0.5 * (p4[i] + c4[i]));
-  F[5 * n + i] = (FT)(0.5 * (p5[i] + c5[i]));
+                                                  FT *F, long n, int nw) {
+  const long i = (long)blockIdx.x * NT + threadIdx.x;   // wake-major node index
+  if (i >= n * nw) return;
+  FT *f = F + (i / n) * 6 * n + i % n;   // the wake's six planes
+  f[0] = (FT)(0.5 * (p0[i] + c0[i]));
+  f[n] = (FT)(0.5 * (p1[i] + c1[i]));
+  f[2 * n] = (FT)(0.5 * (p2[i] + c2[i]));
+  f[3 * n] = (FT)(0.5 * (p3[i] + c3[i]));
+  f[4 * n] = (FT)(0.5 * (p4[i] + c4[i]));
+  f[5 * n] = (FT)(0.5 * (p5[i] + c5[i]));
 }
 
 SpecArgs make_args(qss_ctx *c, int flag) {
   SpecArgs a;
   for (int s = 0; s < SP_N; ++s) a.S[s] = c->S[s];
-  a.ny = c->ny; a.nz = c->nz;
+  a.ny = c->ny; a.nz = c->nz; a.nw = c->nw;
   a.ly = c->p.ly; a.lz = c->p.lz;
   a.hx = c->hxs; a.dens = c->p.dens;
   a.flag = flag;
@@ -256,6 +272,20 @@ SpecArgs make_adj_args(qss_ctx *c, double *const *A, int flag) {
   SpecArgs a = make_args(c, flag);
   for (int s = 0; s < SP_N; ++s) a.S[s] = A[s];
   return a;
+}
+
+// history store inside a replayed graph: wake w's plane of `src` into
+// hist + (w * nslices + *slice_dev) * n
+__global__ void k_store_hist(const double *src, double *hist, long n, int nw, int nslices,
+                             const int *slice_dev) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * nw) return;
+  const long w = i / n;
+  hist[(w * nslices + *slice_dev) * n + i % n] = src[i];
+}
+
+__global__ void k_add_i32(int *p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p += v;
 }
 
 __global__ void k_axpy(double *y, const double *x, double alpha, long n) {
@@ -303,30 +333,44 @@ hipError_t launch_fill_u64(unsigned long long *p, long n, unsigned long long v,
 }
 
 hipError_t launch_guess_update(qss_ctx *c, int first_layer) {
-  hipLaunchKernelGGL(k_guess, grid1(c->ncell), dim3(NT), 0, c->stream,
+  hipLaunchKernelGGL(k_guess, grid1(c->ncell * c->nw), dim3(NT), 0, c->stream,
                      make_args(c, first_layer));
   return hipGetLastError();
 }
 
 hipError_t launch_iterate_update(qss_ctx *c, int final_pass) {
-  hipLaunchKernelGGL(k_iterate, grid1(c->ncell), dim3(NT), 0, c->stream,
+  hipLaunchKernelGGL(k_iterate, grid1(c->ncell * c->nw), dim3(NT), 0, c->stream,
                      make_args(c, final_pass));
   return hipGetLastError();
 }
 
 hipError_t launch_residual_final(qss_ctx *c, double *dst) {
-  hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(64), 0, c->stream, c->d_resid, dst);
+  // capturing: dst is the slot of slice 0 and the slice is read at replay
+  hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(((c->nw + 63) / 64) * 64), 0, c->stream,
+                     c->d_resid, dst, c->nw, (long)c->p.nslices * c->p.nx_split,
+                     c->capturing ? c->d_slice : nullptr, c->p.nx_split);
+  return hipGetLastError();
+}
+
+hipError_t launch_store_hist(qss_ctx *c, const double *src, double *hist) {
+  hipLaunchKernelGGL(k_store_hist, grid1(c->ncell * c->nw), dim3(NT), 0, c->stream, src, hist,
+                     c->ncell, c->nw, c->p.nslices, c->d_slice);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_i32(int *p, int v, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_i32, dim3(1), dim3(64), 0, s, p, v);
   return hipGetLastError();
 }
 
 hipError_t launch_push_field(qss_ctx *c) {
   if (c->Ff)
-    hipLaunchKernelGGL(k_push_field<float>, grid1(c->ncell), dim3(NT), 0, c->stream, c->P[0],
+    hipLaunchKernelGGL(k_push_field<float>, grid1(c->ncell * c->nw), dim3(NT), 0, c->stream, c->P[0],
                        c->P[1], c->P[2], c->P[3], c->P[4], c->P[5], c->C[0], c->C[1],
-                       c->C[2], c->C[3], c->C[4], c->C[5], c->Ff, c->ncell);
+                       c->C[2], c->C[3], c->C[4], c->C[5], c->Ff, c->ncell, c->nw);
   else
-    hipLaunchKernelGGL(k_push_field<double>, grid1(c->ncell), dim3(NT), 0, c->stream, c->P[0],
+    hipLaunchKernelGGL(k_push_field<double>, grid1(c->ncell * c->nw), dim3(NT), 0, c->stream, c->P[0],
                        c->P[1], c->P[2], c->P[3], c->P[4], c->P[5], c->C[0], c->C[1],
-                       c->C[2], c->C[3], c->C[4], c->C[5], c->F, c->ncell);
+                       c->C[2], c->C[3], c->C[4], c->C[5], c->F, c->ncell, c->nw);
   return hipGetLastError();
 }

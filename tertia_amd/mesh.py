@@ -98,14 +98,25 @@ class Mesh:
     def MoveAllSplitLayers(self) -> np.ndarray:
         """Seed the front layer, sweep every layer; returns the final-pass
         residual of every layer step (movepart_splitlayer.cpp:25-558)."""
-        res = np.zeros(self.l_Mx * self.params.nx_split)
+        nw = self.params.nwake
+        res = np.zeros(self.l_Mx * self.params.nx_split * nw)
         check(self.L.qss_sweep(self.h, _p(res)), "qss_sweep")
-        return res
+        return res if nw == 1 else res.reshape(nw, -1)
 
     def sweep_range(self, first_slice: int, count: int) -> np.ndarray:
-        res = np.zeros(count * self.params.nx_split)
+        """Sweep layers [first, first+count); residuals (per wake for a batch)."""
+        nw = self.params.nwake
+        res = np.zeros(count * self.params.nx_split * nw)
         check(self.L.qss_sweep_range(self.h, first_slice, count, _p(res)), "qss_sweep_range")
-        return res
+        return res if nw == 1 else res.reshape(nw, -1)
+
+    def enable_graph(self, on: bool = True) -> None:
+        """Replay layer-step pairs from a captured hipGraph in sweeps."""
+        check(self.L.qss_enable_graph(self.h, int(on)), "qss_enable_graph")
+
+    def select_wake(self, w: int) -> None:
+        """Batched context: the wake that field/history/particle access reads."""
+        check(self.L.qss_select_wake(self.h, int(w)), "qss_select_wake")
 
     def reset(self):
         check(self.L.qss_reset(self.h), "qss_reset")
@@ -249,6 +260,15 @@ class Mesh:
         check(self.L.qss_gemm_choice(self.h, arr), "qss_gemm_choice")
         return {"y_pass": list(arr[1:9]), "z_pass": list(arr[10:18])}
 
+    def set_gemm_choice(self, choice: dict) -> None:
+        """Install a tile-shape choice (as returned by gemm_choice/autotune)."""
+        arr = (ctypes.c_int * 18)()
+        for i, v in enumerate(choice["y_pass"]):
+            arr[1 + i] = v
+        for i, v in enumerate(choice["z_pass"]):
+            arr[10 + i] = v
+        check(self.L.qss_set_gemm_choice(self.h, arr), "qss_set_gemm_choice")
+
     def set_gemm_variant(self, v: int):
         check(self.L.qss_set_gemm_variant(self.h, int(v)), "qss_set_gemm_variant")
 
@@ -277,7 +297,7 @@ class Mesh:
     def sweep_vjp(self, ex_bar=None, device_ptr: Optional[int] = None) -> np.ndarray:
         """dLoss/dbins of the preceding checkpointed sweep, given dLoss/dEx of
         its layers (host array (nrun, nz, ny), or a device pointer)."""
-        g = np.zeros(self.params.nbins if self.params.bins is not None else 0)
+        g = np.zeros(len(self.params.bins) if self.params.bins is not None else 0)
         if device_ptr is not None:
             rc = self.L.qss_sweep_vjp(self.h, ctypes.c_void_p(device_ptr), 0, _p(g))
         elif ex_bar is not None:
@@ -292,6 +312,14 @@ class Mesh:
         """Device-to-device copy of history planes into memory at dst_ptr."""
         check(self.L.qss_copy_history(self.h, FIELD_ID[name], int(first), int(count),
                                       ctypes.c_void_p(dst_ptr)), "qss_copy_history")
+
+    @staticmethod
+    def device_memory(device: int = 0) -> tuple:
+        """(used, total) bytes of device memory on `device`."""
+        u, t = ctypes.c_int64(), ctypes.c_int64()
+        check(_lib.load().qss_device_memory(int(device), ctypes.byref(u), ctypes.byref(t)),
+              "qss_device_memory")
+        return u.value, t.value
 
     # -- timing ------------------------------------------------------------
     def enable_timing(self, on: bool = True):

@@ -85,6 +85,8 @@ class QssParams(ctypes.Structure):
         ("nbins", ctypes.c_int32),
         ("bin_xi0", ctypes.c_double), ("bin_xi1", ctypes.c_double),
         ("bins", ctypes.POINTER(ctypes.c_double)),
+        ("nwake", ctypes.c_int32),
+        ("wake_bunch", ctypes.POINTER(QssBunch)),
     ]
 
 
@@ -123,10 +125,17 @@ class Params:
     bins: Optional[Sequence[float]] = None
     bin_xi0: float = 0.0
     bin_xi1: float = 0.0
+    # batch of independent wakes on this grid: per-wake bunch lists (all the
+    # same length as `bunches`); None = one wake
+    wakes: Optional[List[List[Bunch]]] = None
 
     @property
     def n_particles(self) -> int:
         return self.ny * self.nz * self.ppc_y * self.ppc_z
+
+    @property
+    def nwake(self) -> int:
+        return len(self.wakes) if self.wakes else 1
 
     def to_c(self) -> QssParams:
         if len(self.bunches) > QSS_MAX_BUNCH:
@@ -151,6 +160,21 @@ class Params:
         else:
             c.nbins = 0
             c.bins = None
+        c.nwake = self.nwake
+        if self.wakes:
+            nbunch = len(self.bunches)
+            if any(len(w) != nbunch for w in self.wakes):
+                raise ValueError("every wake needs %d bunches" % nbunch)
+            wb = (QssBunch * max(1, len(self.wakes) * nbunch))()
+            for w, bl in enumerate(self.wakes):
+                for i, b in enumerate(bl):
+                    for name in ("nb", "sigma_r", "sigma_xi", "xi0", "y0", "z0", "polarity",
+                                 "vx", "noise"):
+                        setattr(wb[w * nbunch + i], name, getattr(b, name))
+            c.wake_bunch = ctypes.cast(wb, ctypes.POINTER(QssBunch))
+            c._wb_keepalive = wb
+        else:
+            c.wake_bunch = None
         return c
 
 
@@ -182,6 +206,28 @@ def config2(**kw) -> Params:
     for k, v in kw.items():
         setattr(p, k, v)
     return p
+
+
+def config3(seed: int = 20261202, per_size: int = 16, nslices: int = 512,
+            sizes=(128, 192, 256, 384)) -> List[Params]:
+    """Parameter scan: per_size independent wakes on each of the transverse grids
+    `sizes` (cell size 1/16 k_p^-1, so 8, 12, 16, 24 k_p^-1 boxes), 4 ppc,
+    nslices layers of dxi = 0.02.  Each wake's driver has sigma_r ~ U[0.2, 0.8],
+    sigma_xi ~ U[0.5, 1.5], nb ~ U[0.5, 8] drawn from `seed`, centred
+    3 sigma_xi behind the front.  One batched Params per grid size."""
+    import random
+    rng = random.Random(seed)
+    groups = []
+    for n in sizes:
+        wakes = []
+        for _ in range(per_size):
+            sr = rng.uniform(0.2, 0.8)
+            sx = rng.uniform(0.5, 1.5)
+            nb = rng.uniform(0.5, 8.0)
+            wakes.append([Bunch(nb=nb, sigma_r=sr, sigma_xi=sx, xi0=3.0 * sx)])
+        groups.append(Params(ny=n, nz=n, ly=n / 16.0, lz=n / 16.0, hx=0.02, nslices=nslices,
+                             bunches=list(wakes[0]), wakes=wakes))
+    return groups
 
 
 def config4(**kw) -> Params:
