@@ -12,7 +12,10 @@ beam is deposited per slice on the device.
 
 value = particle-slice pushes/s of the plain FP64 sweep = plasma particles
 per slice x timed slices / timed seconds (one push = one particle advanced
-across one slice with all nIter+1 = 3 passes).  The same timed window is run
+across one slice with all nIter+1 = 3 passes), layer-step pairs replayed
+from a captured hipGraph; the same window is also timed with plain launches
+and with a HIP event pair around every launch (the per-kernel times of the
+roofline come from that instrumented window).  The same timed window is run
 in each mixed-precision mode ("mixed": FP32 push with FP32-MFMA or
 split-BF16-MFMA transforms) and each mode's whole sweep is compared with the
 FP64 sweep: relative L2 over every 8th layer of Ex (the longitudinal field,
@@ -53,7 +56,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2028)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-slices", type=int, default=6,
+    ap.add_argument("--cpu-slices", type=int, default=96,
                     help="slices of the CPU baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--mixed", default="mixed32,bf16x6,bf16x3",
@@ -89,36 +92,51 @@ def cpu_baseline(params, nslices, nthreads):
 
 
 def timed_run(mode, a):
-    """Autotune, sweep `warmup` slices, time `steps` slices."""
+    """Autotune, then time `steps` slices after `warmup` slices three ways on
+    the same context: (1) launches bracketed by HIP events (per-kernel times
+    for the roofline), (2) plain launches, (3) layer-step pairs replayed from
+    a captured hipGraph (the production path; `value` is measured here)."""
     from tertia_amd.params import PRECISIONS, config2
     from tertia_amd.mesh import Mesh
     p = config2(precision=PRECISIONS[mode])
     m = Mesh(p, history=())
+    out = {}
     try:
         choice = m.autotune(3)
         npart = m.n_particles
-        m.reset()
-        if a.warmup:
-            m.sweep_range(0, a.warmup)
-        if not a.no_timing_events:
-            m.enable_timing(True)
-        t0 = time.perf_counter()
-        res = m.sweep_range(a.warmup, a.steps)   # synchronises the stream at the end
-        dt = time.perf_counter() - t0
-        tm = m.timing() if not a.no_timing_events else None
+
+        def window(events, graph):
+            m.enable_graph(graph)
+            m.reset()
+            if a.warmup:
+                m.sweep_range(0, a.warmup)
+            m.enable_timing(events)
+            t0 = time.perf_counter()
+            res = m.sweep_range(a.warmup, a.steps)   # synchronises at the end
+            dt = time.perf_counter() - t0
+            tm = m.timing() if events else None
+            m.enable_timing(False)
+            return dt, res, tm
+
+        dt_ev, _, tm = window(True, False) if not a.no_timing_events else (None, None, None)
+        dt_plain, _, _ = window(False, False)
+        dt, res, _ = window(False, True)
     finally:
         m.close()
-    out = {"pushes_per_s": npart * a.steps / dt, "slices_per_s": a.steps / dt,
-           "ms_per_step": dt * 1e3 / a.steps, "max_residual": float(max(res)),
-           "gemm_tiles": choice}
+    out.update({"pushes_per_s": npart * a.steps / dt, "slices_per_s": a.steps / dt,
+                "ms_per_step": dt * 1e3 / a.steps,
+                "ms_per_step_eager": dt_plain * 1e3 / a.steps,
+                "max_residual": float(max(res)), "gemm_tiles": choice})
     if tm is not None:
         gemm_ms = tm["ms"]["gemm"]
         achieved = tm["gemm_flops"] / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+        out["ms_per_step_events"] = dt_ev * 1e3 / a.steps
         out["gemm"] = {"achieved_tflops": achieved, "avg_launch_us":
                        gemm_ms * 1e3 / max(1, tm["launches"]["gemm"]),
                        "launches": tm["launches"]["gemm"], "flops": tm["gemm_flops"],
                        "ms": gemm_ms}
         out["kernel_ms"] = tm["ms"]
+        out["kernel_launches"] = tm["launches"]
         out["event_window_ms"] = tm["ms_total"]
     return out
 
@@ -240,6 +258,11 @@ def batch_run(nslices: int, per_size: int):
             m.reset()
             t0 = time.perf_counter()
             m.sweep_range(0, nslices)
+            tbe = time.perf_counter() - t0
+            m.enable_graph(True)          # the slice step replayed from a hipGraph
+            m.reset()
+            t0 = time.perf_counter()
+            m.sweep_range(0, nslices)
             tb = time.perf_counter() - t0
         finally:
             m.close()
@@ -261,7 +284,7 @@ def batch_run(nslices: int, per_size: int):
             finally:
                 m.close()
         out["groups"].append({"n": p.ny, "wakes": len(p.wakes), "batched_s": tb,
-                              "sequential_s": ts})
+                              "batched_eager_s": tbe, "sequential_s": ts})
         t_b += tb
         t_s += ts
     total = 4 * per_size * nslices
@@ -315,6 +338,8 @@ def main():
         "steps": a.steps,
         "warmup": a.warmup,
         "ms_per_step": base["ms_per_step"],
+        "ms_per_step_eager": base["ms_per_step_eager"],
+        "ms_per_step_with_events": base.get("ms_per_step_events"),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -229,7 +229,10 @@ __device__ inline double unif_pm1(uint64_t seed, uint64_t a, uint64_t b) {
   return (double)(h >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
 }
 
-constexpr int NB = 1024;
+// kBeamAdjBlocks work-groups, each reducing its share of the cells in a fixed
+// order; partial sums land in q[(2*slice + side) * kBeamAdjBlocks + block]
+// and the host adds them in block order
+constexpr int NB = 256;
 __global__ __launch_bounds__(NB) void k_beam_adj(BeamAdjArgs a) {
   __shared__ double red[NB];
   const int t = threadIdx.x;
@@ -237,11 +240,12 @@ __global__ __launch_bounds__(NB) void k_beam_adj(BeamAdjArgs a) {
   const qss_bunch &B = a.b0;
   const double Ry = sqrt(2.0) * B.sigma_r;
   const long ncell = (long)a.ny * a.nz;
+  const long stride = (long)NB * kBeamAdjBlocks;
   for (int side = 0; side < 2; ++side) {
     const int ci = side == 0 ? i : i - 1;
     double part = 0.0;
     if (ci >= 0 && ci <= a.nslices) {
-      for (long c = t; c < ncell; c += NB) {
+      for (long c = (long)blockIdx.x * NB + t; c < ncell; c += stride) {
         const int cj = (int)(c % a.ny), ck = (int)(c / a.ny);
         const double dy = a.hy * cj - a.ly / 2.0 - B.y0;
         const double dz = a.hz * ck - a.lz / 2.0 - B.z0;
@@ -274,7 +278,7 @@ __global__ __launch_bounds__(NB) void k_beam_adj(BeamAdjArgs a) {
       if (t < o) red[t] += red[t + o];
       __syncthreads();
     }
-    if (t == 0) a.q[2 * a.slice + side] = red[0];
+    if (t == 0) a.q[(2L * a.slice + side) * kBeamAdjBlocks + blockIdx.x] = red[0];
     __syncthreads();
   }
 }
@@ -303,6 +307,6 @@ hipError_t launch_beam_adj(qss_ctx *c, int slice, const double *rbar, const doub
   a.seed = c->p.seed;
   a.b0 = c->p.bunch[0];
   a.rbar = rbar; a.jbar = jbar; a.q = q; a.slice = slice;
-  hipLaunchKernelGGL(k_beam_adj, dim3(1), dim3(NB), 0, c->stream, a);
+  hipLaunchKernelGGL(k_beam_adj, dim3(kBeamAdjBlocks), dim3(NB), 0, c->stream, a);
   return hipGetLastError();
 }

@@ -484,7 +484,7 @@ int adj_alloc(qss_ctx *c, int nrun, int K) {
   if ((r = dalloc(&a.pbarEB, nc * kPushComp)) || (r = dalloc(&a.fbar, nc * kPushComp)) ||
       (r = dalloc(&a.jbar, nc * 3)) || (r = dalloc(&a.bbar, nc * 2)) ||
       (r = dalloc(&a.pbar[0], np * 5)) || (r = dalloc(&a.pbar[1], np * 5)) ||
-      (r = dalloc(&a.fwin, nt * 289 * 6)) || (r = dalloc(&a.qbeam, (long)c->p.nslices * 2)) ||
+      (r = dalloc(&a.fwin, nt * 289 * 6)) || (r = dalloc(&a.qbeam, (long)c->p.nslices * 2 * kBeamAdjBlocks)) ||
       (r = dalloc(&a.chk, a.chk_stride * nchk)) || (r = dalloc(&a.tape, a.tape_stride * K)) ||
       (r = dalloc(&a.exbar, (size_t)nrun * nc)))
     return r;
@@ -1059,7 +1059,7 @@ int qss_sweep_vjp(qss_ctx *c, const double *ex_bar, int on_host, double *grad_bi
   HIPCHECK(launch_fill_f64(a.pbarEB, nc * kPushComp, 0.0, c->stream));
   HIPCHECK(launch_fill_f64(a.pbar[0], c->np_cap * 5, 0.0, c->stream));
   HIPCHECK(launch_fill_f64(a.pbar[1], c->np_cap * 5, 0.0, c->stream));
-  HIPCHECK(launch_fill_f64(a.qbeam, (long)c->p.nslices * 2, 0.0, c->stream));
+  HIPCHECK(launch_fill_f64(a.qbeam, (long)c->p.nslices * 2 * kBeamAdjBlocks, 0.0, c->stream));
   int cur_adj = 0;
   for (int seg = a.nchk - 1; seg >= 0; --seg) {
     const int s0 = seg * a.K, s1 = std::min(a.nrun, s0 + a.K);
@@ -1074,8 +1074,10 @@ int qss_sweep_vjp(qss_ctx *c, const double *ex_bar, int on_host, double *grad_bi
   HIPCHECK(hipEventRecord(c->t1, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   CHECK(check_status(c));
-  std::vector<double> q((size_t)a.nrun * 2);
-  HIPCHECK(hipMemcpy(q.data(), a.qbeam, q.size() * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<double> qp((size_t)a.nrun * 2 * kBeamAdjBlocks), q((size_t)a.nrun * 2, 0.0);
+  HIPCHECK(hipMemcpy(qp.data(), a.qbeam, qp.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < q.size(); ++i)   // work-group partials in a fixed order
+    for (int b = 0; b < kBeamAdjBlocks; ++b) q[i] += qp[i * kBeamAdjBlocks + b];
   // layer s receives the profile at xi = (s+1) hx and (s+2) hx (beam.hip)
   const int nb = c->p.nbins, L = c->p.nslices;
   for (int b = 0; b < nb; ++b) grad_bins[b] = 0.0;
