@@ -22,9 +22,14 @@ FP64 sweep: relative L2 over every 8th layer of Ex (the longitudinal field,
 E_z in the usual wakefield notation) and psi.
 
 roofline: the dominant kernel is the transform GEMM; achieved = its
-algorithmic flops (2*M*N*K per GEMM pass and field; 35 2-D transforms of 4N^3
-per slice) / its summed HIP-event time over the timed region (events on the
-context's stream), against MI355X's 78.6 TFLOP/s FP64 MFMA peak.
+algorithmic flops (2*M*N*K per GEMM pass and field: 37 2-D transforms of 4N^3
+per slice) / its summed duration over the instrumented window, each launch
+timed by a HIP event pair that the dispatch itself stamps with its start and
+end (hipExtLaunchKernel, on the context's stream), against MI355X's 78.6
+TFLOP/s FP64 MFMA peak.  traffic = HBM bytes per launch from the committed
+rocprofv3 FETCH_SIZE/WRITE_SIZE summary.  roofline_push: the push+deposit
+kernel against the 8 TB/s HBM peak, its algorithmic bytes per launch being
+Np*52 B of particle state + 16 N^2 planes of field reads / deposit writes.
 
 cpu_baseline: the FP64 oracle (FFT-form DCT-IV, OpenMP) on the host cores,
 on the first slices of the same workload, reported in the same unit.
@@ -64,6 +69,8 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the whole-sweep accuracy comparison of the mixed modes")
     ap.add_argument("--no-timing-events", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time plain launches only (value from the eager window)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint (gradient) timing on the 256^2 workload")
     ap.add_argument("--fd-bins", type=int, default=16,
@@ -72,7 +79,39 @@ def parse():
                     help="skip the batched parameter scan (64 wakes on 4 grid sizes)")
     ap.add_argument("--batch-slices", type=int, default=512)
     ap.add_argument("--batch-wakes", type=int, default=16, help="wakes per grid size")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (a kernel of known "
+                         "HBM traffic that scales the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+def copy_calibration(mib: int) -> int:
+    """Plain device copies of `mib` MiB (read mib, write mib): the reference
+    kernel for turning rocprof's FETCH_SIZE / WRITE_SIZE counters into bytes."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.arange(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_mib": mib, "launches": 5,
+                      "ok": bool(torch.equal(x[-4:], y[-4:]))}), flush=True)
+    return 0
+
+
+def profiled_traffic(name):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy's counters);
+    None when the summary is absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", name + "_pmc.txt")
+    try:
+        with open(path) as f:
+            m = re.search(r"^traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read(), re.M)
+    except OSError:
+        return None
+    return float(m.group(1)) if m else None
 
 
 def cpu_baseline(params, nslices, nthreads):
@@ -119,8 +158,11 @@ def timed_run(mode, a):
             return dt, res, tm
 
         dt_ev, _, tm = window(True, False) if not a.no_timing_events else (None, None, None)
-        dt_plain, _, _ = window(False, False)
-        dt, res, _ = window(False, True)
+        dt_plain, res_plain, _ = window(False, False)
+        if a.no_graph:
+            dt, res = dt_plain, res_plain
+        else:
+            dt, res, _ = window(False, True)
     finally:
         m.close()
     out.update({"pushes_per_s": npart * a.steps / dt, "slices_per_s": a.steps / dt,
@@ -135,6 +177,15 @@ def timed_run(mode, a):
                        gemm_ms * 1e3 / max(1, tm["launches"]["gemm"]),
                        "launches": tm["launches"]["gemm"], "flops": tm["gemm_flops"],
                        "ms": gemm_ms}
+        # push kernel: algorithmic bytes per pass = Np * 52 B (y, z, px, py, pz
+        # + cell id) + N^2 * (12 field planes read + 4 deposit planes written)
+        fb = 4 if mode != "fp64" else 8
+        nbytes = npart * 52 + p.ny * p.nz * 16 * fb
+        push_ms = tm["ms"]["push"]
+        nl = max(1, tm["launches"]["push"])
+        out["push"] = {"avg_launch_us": push_ms * 1e3 / nl, "launches": nl,
+                       "bytes_per_launch": nbytes,
+                       "achieved_gbs": nbytes * nl / (push_ms * 1e-3) / 1e9 if push_ms > 0 else 0.0}
         out["kernel_ms"] = tm["ms"]
         out["kernel_launches"] = tm["launches"]
         out["event_window_ms"] = tm["ms_total"]
@@ -297,9 +348,12 @@ def main():
     a = parse()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
-              "(the xi sweep is sequential; sharding is described in DESIGN.md)" % a.gpus,
+              "(the xi sweep of one wake is sequential; independent wakes would shard "
+              "one batched context per GPU)" % a.gpus,
               file=sys.stderr)
         return 2
+    if a.copy_calibration:
+        return copy_calibration(a.copy_calibration)
     from tertia_amd.params import config2
     if a.warmup + a.steps > config2().nslices:
         print("warmup + steps must be <= %d slices" % config2().nslices, file=sys.stderr)
@@ -340,6 +394,7 @@ def main():
         "ms_per_step": base["ms_per_step"],
         "ms_per_step_eager": base["ms_per_step_eager"],
         "ms_per_step_with_events": base.get("ms_per_step_events"),
+        "value_window": "eager" if a.no_graph else "graph",
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -353,8 +408,17 @@ def main():
         line["roofline"] = {"bound": "mfma", "achieved": g["achieved_tflops"],
                             "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
                             "frac": g["achieved_tflops"] / PEAK_TFLOPS["fp64"],
-                            "traffic": None, "kernel": "k_gemm<MathF64,...> (transform GEMM)",
+                            "traffic": profiled_traffic("k_gemm_fp64_512x512"),
+                            "kernel": "k_gemm<MathF64,...> (transform GEMM)",
                             "avg_launch_us": g["avg_launch_us"], "launches": g["launches"]}
+        pu = base["push"]
+        line["roofline_push"] = {"bound": "hbm", "achieved": pu["achieved_gbs"], "peak": 8000.0,
+                                 "unit": "GB/s", "frac": pu["achieved_gbs"] / 8000.0,
+                                 "traffic": profiled_traffic("k_push_tile_fp64_512x512"),
+                                 "kernel": "k_push_tile<double,double> (push + deposit)",
+                                 "avg_launch_us": pu["avg_launch_us"],
+                                 "launches": pu["launches"],
+                                 "bytes_per_launch": pu["bytes_per_launch"]}
         line["kernel_ms"] = base["kernel_ms"]
         line["event_window_ms"] = base["event_window_ms"]
     if mixed:

@@ -28,6 +28,8 @@
 // (movefieldshydrolinlayersplit.cpp:1584-1621) and the push field planes
 // (P+C)/2 that the particle kernel gathers (fp64, or fp32 in mixed modes).
 
+#include <hip/hip_ext.h>
+
 #include "qss_internal.h"
 
 namespace {
@@ -646,7 +648,8 @@ int gemm_pieces(int prec) {
   return prec == QSS_PREC_MIXED_BF16X3 ? 2 : prec == QSS_PREC_MIXED_BF16X6 ? 3 : 0;
 }
 
-hipError_t launch_gemm(const GemmArgs &a, hipStream_t s, int prec, int variant) {
+hipError_t launch_gemm(const GemmArgs &a, hipStream_t s, int prec, int variant, hipEvent_t ev0,
+                       hipEvent_t ev1) {
   if (a.nbatch < 1 || a.nbatch > kGemmMaxBatch || a.M < 1 || a.N < 1 || a.K < 1)
     return hipErrorInvalidValue;
   const Table tb = table(prec);
@@ -659,13 +662,16 @@ hipError_t launch_gemm(const GemmArgs &a, hipStream_t s, int prec, int variant) 
   const int tiles = ((a.M + v.bm - 1) / v.bm) * ((a.N + v.bn - 1) / v.bn);
   const bool exact = a.M % v.bm == 0 && a.N % v.bn == 0 && a.K % (2 * v.bk) == 0 &&
                      (!tout || a.M % 4 == 0);
-  hipLaunchKernelGGL(v.fn[exact ? 0 : 1][tout], dim3(tiles * a.nbatch * nw), dim3(v.nt), 0, s,
-                     a);
+  const dim3 grid(tiles * a.nbatch * nw), block(v.nt);
+  if (ev0 || ev1)
+    hipExtLaunchKernelGGL(v.fn[exact ? 0 : 1][tout], grid, block, 0, s, ev0, ev1, 0, a);
+  else
+    hipLaunchKernelGGL(v.fn[exact ? 0 : 1][tout], grid, block, 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_split_bf16(const double *const *in, uint16_t *const *out, int nb, long n,
-                             int npc, hipStream_t s) {
+                             int npc, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
   if (nb < 1 || nb > kGemmMaxBatch || n < 1) return hipErrorInvalidValue;
   SplitArgs a{};
   for (int b = 0; b < nb; ++b) {
@@ -674,8 +680,11 @@ hipError_t launch_split_bf16(const double *const *in, uint16_t *const *out, int 
   }
   a.n = n;
   const dim3 grid((unsigned)((n + 1023) / 1024), (unsigned)nb);
-  if (npc == 2) hipLaunchKernelGGL(k_split<2>, grid, dim3(256), 0, s, a);
-  else if (npc == 3) hipLaunchKernelGGL(k_split<3>, grid, dim3(256), 0, s, a);
-  else return hipErrorInvalidValue;
+  if (npc != 2 && npc != 3) return hipErrorInvalidValue;
+  if (ev0 || ev1)
+    hipExtLaunchKernelGGL(npc == 2 ? k_split<2> : k_split<3>, grid, dim3(256), 0, s, ev0, ev1,
+                          0, a);
+  else
+    hipLaunchKernelGGL(npc == 2 ? k_split<2> : k_split<3>, grid, dim3(256), 0, s, a);
   return hipGetLastError();
 }

@@ -28,6 +28,8 @@
 // order, each group in source slot order.  The storage order, and so every
 // summation order, is a pure function of the input.
 
+#include <hip/hip_ext.h>
+
 #include "qss_internal.h"
 #include "tile_sort.h"
 
@@ -564,14 +566,13 @@ TileArgs make_args(qss_ctx *c, int full) {
 
 }  // namespace
 
-hipError_t launch_push_tile(qss_ctx *c, int full) {
-  const int nt = c->ntj * c->ntk;
-  if (c->mixed)
-    hipLaunchKernelGGL((k_push_tile<float, float>), dim3(nt, c->nw), dim3(NT), 0, c->stream,
-                       make_args(c, full));
+hipError_t launch_push_tile(qss_ctx *c, int full, hipEvent_t ev0, hipEvent_t ev1) {
+  const dim3 grid(c->ntj * c->ntk, c->nw), block(NT);
+  auto fn = c->mixed ? k_push_tile<float, float> : k_push_tile<double, double>;
+  if (ev0 || ev1)
+    hipExtLaunchKernelGGL(fn, grid, block, 0, c->stream, ev0, ev1, 0, make_args(c, full));
   else
-    hipLaunchKernelGGL((k_push_tile<double, double>), dim3(nt, c->nw), dim3(NT), 0, c->stream,
-                       make_args(c, full));
+    hipLaunchKernelGGL(fn, grid, block, 0, c->stream, make_args(c, full));
   return hipGetLastError();
 }
 

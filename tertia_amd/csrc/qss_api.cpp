@@ -42,11 +42,16 @@ thread_local std::string g_err;
 constexpr int kNClass = QSS_NKCLASS;
 
 // ---- timing helpers --------------------------------------------------------
+// Timed(c, cls): event markers around the launches of the scope.
+// Timed(c, cls, true): an event pair for ONE launch that takes them itself
+// (hipExtLaunchKernel stamps them with the dispatch's start and end), so the
+// measured time is the kernel's own duration.
 struct Timed {
   qss_ctx *c;
   int cls;
-  hipEvent_t e1 = nullptr;
-  Timed(qss_ctx *c_, int cls_) : c(c_), cls(cls_) {
+  bool ext;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  Timed(qss_ctx *c_, int cls_, bool ext_ = false) : c(c_), cls(cls_), ext(ext_) {
     if (!c->timing) return;
     auto &pool = c->ev_pool[cls];
     size_t &used = c->ev_used[cls];
@@ -58,12 +63,13 @@ struct Timed {
       }
       pool.emplace_back(a, b);
     }
-    (void)hipEventRecord(pool[used].first, c->stream);
+    e0 = pool[used].first;
     e1 = pool[used].second;
     ++used;
+    if (!ext) (void)hipEventRecord(e0, c->stream);
   }
   ~Timed() {
-    if (e1) (void)hipEventRecord(e1, c->stream);
+    if (e1 && !ext) (void)hipEventRecord(e1, c->stream);
   }
 };
 
@@ -181,8 +187,8 @@ int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
   if (c->npc) {
     // split-bf16: the inputs become npc bf16 planes; the y pass leaves its
     // result transposed and split for the z pass
-    Timed t(c, KC_GEMM);
-    HIPCHECK(launch_split_bf16(in, c->xs, n, c->ncell, c->npc, c->stream));
+    Timed t(c, KC_GEMM, true);
+    HIPCHECK(launch_split_bf16(in, c->xs, n, c->ncell, c->npc, c->stream, t.e0, t.e1));
     g.tout = 1;
   }
   for (int b = 0; b < n; ++b) {
@@ -197,8 +203,8 @@ int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
     g.b[b].push = -1;
   }
   {
-    Timed t(c, KC_GEMM);
-    HIPCHECK(launch_gemm(g, c->stream, c->p.precision, gemm_pick(c, 0, n)));
+    Timed t(c, KC_GEMM, true);
+    HIPCHECK(launch_gemm(g, c->stream, c->p.precision, gemm_pick(c, 0, n), t.e0, t.e1));
   }
   if (c->timing) c->gemm_flops += 2.0 * g.M * g.N * g.K * n * g.nw;
   g.M = c->nz; g.N = c->ny; g.K = c->nz;
@@ -218,8 +224,8 @@ int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
     g.b[b].P = pplane ? pplane[b] : nullptr;
   }
   {
-    Timed t(c, KC_GEMM);
-    HIPCHECK(launch_gemm(g, c->stream, c->p.precision, gemm_pick(c, 1, n)));
+    Timed t(c, KC_GEMM, true);
+    HIPCHECK(launch_gemm(g, c->stream, c->p.precision, gemm_pick(c, 1, n), t.e0, t.e1));
   }
   if (c->timing) c->gemm_flops += 2.0 * g.M * g.N * g.K * n * g.nw;
   return QSS_OK;
@@ -279,8 +285,8 @@ int do_guess(qss_ctx *c, int first) {
 int do_push_deposit(qss_ctx *c, int full) {
   if (c->tiled) {
     {
-      Timed t(c, KC_PUSH);
-      HIPCHECK(launch_push_tile(c, full));
+      Timed t(c, KC_PUSH, true);
+      HIPCHECK(launch_push_tile(c, full, t.e0, t.e1));
     }
     {
       Timed t(c, KC_REDUCE);
@@ -323,8 +329,7 @@ int do_iterate(qss_ctx *c, int final_pass, double *res_dev) {
   }
   {
     Timed t(c, KC_SPEC);
-    HIPCHECK(launch_iterate_update(c, final_pass));
-    HIPCHECK(launch_fill_u64(c->d_resid, 6 * c->nw, 0ull, c->stream));
+    HIPCHECK(launch_iterate_update(c, final_pass));   // also clears the residual slots
   }
   // E (residual + push field), B (push field); final pass: rho and psi too
   int kz[8] = {QSS_KIND_C, QSS_KIND_C, QSS_KIND_S, QSS_KIND_S, QSS_KIND_S, QSS_KIND_C,

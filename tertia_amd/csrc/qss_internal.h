@@ -57,13 +57,17 @@ struct GemmArgs {
   long wA, wB, wC, wP, wF, wR;
 };
 // prec = QSS_PREC_*: selects the MFMA type and the variant table
-hipError_t launch_gemm(const GemmArgs &a, hipStream_t s, int prec, int variant);
+// ev0/ev1 (optional): events that receive the kernel's own start/end
+// timestamps (hipExtLaunchKernel), no marker packets around the dispatch
+hipError_t launch_gemm(const GemmArgs &a, hipStream_t s, int prec, int variant,
+                       hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 int gemm_num_variants(int prec);
 const char *gemm_variant_name(int prec, int v);
 int gemm_pieces(int prec);   // bf16 pieces per operand element (0: not split)
 // out[b] (npc planes of n bf16) = pieces of in[b] (n doubles), b < nb
 hipError_t launch_split_bf16(const double *const *in, uint16_t *const *out, int nb, long n,
-                             int npc, hipStream_t s);
+                             int npc, hipStream_t s, hipEvent_t ev0 = nullptr,
+                             hipEvent_t ev1 = nullptr);
 
 // ---------------------------------------------------------------------------
 // adjoint sweep (tertia_amd/csrc/adjoint.hip, qss_sweep_vjp)
@@ -217,7 +221,8 @@ hipError_t launch_scatter(qss_ctx *c);
 hipError_t launch_tile_reduce(qss_ctx *c, int full_step);
 hipError_t launch_residual_final(qss_ctx *c, double *dst);
 hipError_t launch_push_field(qss_ctx *c);
-hipError_t launch_push_tile(qss_ctx *c, int full);
+hipError_t launch_push_tile(qss_ctx *c, int full, hipEvent_t ev0 = nullptr,
+                            hipEvent_t ev1 = nullptr);
 hipError_t launch_combine(qss_ctx *c);
 hipError_t launch_tile_reorder(qss_ctx *c);
 hipError_t launch_seed_tiled(qss_ctx *c);

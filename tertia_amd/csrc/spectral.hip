@@ -37,6 +37,9 @@ struct SpecArgs {
   int ny, nz, nw;
   double ly, lz, hx, dens;
   int flag;  // iterate: final pass
+  // iterate: residual slots (6 per wake) cleared for the inverse-GEMM
+  // epilogues that follow on the stream (null: untouched)
+  unsigned long long *resid;
 };
 
 __device__ inline void wavenumbers(const SpecArgs &a, int j, int k, double &aky,
@@ -99,6 +102,7 @@ __global__ __launch_bounds__(NT) void k_iterate(SpecArgs a) {
   using namespace qss_spec;
   const long n1 = (long)blockIdx.x * NT + threadIdx.x;   // wake-major mode index
   const long nc = (long)a.ny * a.nz;
+  if (a.resid && n1 < 6L * a.nw) a.resid[n1] = 0ull;
   if (n1 >= nc * a.nw) return;
   const long mi = n1 % nc;
   const Mode m = mode_of(a, (int)(mi % a.ny), (int)(mi / a.ny));
@@ -265,6 +269,7 @@ SpecArgs make_args(qss_ctx *c, int flag) {
   a.ly = c->p.ly; a.lz = c->p.lz;
   a.hx = c->hxs; a.dens = c->p.dens;
   a.flag = flag;
+  a.resid = nullptr;
   return a;
 }
 
@@ -339,8 +344,10 @@ hipError_t launch_guess_update(qss_ctx *c, int first_layer) {
 }
 
 hipError_t launch_iterate_update(qss_ctx *c, int final_pass) {
-  hipLaunchKernelGGL(k_iterate, grid1(c->ncell * c->nw), dim3(NT), 0, c->stream,
-                     make_args(c, final_pass));
+  SpecArgs a = make_args(c, final_pass);
+  a.resid = c->d_resid;
+  const long n = c->ncell > 6 ? c->ncell : 6;   // threads for every mode and slot
+  hipLaunchKernelGGL(k_iterate, grid1(n * c->nw), dim3(NT), 0, c->stream, a);
   return hipGetLastError();
 }
 
