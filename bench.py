@@ -341,6 +341,41 @@ def batch_run(nslices: int, per_size: int):
     total = 4 * per_size * nslices
     out.update({"batched_slices_per_s": total / t_b, "sequential_slices_per_s": total / t_s,
                 "speedup": t_s / t_b})
+    # all four grid sizes at once: one batched context per size, each on its
+    # own stream, swept from its own host thread (the library releases the
+    # interpreter during a sweep), so the sizes' launches overlap on the GPU
+    import threading
+    ms = []
+    try:
+        for p in groups:
+            m = Mesh(p, history=())
+            ms.append(m)
+            m.autotune(2)
+            m.enable_graph(True)
+            m.reset()
+            m.sweep_range(0, 3)          # captures the graph before the timed threads
+            m.reset()
+        errs = []
+
+        def run(m):
+            try:
+                m.sweep_range(0, nslices)
+            except Exception as e:   # reported after the join
+                errs.append(e)
+        th = [threading.Thread(target=run, args=(m,)) for m in ms]
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        tc = time.perf_counter() - t0
+        if errs:
+            raise errs[0]
+    finally:
+        for m in ms:
+            m.close()
+    out.update({"concurrent_s": tc, "concurrent_slices_per_s": total / tc,
+                "speedup_concurrent": t_s / tc})
     return out
 
 

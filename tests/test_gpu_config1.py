@@ -29,6 +29,7 @@ def test_config1_blowout_vs_oracle(oracle_mod):
         rg = m.MoveAllSplitLayers()
         st, ro = o.sweep()
         assert st == 0
+        print("config1 residual max diff %.1e" % np.abs(rg - ro).max(), flush=True)
         assert np.abs(rg - ro).max() < 1e-8
         stats = {}
         for f in FIELDS:
