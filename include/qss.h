@@ -206,7 +206,12 @@ int  qss_transform2d_batched(qss_ctx *c, int n, const int *kind_z,
  * (movefieldshydrolinlayersplit.cpp:389-427).  Here the transform is a
  * batched MFMA GEMM compiled for several tile shapes; qss_autotune times
  * every shape on this context's buffers for every launch shape the sweep
- * uses (1..8 fields, both passes) and keeps the fastest.  reps >= 1.       */
+ * uses (1..8 fields, both passes, the 6- and 8-field inverse passes with
+ * their corrector epilogue) and keeps the fastest.  It also picks the
+ * particles per work-group of the tiled push (qss_set_particle_split) by
+ * timing a predictor pass.  It overwrites only scratch state (transform
+ * intermediates, the push field and residual slots that the next layer step
+ * rebuilds).  reps >= 1.                                                   */
 int  qss_autotune(qss_ctx *c, int reps, float *table_ms);
 /* table_ms (may be NULL): 2*8*qss_gemm_variants(precision) floats, the mean
  * time of one launch for [pass][nbatch-1][variant].                         */
@@ -220,6 +225,15 @@ int  qss_gemm_choice(const qss_ctx *c, int *chosen);
 /* Installs a choice table (same layout), e.g. one tuned on another context
  * of the same grid and precision.                                           */
 int  qss_set_gemm_choice(qss_ctx *c, const int *chosen);
+/* Particles per work-group of the tiled push: a 16x16-cell tile's particles
+ * are shared by nsplit = 1, 2 or 4 work-groups (each takes every nsplit-th
+ * chunk of 256).  No reference counterpart (the reference launches one
+ * thread per particle, plasma_particles.cu:761-863).  The default depends on
+ * the grid only (1 from 1024 tiles, 2 from 512, else 4); results are
+ * bitwise reproducible for a given nsplit and differ across nsplit by
+ * summation order only.  qss_particle_split returns the current value.    */
+int  qss_set_particle_split(qss_ctx *c, int nsplit);
+int  qss_particle_split(const qss_ctx *c);
 
 /* ---- data access (host buffers, Ny*Nz doubles, synchronous) ------------ */
 /* which: 0 = C layer, 1 = P layer.  field: QSS_EX..QSS_PSI. */

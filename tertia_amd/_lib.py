@@ -65,6 +65,8 @@ SIGNATURES = {
     "qss_device_memory": (ctypes.c_int, [ctypes.c_int, _i64p, _i64p]),
     "qss_select_wake": (ctypes.c_int, [_vp, ctypes.c_int]),
     "qss_set_gemm_choice": (ctypes.c_int, [_vp, _ip]),
+    "qss_set_particle_split": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "qss_particle_split": (ctypes.c_int, [_vp]),
     "qss_enable_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "qss_timing": (ctypes.c_int, [_vp, _dp, _dp, _i64p, _dp]),
 }

@@ -62,9 +62,10 @@ struct TileArgs {
   double *sy, *sz, *spx, *spy, *spz;
   int32_t *scell, *sdest;  // sdest = direction | rank << 4
   int64_t *spid;
-  int32_t *tcount;         // ntiles x 9
+  int32_t *tcount;         // ntiles x nsplit x 9
   int32_t *counters;
   long npw;                // particle slots per wake (batched contexts)
+  int nsplit;              // work-groups per tile (each takes every nsplit-th chunk)
 };
 
 __device__ inline int wrapi(int a, int n) {
@@ -83,10 +84,10 @@ __host__ __device__ inline TileArgs wake_view(const TileArgs &a0, int w) {
   a.tile_start += w * (nt + 1);
   if (a.F) a.F += w * kPushComp * plane;
   if (a.Ff) a.Ff += w * kPushComp * plane;
-  a.win += w * nt * NWIN * 4;
+  a.win += w * nt * a0.nsplit * NWIN * 4;
   a.sy += pw; a.sz += pw; a.spx += pw; a.spy += pw; a.spz += pw;
   a.scell += pw; a.sdest += pw; a.spid += pw;
-  a.tcount += w * nt * 9;
+  a.tcount += w * nt * a0.nsplit * 9;
   return a;
 }
 
@@ -98,8 +99,16 @@ using qss_tile::sort256;
 // displacement in float and adds them to the double state, and the deposit
 // weights are summed in double.  With R = double every expression below
 // evaluates exactly as the reference's (no contraction, same association).
+//
+// A tile's particles may be shared by nsplit work-groups (part p takes
+// chunks p, p + nsplit, ...): each part writes its own window, direction
+// counts and ranks, and the combine / reorder kernels visit the parts in
+// order, so results are a pure function of the input and nsplit.
+// 128 VGPRs at most: four waves per SIMD, so all the tiles of a 512^2 grid
+// (4 work-groups per CU) are resident at once.
 template <typename R, typename FT>
-__global__ __launch_bounds__(NT) void k_push_tile(TileArgs a0) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
+void k_push_tile(TileArgs a0) {
   const TileArgs a = wake_view(a0, blockIdx.y);   // wake of a batch
   __shared__ FT Fw[NF * 6];
   __shared__ int keys_sh[NT];
@@ -112,7 +121,7 @@ __global__ __launch_bounds__(NT) void k_push_tile(TileArgs a0) {
   __shared__ int flag_sh;
 
   const int t = threadIdx.x;
-  const int tile = blockIdx.x;
+  const int tile = blockIdx.x / a.nsplit, part = blockIdx.x - tile * a.nsplit;
   const int tj = tile % a.ntj, tk = tile / a.ntj;
   const int j0 = tj * T, k0 = tk * T;
   const int ny = a.ny, nz = a.nz;
@@ -139,7 +148,7 @@ __global__ __launch_bounds__(NT) void k_push_tile(TileArgs a0) {
 
   const int base = a.tile_start[tile];
   const int np = a.tile_start[tile + 1] - base;
-  for (int c0 = 0; c0 < np; c0 += NT) {
+  for (int c0 = part * NT; c0 < np; c0 += a.nsplit * NT) {
     const int slot = base + c0 + t;
     const bool active = c0 + t < np;
     int j = 0, k = 0;
@@ -364,7 +373,7 @@ __global__ __launch_bounds__(NT) void k_push_tile(TileArgs a0) {
         a.spz[slot] = pz;
         a.scell[slot] = jj + ny * kk;
         a.spid[slot] = a.pid[slot];
-        a.sdest[slot] = dir | ((off + myrank_w) << 4);
+        a.sdest[slot] = dir | (part << 4) | ((off + myrank_w) << 8);
       }
       __syncthreads();
       if (t < 9) {
@@ -376,21 +385,21 @@ __global__ __launch_bounds__(NT) void k_push_tile(TileArgs a0) {
     }
   }
   // 4. window to HBM
-  double *wout = a.win + (long)tile * NWIN * 4;
+  double *wout = a.win + (long)blockIdx.x * NWIN * 4;
   reinterpret_cast<double4 *>(wout)[n0] = make_double4(acc0[0], acc0[1], acc0[2], acc0[3]);
   if (n1 < NWIN)
     reinterpret_cast<double4 *>(wout)[n1] = make_double4(acc1[0], acc1[1], acc1[2], acc1[3]);
-  if (a.full && t < 9) a.tcount[tile * 9 + t] = dir_total[t];
+  if (a.full && t < 9) a.tcount[blockIdx.x * 9 + t] = dir_total[t];
 }
 
 // sum of the windows covering each node, fixed (dtk, dtj) order
 __global__ __launch_bounds__(NT) void k_combine(const double *win, int ntj, int ntk, int ny,
-                                                int nz, int nw, double *J0, double *J1,
-                                                double *J2, double *J3) {
+                                                int nz, int nw, int nsplit, double *J0,
+                                                double *J1, double *J2, double *J3) {
   const long n = (long)blockIdx.x * NT + threadIdx.x;   // wake-major node index
   const long plane = (long)ny * nz;
   if (n >= plane * nw) return;
-  win += (n / plane) * ntj * ntk * NWIN * 4;   // the wake's windows
+  win += (n / plane) * ntj * ntk * nsplit * NWIN * 4;   // the wake's windows
   const long n1 = n % plane;
   const int nj = (int)(n1 % ny), nk = (int)(n1 / ny);
   const int tj = nj / T, tk = nk / T;
@@ -405,36 +414,39 @@ __global__ __launch_bounds__(NT) void k_combine(const double *win, int ntj, int 
       const int lj = nj - (utj * T - H);
       if (lj < 0 || lj >= W) continue;
       const int wtj = wrapi(utj, ntj);
-      const double4 v =
-          reinterpret_cast<const double4 *>(win + ((long)(wtk * ntj + wtj) * NWIN) * 4)[lk * W + lj];
-      s[0] += v.x; s[1] += v.y; s[2] += v.z; s[3] += v.w;
+      for (int q = 0; q < nsplit; ++q) {
+        const long wi = (long)(wtk * ntj + wtj) * nsplit + q;
+        const double4 v = reinterpret_cast<const double4 *>(win + wi * NWIN * 4)[lk * W + lj];
+        s[0] += v.x; s[1] += v.y; s[2] += v.z; s[3] += v.w;
+      }
     }
   }
   J0[n] = s[0]; J1[n] = s[1]; J2[n] = s[2]; J3[n] = s[3];
 }
 
 // new tile starts and group offsets: destination tile u receives the groups
-// of source tiles u - dir(d) for d = 0..8 in that order
+// of source tiles u - dir(d) for d = 0..8 in that order, each as the groups
+// of the source's parts 0..nsplit-1
 __global__ __launch_bounds__(NT) void k_tile_scan(const int32_t *tcount, int ntj, int ntk,
-                                                  int32_t *tile_start, int32_t *goff) {
+                                                  int nsplit, int32_t *tile_start,
+                                                  int32_t *goff) {
   __shared__ int sh[NT];
   const int nt = ntj * ntk;
   const int w = blockIdx.x;   // one work-group per wake
-  tcount += (long)w * nt * 9;
-  goff += (long)w * nt * 9;
+  tcount += (long)w * nt * nsplit * 9;
+  goff += (long)w * nt * nsplit * 9;
   tile_start += (long)w * (nt + 1);
   int carry = 0;
   for (int b0 = 0; b0 < nt; b0 += NT) {
     const int u = b0 + threadIdx.x;
     int tot = 0;
-    int g[9];
+    int src[9];
     if (u < nt) {
       const int uj = u % ntj, uk = u / ntj;
       for (int d = 0; d < 9; ++d) {
         const int dj = d % 3 - 1, dk = d / 3 - 1;
-        const int src = wrapi(uk - dk, ntk) * ntj + wrapi(uj - dj, ntj);
-        g[d] = tcount[src * 9 + d];
-        tot += g[d];
+        src[d] = wrapi(uk - dk, ntk) * ntj + wrapi(uj - dj, ntj);
+        for (int q = 0; q < nsplit; ++q) tot += tcount[(src[d] * nsplit + q) * 9 + d];
       }
     }
     // block exclusive scan of tot
@@ -451,10 +463,11 @@ __global__ __launch_bounds__(NT) void k_tile_scan(const int32_t *tcount, int ntj
     if (u < nt) {
       tile_start[u] = start;
       int o = start;
-      for (int d = 0; d < 9; ++d) {
-        goff[u * 9 + d] = o;
-        o += g[d];
-      }
+      for (int d = 0; d < 9; ++d)
+        for (int q = 0; q < nsplit; ++q) {
+          goff[(u * 9 + d) * nsplit + q] = o;
+          o += tcount[(src[d] * nsplit + q) * 9 + d];
+        }
     }
     const int blk_tot = sh[NT - 1];
     __syncthreads();
@@ -472,13 +485,13 @@ __global__ __launch_bounds__(NT) void k_permute(TileArgs a0, const int32_t *goff
   const int w = (int)(sg / np);
   const long s = sg - (long)w * np;
   const TileArgs a = wake_view(a0, w);
-  goff += (long)w * a.ntj * a.ntk * 9;
+  goff += (long)w * a.ntj * a.ntk * a.nsplit * 9;
   const long pw = (long)w * a0.npw;
   y += pw; z += pw; px += pw; py += pw; pz += pw; cell += pw; pid += pw;
   const int32_t c = a.scell[s];
   const int dst_tile = ((c / a.ny) / T) * a.ntj + (c % a.ny) / T;
-  const int sd = a.sdest[s];
-  const long o = goff[dst_tile * 9 + (sd & 15)] + (sd >> 4);
+  const int sd = a.sdest[s];   // direction | part << 4 | rank << 8
+  const long o = goff[(dst_tile * 9 + (sd & 15)) * a.nsplit + ((sd >> 4) & 15)] + (sd >> 8);
   y[o] = a.sy[s];
   z[o] = a.sz[s];
   px[o] = a.spx[s];
@@ -561,13 +574,14 @@ TileArgs make_args(qss_ctx *c, int full) {
   a.tcount = c->tcount;
   a.counters = c->d_counters;
   a.npw = c->np_cap;
+  a.nsplit = c->nsplit;
   return a;
 }
 
 }  // namespace
 
 hipError_t launch_push_tile(qss_ctx *c, int full, hipEvent_t ev0, hipEvent_t ev1) {
-  const dim3 grid(c->ntj * c->ntk, c->nw), block(NT);
+  const dim3 grid(c->ntj * c->ntk * c->nsplit, c->nw), block(NT);
   auto fn = c->mixed ? k_push_tile<float, float> : k_push_tile<double, double>;
   if (ev0 || ev1)
     hipExtLaunchKernelGGL(fn, grid, block, 0, c->stream, ev0, ev1, 0, make_args(c, full));
@@ -576,17 +590,19 @@ hipError_t launch_push_tile(qss_ctx *c, int full, hipEvent_t ev0, hipEvent_t ev1
   return hipGetLastError();
 }
 
-hipError_t launch_combine(qss_ctx *c) {
+hipError_t launch_combine(qss_ctx *c, double *const *out) {
+  double *const J[4] = {c->C[QSS_JX], c->C[QSS_JY], c->C[QSS_JZ], c->C[QSS_RHO]};
+  if (!out) out = J;
   hipLaunchKernelGGL(k_combine, grid1(c->ncell * c->nw), dim3(NT), 0, c->stream, c->win,
-                     c->ntj, c->ntk, c->ny, c->nz, c->nw, c->C[QSS_JX], c->C[QSS_JY],
-                     c->C[QSS_JZ], c->C[QSS_RHO]);
+                     c->ntj, c->ntk, c->ny, c->nz, c->nw, c->nsplit, out[0], out[1], out[2],
+                     out[3]);
   return hipGetLastError();
 }
 
 hipError_t launch_tile_reorder(qss_ctx *c) {
   const int nb = c->cur ^ 1;
   hipLaunchKernelGGL(k_tile_scan, dim3(c->nw), dim3(NT), 0, c->stream, c->tcount, c->ntj,
-                     c->ntk, c->tile_start[nb], c->goff);
+                     c->ntk, c->nsplit, c->tile_start[nb], c->goff);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_permute, grid1(c->np * c->nw), dim3(NT), 0, c->stream, make_args(c, 1),

@@ -750,6 +750,13 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
              (c->mixed || std::getenv("QSS_GLOBAL_DEPOSIT") == nullptr);
   c->ntj = p->ny / kTile;
   c->ntk = p->nz / kTile;
+  // default work-groups per tile: about a thousand push work-groups per wake
+  // (a function of the grid only, so every wake of a batch sums alike);
+  // qss_autotune / qss_set_particle_split override it
+  {
+    const long nt = (long)c->ntj * c->ntk;
+    c->nsplit = nt >= 1024 ? 1 : nt >= 512 ? 2 : 4;
+  }
   int r = QSS_OK;
   auto fail = [&](int code) { qss_destroy(c); return code; };
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -832,8 +839,9 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
     const long nt = (long)c->ntj * c->ntk;
     if ((r = dalloc(&c->tile_start[0], (nt + 1) * nw)) ||
         (r = dalloc(&c->tile_start[1], (nt + 1) * nw)) ||
-        (r = dalloc(&c->win, nt * 441 * 4 * nw)) || (r = dalloc(&c->tcount, nt * 9 * nw)) ||
-        (r = dalloc(&c->goff, nt * 9 * nw)))
+        (r = dalloc(&c->win, nt * kMaxSplit * 441 * 4 * nw)) ||
+        (r = dalloc(&c->tcount, nt * kMaxSplit * 9 * nw)) ||
+        (r = dalloc(&c->goff, nt * kMaxSplit * 9 * nw)))
       return fail(r);
   }
   c->hmask = history_mask;
@@ -1400,13 +1408,20 @@ int qss_autotune(qss_ctx *c, int reps, float *table_ms) {
     for (int b = 1; b <= kGemmMaxBatch && rc == QSS_OK; ++b) {
       GemmArgs g{};
       g.M = c->nz; g.N = c->ny; g.K = pass == 0 ? c->ny : c->nz;
-      g.alpha = 1.0; g.nbatch = b; g.F = c->F; g.Ff = nullptr; g.resid = c->d_resid;
+      g.alpha = 1.0; g.nbatch = b; g.F = c->F; g.Ff = c->Ff; g.resid = c->d_resid;
       // time the launch shape the sweep uses: all wakes of the batch
       g.nw = c->nw;
       g.wA = pass == 0 ? c->ncell : 0;
       g.wB = pass == 0 ? 0 : c->ncell;
       g.wC = c->ncell;
+      g.wP = pass == 0 ? 0 : c->ncell;
+      g.wF = pass == 0 ? 0 : c->ncell * kPushComp;
+      g.wR = pass == 0 ? 0 : 6;
       g.tout = (c->npc && pass == 0) ? 1 : 0;
+      // the z passes of 6 and 8 fields carry the corrector's epilogue in the
+      // sweep (push field from the P plane, residual of E): time them with it.
+      // The push field and residual slots are scratch between layer steps.
+      const bool epi = pass == 1 && (b == 6 || b == 8);
       for (int q = 0; q < b; ++q) {
         // operands: any plane of the right size; outputs: the scratch planes
         g.b[q].A = pass == 0 ? c->S[q] : c->d_T[2];
@@ -1415,7 +1430,9 @@ int qss_autotune(qss_ctx *c, int reps, float *table_ms) {
         g.b[q].As = pass == 0 ? c->xs[q] : c->Ts[2];
         g.b[q].Bts = pass == 0 ? c->Ts[0] : c->ts[q];
         g.b[q].Cts = c->ts[q];
-        g.b[q].P = nullptr; g.b[q].resid = -1; g.b[q].push = -1;
+        g.b[q].P = epi && q < 6 ? c->P[q] : nullptr;
+        g.b[q].resid = epi && q < 3 ? q : -1;
+        g.b[q].push = epi && q < 6 ? q : -1;
       }
       float best = 1e30f;
       int bestv = 0;
@@ -1432,11 +1449,48 @@ int qss_autotune(qss_ctx *c, int reps, float *table_ms) {
       }
       c->gemm_var[pass][b] = bestv;
     }
+  // work-groups per tile of the push: one predictor pass (state unchanged)
+  // plus the window combine into scratch planes, per candidate
+  if (rc == QSS_OK && c->tiled && c->np > 0) {
+    double *const outs[4] = {c->tmp[0], c->tmp[1], c->tmp[2], c->tmp[3]};
+    float best = 1e30f;
+    int bests = c->nsplit;
+    for (int sp = 1; sp <= kMaxSplit && rc == QSS_OK; sp *= 2) {
+      c->nsplit = sp;
+      if (launch_push_tile(c, 0) != hipSuccess || launch_combine(c, outs) != hipSuccess) {
+        rc = QSS_E_HIP;
+        break;
+      }
+      (void)hipEventRecord(e0, c->stream);
+      for (int r = 0; r < reps; ++r) {
+        (void)launch_push_tile(c, 0);
+        (void)launch_combine(c, outs);
+      }
+      (void)hipEventRecord(e1, c->stream);
+      if (hipEventSynchronize(e1) != hipSuccess) { rc = QSS_E_HIP; break; }
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) { best = ms; bests = sp; }
+    }
+    c->nsplit = bests;
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != QSS_OK) qss_set_error("autotune launch failed");
   return rc;
 }
+
+int qss_set_particle_split(qss_ctx *c, int nsplit) {
+  if (!c || (nsplit != 1 && nsplit != 2 && nsplit != 4)) {
+    qss_set_error("particle split must be 1, 2 or 4");
+    return QSS_E_ARG;
+  }
+  graph_free(c);
+  c->nsplit = nsplit;
+  return QSS_OK;
+}
+
+int qss_particle_split(const qss_ctx *c) { return c ? c->nsplit : QSS_E_ARG; }
 
 int qss_enable_timing(qss_ctx *c, int on) {
   if (!c) return QSS_E_ARG;

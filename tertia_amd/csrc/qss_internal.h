@@ -19,6 +19,8 @@ enum {
 constexpr int kPushComp = 6;
 // cells per tile edge of the tiled particle path
 constexpr int kTile = 16;
+// work-groups per tile of the tiled push (1, 2 or 4; chosen per context)
+constexpr int kMaxSplit = 4;
 
 // ---------------------------------------------------------------------------
 // transform GEMM (tertia_amd/csrc/gemm.hip)
@@ -167,10 +169,11 @@ struct qss_ctx {
   // sub-cycle restart quirk is kept): particles sorted by tile
   bool tiled = false;
   int ntj = 0, ntk = 0;
+  int nsplit = 1;                // work-groups per tile
   int32_t *tile_start[2] = {};   // ping-pong with the particle arrays
-  double *win = nullptr;         // ntiles x 441 x 4 deposit windows
-  int32_t *tcount = nullptr;     // ntiles x 9 full-step direction counts
-  int32_t *goff = nullptr;       // ntiles x 9 group offsets
+  double *win = nullptr;         // ntiles x nsplit x 441 x 4 deposit windows
+  int32_t *tcount = nullptr;     // ntiles x nsplit x 9 full-step direction counts
+  int32_t *goff = nullptr;       // ntiles x 9 x nsplit group offsets
 
   unsigned long long *d_resid = nullptr;   // 6 slots per pass
   double *d_res_hist = nullptr;            // residual of each layer step
@@ -223,7 +226,8 @@ hipError_t launch_residual_final(qss_ctx *c, double *dst);
 hipError_t launch_push_field(qss_ctx *c);
 hipError_t launch_push_tile(qss_ctx *c, int full, hipEvent_t ev0 = nullptr,
                             hipEvent_t ev1 = nullptr);
-hipError_t launch_combine(qss_ctx *c);
+// out (optional): 4 destination planes instead of the C-layer Jx, Jy, Jz, Rho
+hipError_t launch_combine(qss_ctx *c, double *const *out = nullptr);
 hipError_t launch_tile_reorder(qss_ctx *c);
 hipError_t launch_seed_tiled(qss_ctx *c);
 hipError_t launch_axpy(double *y, const double *x, double a, long n, hipStream_t s);

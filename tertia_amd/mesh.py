@@ -258,16 +258,29 @@ class Mesh:
     def gemm_choice(self) -> dict:
         arr = (ctypes.c_int * 18)()
         check(self.L.qss_gemm_choice(self.h, arr), "qss_gemm_choice")
-        return {"y_pass": list(arr[1:9]), "z_pass": list(arr[10:18])}
+        return {"y_pass": list(arr[1:9]), "z_pass": list(arr[10:18]),
+                "particle_split": self.particle_split}
 
     def set_gemm_choice(self, choice: dict) -> None:
-        """Install a tile-shape choice (as returned by gemm_choice/autotune)."""
+        """Install a tuning choice (as returned by gemm_choice/autotune):
+        transform tile shapes and, when present, the push work-groups per tile."""
         arr = (ctypes.c_int * 18)()
         for i, v in enumerate(choice["y_pass"]):
             arr[1 + i] = v
         for i, v in enumerate(choice["z_pass"]):
             arr[10 + i] = v
         check(self.L.qss_set_gemm_choice(self.h, arr), "qss_set_gemm_choice")
+        if "particle_split" in choice:
+            self.particle_split = choice["particle_split"]
+
+    @property
+    def particle_split(self) -> int:
+        """Work-groups sharing one 16x16-cell tile in the tiled push."""
+        return int(self.L.qss_particle_split(self.h))
+
+    @particle_split.setter
+    def particle_split(self, n: int) -> None:
+        check(self.L.qss_set_particle_split(self.h, int(n)), "qss_set_particle_split")
 
     def set_gemm_variant(self, v: int):
         check(self.L.qss_set_gemm_variant(self.h, int(v)), "qss_set_gemm_variant")

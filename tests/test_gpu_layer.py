@@ -174,3 +174,48 @@ def test_push_error_paths(oracle_mod):
     with pytest.raises(QssError):
         m.MoveParticlesLayerSplit(0, 0, 0)
     m.close()
+
+
+@pytest.mark.parametrize("nsplit", [1, 2, 4])
+def test_push_deposit_particle_split(oracle_mod, nsplit):
+    """Tiled push with a tile's particles shared by 1, 2 or 4 work-groups:
+    8 chunks of 256 per tile, two full steps (the second one reads the
+    storage the first reorder wrote), parity with the oracle each pass, and
+    the same bits on a second run with the same split."""
+    rng = np.random.default_rng(500 + nsplit)
+    ny = nz = 48
+    p = small(ny=ny, nz=nz, nslices=3, hx=0.3, shape=1, ppc_y=2, ppc_z=4)
+    n = ny * nz * 8
+    parts = random_particles(rng, ny, nz, n, pmax=0.8, hot_fraction=0.05, hot_p=4.0)
+    fields = {f: smooth_field(rng, nz, ny, 0.8) for f in EB}
+    runs = []
+    for rep in range(2):
+        m, o = make_pair(oracle_mod, p)
+        m.particle_split = nsplit
+        assert m.particle_split == nsplit
+        for lay, which in (("C", 0), ("P", 1)):
+            for f in EB:
+                o.set_field(f, fields[f], which=which)
+                m.set_field(f, fields[f], layer=lay)
+        o.set_particles(*parts)
+        m.set_particles(*parts)
+        got = []
+        for full in (0, 0, 1, 0, 1):
+            o.push_deposit(full)
+            m.MoveParticlesLayerSplit(0, 0, full)
+            for f in ("Jx", "Jy", "Jz", "Rho"):
+                ref, g = o.get_field(f), m.get_field(f)
+                assert max_rel(g, ref) <= 1e-13, (nsplit, full, f)
+                got.append(g)
+        po, pg = o.particles(), m.particles()
+        oo, og = np.argsort(po["id"]), np.argsort(pg["id"])
+        assert np.array_equal(po["id"][oo], pg["id"][og])
+        for k in ("fy", "fz", "px", "py", "pz"):
+            d = np.abs(po[k][oo] - pg[k][og]).max()
+            assert d <= 1e-14 * max(1.0, np.abs(po[k]).max()), k
+        runs.append((got, pg))
+        m.close()
+    for a, b in zip(runs[0][0], runs[1][0]):
+        assert np.array_equal(a, b)
+    for k in ("fy", "fz", "px", "py", "pz", "id"):
+        assert np.array_equal(runs[0][1][k], runs[1][1][k]), k
