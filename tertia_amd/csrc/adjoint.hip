@@ -51,7 +51,8 @@ __global__ __launch_bounds__(NT) void k_push_tile_adj(AdjTileArgs a) {
   __shared__ short cstart[T * T], cend[T * T];
 
   const int t = threadIdx.x;
-  const int tile = blockIdx.x;
+  // a tile's particles shared by nsplit work-groups, as in the forward push
+  const int tile = blockIdx.x / a.nsplit, part = blockIdx.x - tile * a.nsplit;
   const int tj = tile % a.ntj, tk = tile / a.ntj;
   const int j0 = tj * T, k0 = tk * T;
   const int ny = a.ny, nz = a.nz;
@@ -78,7 +79,7 @@ __global__ __launch_bounds__(NT) void k_push_tile_adj(AdjTileArgs a) {
   const long np = a.np;
   const int base = a.tile_start[tile];
   const int npt = a.tile_start[tile + 1] - base;
-  for (int c0 = 0; c0 < npt; c0 += NT) {
+  for (int c0 = part * NT; c0 < npt; c0 += a.nsplit * NT) {
     const int slot = base + c0 + t;
     int key = kInvalid;
     if (c0 + t < npt) {
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(NT) void k_push_tile_adj(AdjTileArgs a) {
     }
     __syncthreads();
   }
-  double *wout = a.fwin + (long)tile * NF * 6;
+  double *wout = a.fwin + (long)blockIdx.x * NF * 6;   // window of this part
 #pragma unroll
   for (int q = 0; q < 6; ++q) wout[n0 * 6 + q] = acc0[q];
   if (n1 < NF)
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(NT) void k_push_tile_adj(AdjTileArgs a) {
 
 // out[q][node] = sum of the 17x17 windows holding the node, fixed order
 __global__ __launch_bounds__(NT) void k_fbar_combine(const double *fwin, int ntj, int ntk, int ny,
-                                                     int nz, double *out) {
+                                                     int nz, int nsplit, double *out) {
   const long n = (long)blockIdx.x * NT + threadIdx.x;
   const long plane = (long)ny * nz;
   if (n >= plane) return;
@@ -197,9 +198,11 @@ __global__ __launch_bounds__(NT) void k_fbar_combine(const double *fwin, int ntj
       if (c == 1 && nj % T != 0) continue;
       const int utj = nj / T - c, lj = nj - utj * T;
       const int wtj = wrapi(utj, ntj);
-      const double *v = fwin + ((long)(wtk * ntj + wtj) * NF + lk * FW + lj) * 6;
+      for (int p = 0; p < nsplit; ++p) {
+        const double *v = fwin + (((long)(wtk * ntj + wtj) * nsplit + p) * NF + lk * FW + lj) * 6;
 #pragma unroll
-      for (int q = 0; q < 6; ++q) s[q] += v[q];
+        for (int q = 0; q < 6; ++q) s[q] += v[q];
+      }
     }
   }
 #pragma unroll
@@ -288,13 +291,14 @@ inline dim3 grid1(long n) { return dim3((unsigned)((n + NT - 1) / NT)); }
 }  // namespace
 
 hipError_t launch_push_tile_adj(qss_ctx *c, const AdjTileArgs &a) {
-  hipLaunchKernelGGL(k_push_tile_adj, dim3(c->ntj * c->ntk), dim3(NT), 0, c->stream, a);
+  hipLaunchKernelGGL(k_push_tile_adj, dim3(c->ntj * c->ntk * a.nsplit), dim3(NT), 0, c->stream,
+                     a);
   return hipGetLastError();
 }
 
 hipError_t launch_fbar_combine(qss_ctx *c, const double *fwin, double *out) {
   hipLaunchKernelGGL(k_fbar_combine, grid1(c->ncell), dim3(NT), 0, c->stream, fwin, c->ntj,
-                     c->ntk, c->ny, c->nz, out);
+                     c->ntk, c->ny, c->nz, c->nsplit, out);
   return hipGetLastError();
 }
 

@@ -347,7 +347,10 @@ int do_iterate(qss_ctx *c, int final_pass, double *res_dev) {
   const double *pp[8] = {c->P[QSS_EX], c->P[QSS_EY], c->P[QSS_EZ], c->P[QSS_BX],
                          c->P[QSS_BY], c->P[QSS_BZ], nullptr, nullptr};
   const double alpha = 1.0 / (4.0 * c->ny * c->nz);
-  CHECK(transform_batch(c, final_pass ? 8 : 6, kz, ky, in, out, alpha, resid, push, pp));
+  // rho and psi are outputs only (the next layer starts from the kept
+  // spectra); a sweep that stores neither leaves them to its end
+  const int nb = final_pass && !c->skip_rho_psi ? 8 : 6;
+  CHECK(transform_batch(c, nb, kz, ky, in, out, alpha, resid, push, pp));
   if (res_dev) {
     Timed t(c, KC_SPEC);
     HIPCHECK(launch_residual_final(c, res_dev));
@@ -489,7 +492,7 @@ int adj_alloc(qss_ctx *c, int nrun, int K) {
   if ((r = dalloc(&a.pbarEB, nc * kPushComp)) || (r = dalloc(&a.fbar, nc * kPushComp)) ||
       (r = dalloc(&a.jbar, nc * 3)) || (r = dalloc(&a.bbar, nc * 2)) ||
       (r = dalloc(&a.pbar[0], np * 5)) || (r = dalloc(&a.pbar[1], np * 5)) ||
-      (r = dalloc(&a.fwin, nt * 289 * 6)) || (r = dalloc(&a.qbeam, (long)c->p.nslices * 2 * kBeamAdjBlocks)) ||
+      (r = dalloc(&a.fwin, nt * kMaxSplit * 289 * 6)) || (r = dalloc(&a.qbeam, (long)c->p.nslices * 2 * kBeamAdjBlocks)) ||
       (r = dalloc(&a.chk, a.chk_stride * nchk)) || (r = dalloc(&a.tape, a.tape_stride * K)) ||
       (r = dalloc(&a.exbar, (size_t)nrun * nc)))
     return r;
@@ -518,6 +521,7 @@ int adj_push_pass(qss_ctx *c, char *slot, int pass, int full, int cur_adj) {
   g.pout = s.pbar[cur_adj ^ 1];
   g.np = c->np_cap;
   g.fwin = s.fwin;
+  g.nsplit = c->nsplit;
   g.counters = c->d_counters;
   {
     Timed t(c, KC_PUSH);
@@ -944,6 +948,14 @@ int qss_sweep_range(qss_ctx *c, int first, int count, double *residuals) {
   if (!c->t0) { HIPCHECK(hipEventCreate(&c->t0)); HIPCHECK(hipEventCreate(&c->t1)); }
   HIPCHECK(hipEventRecord(c->t0, c->stream));
   const int S = c->p.nx_split;
+  // without rho or psi in the history the layers' rho and psi planes are
+  // not needed inside the sweep: only the last layer's are produced, below
+  const uint32_t rho_psi = (1u << QSS_RHO) | (1u << QSS_PSI);
+  c->skip_rho_psi = (c->hmask & rho_psi) == 0;
+  struct Reset {
+    qss_ctx *c;
+    ~Reset() { c->skip_rho_psi = false; }   // also on an early error return
+  } reset_skip{c};
   for (int s = first; s < first + count;) {
     if (c->use_graph && !c->timing && S == 1 && s >= 1 && first + count - s >= 2) {
       int done = 0;
@@ -955,6 +967,15 @@ int qss_sweep_range(qss_ctx *c, int first, int count, double *residuals) {
       CHECK(do_layer_step(c, s, is, c->d_res_hist + (size_t)s * S + is));
     CHECK(store_history(c, s));
     ++s;
+  }
+  if (c->skip_rho_psi && count > 0) {
+    // the last layer (P after the swap) from its final spectra
+    const int kz[2] = {QSS_KIND_C, QSS_KIND_C}, ky[2] = {QSS_KIND_C, QSS_KIND_C};
+    const double *in[2] = {c->S[SP_RHO], c->S[SP_PSI]};
+    double *out[2] = {c->P[QSS_RHO], c->P[QSS_PSI]};
+    c->skip_rho_psi = false;
+    CHECK(transform_batch(c, 2, kz, ky, in, out, 1.0 / (4.0 * c->ny * c->nz), nullptr,
+                          nullptr, nullptr));
   }
   HIPCHECK(hipEventRecord(c->t1, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));

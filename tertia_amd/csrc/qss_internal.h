@@ -87,7 +87,8 @@ struct AdjTileArgs {
   const double *pin;    // full step: adjoint of the new state by id (5 x np)
   double *pout;         // adjoint of the start state by id (5 x np); full: =, else +=
   long np;
-  double *fwin;         // per tile 17x17 nodes x 6 field adjoints
+  double *fwin;         // per tile and part: 17x17 nodes x 6 field adjoints
+  int nsplit;           // work-groups per tile
   int32_t *counters;
 };
 
@@ -100,7 +101,7 @@ struct AdjState {
   double *jbar = nullptr;      // 3 planes: adjoint of the deposited Jx,Jy,Jz
   double *bbar = nullptr;      // 2 planes: adjoint of the beam rho, Jx planes
   double *pbar[2] = {};        // particle-state adjoints by id (5 x np_cap)
-  double *fwin = nullptr;      // ntiles x 289 x 6
+  double *fwin = nullptr;      // ntiles x nsplit x 289 x 6
   double *qbeam = nullptr;     // per layer and x-cell side: d(loss)/d(profile)
   char *chk = nullptr;         // checkpoints (state at layer starts)
   size_t chk_stride = 0;
@@ -170,6 +171,7 @@ struct qss_ctx {
   bool tiled = false;
   int ntj = 0, ntk = 0;
   int nsplit = 1;                // work-groups per tile
+  bool skip_rho_psi = false;     // inside a sweep: final passes leave rho, psi
   int32_t *tile_start[2] = {};   // ping-pong with the particle arrays
   double *win = nullptr;         // ntiles x nsplit x 441 x 4 deposit windows
   int32_t *tcount = nullptr;     // ntiles x nsplit x 9 full-step direction counts

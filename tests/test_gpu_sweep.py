@@ -100,3 +100,32 @@ def test_deterministic(oracle_mod):
     for k in ("fy", "fz", "px", "py", "pz", "id"):
         assert np.array_equal(pa[k], pb[k])
     a.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_sweep_without_rho_psi_history(graph):
+    """A sweep storing neither rho nor psi skips their per-layer planes and
+    produces only the last layer's: the same Ex history bit for bit, and the
+    last layer's rho and psi planes as in a sweep that stores them."""
+    from tertia_amd.mesh import Mesh
+    p = small(ny=32, nz=32, nslices=17,
+              bunches=[Bunch(nb=3.0, sigma_r=0.5, sigma_xi=0.3, xi0=0.5)])
+    out = {}
+    for hist in (("Ex", "Rho", "Psi"), ("Ex",)):
+        m = Mesh(p, history=hist)
+        try:
+            m.enable_graph(graph)
+            r = m.MoveAllSplitLayers()
+            out[hist] = (r, np.stack([m.history("Ex", s) for s in range(p.nslices)]),
+                         m.get_field("Rho", layer="P"), m.get_field("Psi", layer="P"),
+                         m.history("Rho", p.nslices - 1) if "Rho" in hist else None,
+                         m.history("Psi", p.nslices - 1) if "Psi" in hist else None)
+        finally:
+            m.close()
+    full, lean = out[("Ex", "Rho", "Psi")], out[("Ex",)]
+    assert np.array_equal(full[0], lean[0])
+    assert np.array_equal(full[1], lean[1])
+    # the stored last layer is the P layer after the sweep
+    assert np.array_equal(full[2], full[4]) and np.array_equal(full[3], full[5])
+    for a, b in ((lean[2], full[2]), (lean[3], full[3])):
+        assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max()
