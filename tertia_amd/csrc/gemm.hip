@@ -183,7 +183,13 @@ __device__ inline void store_tiles(const GemmArgs &a, const GemmBatch &B, int w,
 struct MathF64 {
   typedef double lds_t;
   typedef d4 acc_t;
-  static constexpr int PAD = 2;
+  // unpadded k-major LDS tiles with an XOR swizzle of the m (n) index by the
+  // k row: the 16-lane groups of an MFMA operand read (row k, k+1 in the
+  // other 32 banks) and the 8-lane groups that store one row of A's global
+  // slab (8 values of k) hit distinct banks.  Needs BM, BN >= 32.
+  static constexpr int PAD = 0;
+  __device__ static int xa(int k) { return ((k & 1) << 4) ^ (((k >> 1) & 7) << 2); }
+  __device__ static int xb(int k) { return (k & 1) << 4; }
   __device__ static acc_t mma(double x, double y, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
   }
@@ -194,6 +200,8 @@ struct MathF32 {
   typedef float lds_t;
   typedef f4 acc_t;
   static constexpr int PAD = 16;   // row pitch = 16 mod 64 banks
+  __device__ static int xa(int) { return 0; }
+  __device__ static int xb(int) { return 0; }
   __device__ static acc_t mma(float x, float y, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0);
   }
@@ -225,6 +233,7 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_gemm(GemmArgs a) {
   using C = Cfg<BM, BN, BK, WGM, WGN, KS>;
   typedef typename MT::lds_t T;
   typedef typename MT::acc_t ACC;
+  static_assert(BM >= 32 && BN >= 32, "the LDS swizzle permutes m, n within 32");
   const int M = a.M, N = a.N, K = a.K;
   const Item it = block_item<BM, BN>(a);
   const GemmBatch &B = a.b[it.f];
@@ -280,15 +289,15 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_gemm(GemmArgs a) {
     for (int q = 0; q < C::A_PER_T / 2; ++q) {
       const int e = (q * C::NT + t) * 2;
       const int r = e / BK, cc = e - r * BK;
-      As[buf][cc][r] = (T)ra_[2 * q];
-      As[buf][cc + 1][r] = (T)ra_[2 * q + 1];
+      As[buf][cc][r ^ MT::xa(cc)] = (T)ra_[2 * q];
+      As[buf][cc + 1][r ^ MT::xa(cc + 1)] = (T)ra_[2 * q + 1];
     }
 #pragma unroll
     for (int q = 0; q < C::B_PER_T / 2; ++q) {
       const int e = (q * C::NT + t) * 2;
       const int r = e / BN, cc = e - r * BN;
-      Bs[buf][r][cc] = (T)rb_[2 * q];
-      Bs[buf][r][cc + 1] = (T)rb_[2 * q + 1];
+      Bs[buf][r][cc ^ MT::xb(r)] = (T)rb_[2 * q];
+      Bs[buf][r][(cc + 1) ^ MT::xb(r)] = (T)rb_[2 * q + 1];
     }
   };
 
@@ -302,11 +311,12 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_gemm(GemmArgs a) {
 #pragma unroll
     for (int kq = 0; kq < C::KK; ++kq) {
       const int kr = (ks * C::KK + kq) * 4 + lk;
+      const int sa = MT::xa(kr), sb = MT::xb(kr);
       T av[C::TI], bv[C::TJ];
 #pragma unroll
-      for (int i = 0; i < C::TI; ++i) av[i] = As[buf][kr][wm * C::WTM + i * 16 + li];
+      for (int i = 0; i < C::TI; ++i) av[i] = As[buf][kr][(wm * C::WTM + i * 16 + li) ^ sa];
 #pragma unroll
-      for (int j = 0; j < C::TJ; ++j) bv[j] = Bs[buf][kr][wn * C::WTN + j * 16 + li];
+      for (int j = 0; j < C::TJ; ++j) bv[j] = Bs[buf][kr][(wn * C::WTN + j * 16 + li) ^ sb];
 #pragma unroll
       for (int i = 0; i < C::TI; ++i)
 #pragma unroll
