@@ -126,23 +126,54 @@ __device__ inline void k_loop(int K, L0 load0, L1 load1, S0 store0, S1 store1, C
 
 // Store of one wave's TI x TJ MFMA tiles.  val(i, j, r) is the fp64 result of
 // tile (i, j), register r; ROW(lk, r) the tile row that register r of a lane
-// in row group lk = lane/16 holds (the column is lane % 16).
-template <bool GUARD, int TI, int TJ, class ROW, class V>
+// in row group lk = lane/16 holds (the column is lane % 16).  The epilogue's
+// loads (the old C for the residual or the beta term, the P plane for the
+// push field) of one MFMA tile are all issued before any of them is used,
+// so their latencies overlap instead of adding up.
+// EPI = false: a launch with no epilogue work (no beta, residual or push
+// field) compiles to plain scaled stores, keeping its register count low.
+template <bool GUARD, int TI, int TJ, class ROW, class V, bool EPI = true>
 __device__ inline void store_tiles(const GemmArgs &a, const GemmBatch &B, int w, int r0, int c0,
                                    int lane, V val) {
   const int M = a.M, N = a.N;
   const int li = lane & 15, lk = lane >> 4;
+  if constexpr (!EPI) {
+    double *const Cw = B.C + w * a.wC;
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = r0 + i * 16 + ROW::row(lk, r);
+          const int col = c0 + j * 16 + li;
+          if (!GUARD || (row < M && col < N)) Cw[(long)row * N + col] = a.alpha * val(i, j, r);
+        }
+    return;
+  }
   double dmax = 0.0, vmax = 0.0;
-  const double alpha = a.alpha;
+  const double alpha = a.alpha, beta = a.beta;
   double *const Cw = B.C + w * a.wC;
   const double *const Pw = B.P ? B.P + w * a.wP : nullptr;
   double *const Fw = a.F ? a.F + w * a.wF : nullptr;
   float *const Ffw = a.Ff ? a.Ff + w * a.wF : nullptr;
   unsigned long long *const Rw = a.resid + w * a.wR;
+  const bool need_old = B.resid >= 0 || beta != 0.0;
+  const bool need_p = B.push >= 0;
 #pragma unroll
   for (int i = 0; i < TI; ++i)
 #pragma unroll
-    for (int j = 0; j < TJ; ++j)
+    for (int j = 0; j < TJ; ++j) {
+      double oldv[4], pv[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = r0 + i * 16 + ROW::row(lk, r);
+        const int col = c0 + j * 16 + li;
+        const bool ok = !GUARD || (row < M && col < N);
+        const long n = (long)row * N + col;
+        oldv[r] = (need_old && ok) ? Cw[n] : 0.0;
+        pv[r] = (need_p && ok) ? Pw[n] : 0.0;
+      }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = r0 + i * 16 + ROW::row(lk, r);
@@ -150,23 +181,23 @@ __device__ inline void store_tiles(const GemmArgs &a, const GemmBatch &B, int w,
         if (!GUARD || (row < M && col < N)) {
           const long n = (long)row * N + col;
           double v = alpha * val(i, j, r);
-          if (a.beta != 0.0) v += a.beta * Cw[n];   // accumulate (adjoint sweep)
+          if (beta != 0.0) v += beta * oldv[r];   // accumulate (adjoint sweep)
           if (B.resid >= 0) {
             // NaN -> +inf so the non-finite guard sees it (fmax drops NaN)
-            const double old = Cw[n];
-            const double d = fabs(old - v), m = fabs(v);
+            const double d = fabs(oldv[r] - v), m = fabs(v);
             dmax = fmax(dmax, d == d ? d : INFINITY);
             vmax = fmax(vmax, m == m ? m : INFINITY);
           }
           Cw[n] = v;
-          if (B.push >= 0) {
-            const double pv = 0.5 * (Pw[n] + v);
+          if (need_p) {
+            const double q = 0.5 * (pv[r] + v);
             const long pn = (long)B.push * M * N + n;
-            if (Ffw) Ffw[pn] = (float)pv;
-            else Fw[pn] = pv;
+            if (Ffw) Ffw[pn] = (float)q;
+            else Fw[pn] = q;
           }
         }
       }
+  }
   if (B.resid >= 0) {
     dmax = wave_max(dmax);
     vmax = wave_max(vmax);
@@ -228,7 +259,7 @@ struct Cfg {
   static constexpr int KK = BK / 4 / KS;   // MFMA k-steps per wave per slab
 };
 
-template <class MT, int BM, int BN, int BK, int WGM, int WGN, int KS, bool GUARD>
+template <class MT, int BM, int BN, int BK, int WGM, int WGN, int KS, bool GUARD, bool EPI>
 __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_gemm(GemmArgs a) {
   using C = Cfg<BM, BN, BK, WGM, WGN, KS>;
   typedef typename MT::lds_t T;
@@ -348,8 +379,9 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_gemm(GemmArgs a) {
         for (int j = 0; j < C::TJ; ++j) acc[i][j] += red[q][wl][i * C::TJ + j][lane];
   }
 
-  store_tiles<GUARD, C::TI, C::TJ, MT>(a, B, it.w, m0 + wm * C::WTM, n0 + wn * C::WTN, lane,
-                                       [&](int i, int j, int r) { return (double)acc[i][j][r]; });
+  auto val = [&](int i, int j, int r) { return (double)acc[i][j][r]; };
+  store_tiles<GUARD, C::TI, C::TJ, MT, decltype(val), EPI>(a, B, it.w, m0 + wm * C::WTM,
+                                                           n0 + wn * C::WTN, lane, val);
 }
 
 
@@ -564,14 +596,16 @@ __global__ __launch_bounds__(256) void k_split(SplitArgs a) {
 typedef void (*GemmFn)(GemmArgs);
 struct Variant {
   int bm, bn, bk, nt;
-  GemmFn fn[2][2];   // [guarded][tout]
+  GemmFn fn[2][2];   // [guarded][tout]; dense kernels: [guarded][epilogue]
   const char *name;
 };
 
 #define QSS_GEMM_DENSE(MT, BM, BN, BK, WGM, WGN, KS)                                   \
   Variant{BM, BN, BK, Cfg<BM, BN, BK, WGM, WGN, KS>::NT,                               \
-          {{k_gemm<MT, BM, BN, BK, WGM, WGN, KS, false>, nullptr},                     \
-           {k_gemm<MT, BM, BN, BK, WGM, WGN, KS, true>, nullptr}},                     \
+          {{k_gemm<MT, BM, BN, BK, WGM, WGN, KS, false, false>,                        \
+            k_gemm<MT, BM, BN, BK, WGM, WGN, KS, false, true>},                        \
+           {k_gemm<MT, BM, BN, BK, WGM, WGN, KS, true, false>,                         \
+            k_gemm<MT, BM, BN, BK, WGM, WGN, KS, true, true>}},                        \
           #BM "x" #BN "x" #BK "_w" #WGM "x" #WGN "_k" #KS}
 
 #define QSS_GEMM_BF16(NPC, BM, BN, BK, WGM, WGN)                                       \
@@ -601,6 +635,11 @@ const Variant kF64[] = {
     QSS_GEMM_DENSE(MathF64, 32, 64, 32, 1, 2, 2),    // 14
     QSS_GEMM_DENSE(MathF64, 64, 32, 16, 2, 1, 2),    // 15
     QSS_GEMM_DENSE(MathF64, 64, 64, 16, 1, 1, 2),    // 16: 2 waves share one 64x64 tile
+    QSS_GEMM_DENSE(MathF64, 32, 32, 32, 1, 1, 2),    // 17
+    QSS_GEMM_DENSE(MathF64, 64, 32, 32, 2, 1, 4),    // 18: 8 waves
+    QSS_GEMM_DENSE(MathF64, 32, 64, 32, 1, 2, 4),    // 19: 8 waves
+    QSS_GEMM_DENSE(MathF64, 64, 64, 32, 2, 2, 2),    // 20: 8 waves, deep slab
+    QSS_GEMM_DENSE(MathF64, 32, 32, 64, 1, 1, 4),    // 21: deep slab, 4 waves
 };
 
 const Variant kF32[] = {
@@ -673,10 +712,19 @@ hipError_t launch_gemm(const GemmArgs &a, hipStream_t s, int prec, int variant, 
   const bool exact = a.M % v.bm == 0 && a.N % v.bn == 0 && a.K % (2 * v.bk) == 0 &&
                      (!tout || a.M % 4 == 0);
   const dim3 grid(tiles * a.nbatch * nw), block(v.nt);
+  // dense kernels: second index = epilogue work present (beta, residual,
+  // push field); split-bf16 kernels: second index = transposed split store
+  int second = tout;
+  if (!gemm_pieces(prec)) {
+    bool epi = a.beta != 0.0;
+    for (int b = 0; b < a.nbatch; ++b) epi = epi || a.b[b].resid >= 0 || a.b[b].push >= 0;
+    second = epi ? 1 : 0;
+  }
+  const GemmFn fn = v.fn[exact ? 0 : 1][second];
   if (ev0 || ev1)
-    hipExtLaunchKernelGGL(v.fn[exact ? 0 : 1][tout], grid, block, 0, s, ev0, ev1, 0, a);
+    hipExtLaunchKernelGGL(fn, grid, block, 0, s, ev0, ev1, 0, a);
   else
-    hipLaunchKernelGGL(v.fn[exact ? 0 : 1][tout], grid, block, 0, s, a);
+    hipLaunchKernelGGL(fn, grid, block, 0, s, a);
   return hipGetLastError();
 }
 
