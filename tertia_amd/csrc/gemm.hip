@@ -132,9 +132,12 @@ __device__ inline void k_loop(int K, L0 load0, L1 load1, S0 store0, S1 store1, C
 // so their latencies overlap instead of adding up.
 // EPI = false: a launch with no epilogue work (no beta, residual or push
 // field) compiles to plain scaled stores, keeping its register count low.
-template <bool GUARD, int TI, int TJ, class ROW, class V, bool EPI = true>
+// PRE: the caller loaded those operands already (pre_old(i, j, r),
+// pre_p(i, j, r)), e.g. before its K loop so the loads overlap the MFMAs.
+template <bool GUARD, int TI, int TJ, class ROW, class V, bool EPI = true, bool PRE = false,
+          class PO = int, class PP = int>
 __device__ inline void store_tiles(const GemmArgs &a, const GemmBatch &B, int w, int r0, int c0,
-                                   int lane, V val) {
+                                   int lane, V val, PO pre_old = 0, PP pre_p = 0) {
   const int M = a.M, N = a.N;
   const int li = lane & 15, lk = lane >> 4;
   if constexpr (!EPI) {
@@ -157,7 +160,7 @@ __device__ inline void store_tiles(const GemmArgs &a, const GemmBatch &B, int w,
   const double *const Pw = B.P ? B.P + w * a.wP : nullptr;
   double *const Fw = a.F ? a.F + w * a.wF : nullptr;
   float *const Ffw = a.Ff ? a.Ff + w * a.wF : nullptr;
-  unsigned long long *const Rw = a.resid + w * a.wR;
+  unsigned long long *const Rw = a.resid + w * a.wR + (blockIdx.x % kResidSlots) * 8;
   const bool need_old = B.resid >= 0 || beta != 0.0;
   const bool need_p = B.push >= 0;
 #pragma unroll
@@ -167,12 +170,17 @@ __device__ inline void store_tiles(const GemmArgs &a, const GemmBatch &B, int w,
       double oldv[4], pv[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = r0 + i * 16 + ROW::row(lk, r);
-        const int col = c0 + j * 16 + li;
-        const bool ok = !GUARD || (row < M && col < N);
-        const long n = (long)row * N + col;
-        oldv[r] = (need_old && ok) ? Cw[n] : 0.0;
-        pv[r] = (need_p && ok) ? Pw[n] : 0.0;
+        if constexpr (PRE) {
+          oldv[r] = pre_old(i, j, r);
+          pv[r] = pre_p(i, j, r);
+        } else {
+          const int row = r0 + i * 16 + ROW::row(lk, r);
+          const int col = c0 + j * 16 + li;
+          const bool ok = !GUARD || (row < M && col < N);
+          const long n = (long)row * N + col;
+          oldv[r] = (need_old && ok) ? Cw[n] : 0.0;
+          pv[r] = (need_p && ok) ? Pw[n] : 0.0;
+        }
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -355,6 +363,28 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_gemm(GemmArgs a) {
     }
   };
 
+  // epilogue operands of this wave tile (old C, P plane), loaded now so that
+  // their latency hides behind the K loop; only the waves that store need them
+  double po[EPI ? C::TI : 1][EPI ? C::TJ : 1][4], pp[EPI ? C::TI : 1][EPI ? C::TJ : 1][4];
+  if constexpr (EPI) {
+    const int r0 = m0 + wm * C::WTM, c0 = n0 + wn * C::WTN;
+    const bool need_old = B.resid >= 0 || a.beta != 0.0, need_p = B.push >= 0;
+    const double *Cw = B.C + it.w * a.wC;
+    const double *Pw = B.P ? B.P + it.w * a.wP : nullptr;
+#pragma unroll
+    for (int i = 0; i < C::TI; ++i)
+#pragma unroll
+      for (int j = 0; j < C::TJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = r0 + i * 16 + MT::row(lk, r), col = c0 + j * 16 + li;
+          const bool ok = ks == 0 && (!GUARD || (row < M && col < N));
+          const long n = (long)row * N + col;
+          po[i][j][r] = (need_old && ok) ? Cw[n] : 0.0;
+          pp[i][j][r] = (need_p && ok) ? Pw[n] : 0.0;
+        }
+  }
+
   k_loop<GUARD, BK>(
       K, [&](int k0) { load(k0, ra[0], rb[0]); }, [&](int k0) { load(k0, ra[1], rb[1]); },
       [&](int buf) { store(buf, ra[0], rb[0]); }, [&](int buf) { store(buf, ra[1], rb[1]); },
@@ -380,8 +410,11 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_gemm(GemmArgs a) {
   }
 
   auto val = [&](int i, int j, int r) { return (double)acc[i][j][r]; };
-  store_tiles<GUARD, C::TI, C::TJ, MT, decltype(val), EPI>(a, B, it.w, m0 + wm * C::WTM,
-                                                           n0 + wn * C::WTN, lane, val);
+  auto pre_old = [&](int i, int j, int r) { return po[EPI ? i : 0][EPI ? j : 0][r]; };
+  auto pre_p = [&](int i, int j, int r) { return pp[EPI ? i : 0][EPI ? j : 0][r]; };
+  store_tiles<GUARD, C::TI, C::TJ, MT, decltype(val), EPI, true, decltype(pre_old),
+              decltype(pre_p)>(a, B, it.w, m0 + wm * C::WTM, n0 + wn * C::WTN, lane, val,
+                               pre_old, pre_p);
 }
 
 

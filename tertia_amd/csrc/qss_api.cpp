@@ -211,7 +211,7 @@ int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
   g.alpha = alpha;
   g.beta = beta;
   g.tout = 0;
-  g.wA = 0; g.wB = nc; g.wC = nc; g.wP = nc; g.wF = nc * kPushComp; g.wR = 6;
+  g.wA = 0; g.wB = nc; g.wC = nc; g.wP = nc; g.wF = nc * kPushComp; g.wR = kResidPerWake;
   for (int b = 0; b < n; ++b) {
     g.b[b].A = c->d_T[kz[b] == QSS_KIND_C ? 2 : 3];
     g.b[b].B = c->tmp[b];
@@ -835,7 +835,7 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
       (r = dalloc(&c->cnt, nc)) || (r = dalloc(&c->cnt_main, nc)) ||
       (r = dalloc(&c->off, nc)) || (r = dalloc(&c->off_main, nc)) ||
       (r = dalloc(&c->blk, c->nblk)) || (r = dalloc(&c->blk_main, c->nblk)) ||
-      (r = dalloc(&c->list, np + ov)) || (r = dalloc(&c->d_resid, 6 * nw)) ||
+      (r = dalloc(&c->list, np + ov)) || (r = dalloc(&c->d_resid, (long)kResidPerWake * nw)) ||
       (r = dalloc(&c->d_slice, 1)) ||
       (r = dalloc(&c->d_res_hist, (size_t)p->nslices * p->nx_split * nw)))
     return fail(r);
@@ -1437,7 +1437,7 @@ int qss_autotune(qss_ctx *c, int reps, float *table_ms) {
       g.wC = c->ncell;
       g.wP = pass == 0 ? 0 : c->ncell;
       g.wF = pass == 0 ? 0 : c->ncell * kPushComp;
-      g.wR = pass == 0 ? 0 : 6;
+      g.wR = pass == 0 ? 0 : kResidPerWake;
       g.tout = (c->npc && pass == 0) ? 1 : 0;
       // the z passes of 6 and 8 fields carry the corrector's epilogue in the
       // sweep (push field from the P plane, residual of E): time them with it.

@@ -31,6 +31,12 @@ constexpr int kMaxSplit = 4;
 // transposed and split (Ct[N][M]), ready to be the next pass's Bt.
 // ---------------------------------------------------------------------------
 constexpr int kGemmMaxBatch = 8;
+// residual maxima of the corrector: each wave adds its maxima (atomicMax on
+// the bit patterns) into slot blockIdx % kResidSlots, 8 u64 per slot (dif[3],
+// max[3]); spreading the atomics over slots keeps thousands of waves from
+// serialising on one address.  The finaliser takes the max over the slots.
+constexpr int kResidSlots = 128;
+constexpr int kResidPerWake = kResidSlots * 8;
 constexpr int kMaxPieces = 3;
 struct GemmBatch {
   const double *A;
@@ -51,7 +57,7 @@ struct GemmArgs {
   int tout;                     // bf16 modes: store Cts instead of C
   double *F;                    // push field planes (kPushComp x M*N)
   float *Ff;                    // its fp32 form (mixed modes; F unused when set)
-  unsigned long long *resid;    // 6 slots: dif[3], max[3] as IEEE bit patterns
+  unsigned long long *resid;    // kResidPerWake per wake: slots of dif[3], max[3] (bits)
   GemmBatch b[kGemmMaxBatch];
   // batched wakes (dense kernels): the launch covers nw copies of every
   // field; copy w uses operand + w * stride (stride 0 = shared operand)
@@ -177,7 +183,7 @@ struct qss_ctx {
   int32_t *tcount = nullptr;     // ntiles x nsplit x 9 full-step direction counts
   int32_t *goff = nullptr;       // ntiles x 9 x nsplit group offsets
 
-  unsigned long long *d_resid = nullptr;   // 6 slots per pass
+  unsigned long long *d_resid = nullptr;   // kResidPerWake per wake and pass
   double *d_res_hist = nullptr;            // residual of each layer step
   int res_count = 0;
 

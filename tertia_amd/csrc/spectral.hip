@@ -37,7 +37,7 @@ struct SpecArgs {
   int ny, nz, nw;
   double ly, lz, hx, dens;
   int flag;  // iterate: final pass
-  // iterate: residual slots (6 per wake) cleared for the inverse-GEMM
+  // iterate: residual slots (kResidPerWake per wake) cleared for the inverse-GEMM
   // epilogues that follow on the stream (null: untouched)
   unsigned long long *resid;
 };
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(NT) void k_iterate(SpecArgs a) {
   using namespace qss_spec;
   const long n1 = (long)blockIdx.x * NT + threadIdx.x;   // wake-major mode index
   const long nc = (long)a.ny * a.nz;
-  if (a.resid && n1 < 6L * a.nw) a.resid[n1] = 0ull;
+  if (a.resid && n1 < (long)kResidPerWake * a.nw) a.resid[n1] = 0ull;
   if (n1 >= nc * a.nw) return;
   const long mi = n1 % nc;
   const Mode m = mode_of(a, (int)(mi % a.ny), (int)(mi / a.ny));
@@ -221,19 +221,30 @@ __global__ __launch_bounds__(NT) void k_iterate_adj(SpecArgs a) {
 }
 
 // Eerr = max_c dif_c / max(1e-5, max_c)  (:1541, :1584-1621, :1641-1643)
-// one thread per wake: slots r[6w..6w+5] -> dst[w * stride]; during a graph
-// replay the layer slot comes from device memory: dst += *slice_dev * per_slice
+// wake w's residual slots -> dst[w * stride]; during a graph replay the
+// layer slot comes from device memory: dst += *slice_dev * per_slice
 __global__ void k_resid_final(const unsigned long long *r, double *dst, int nw, long stride,
                               const int *slice_dev, int per_slice) {
-  const int w = threadIdx.x;
+  // one wave per wake (blockIdx.x): max over the slots, lanes then shuffles;
+  // the bit patterns of non-negative doubles (and of +inf) order like the values
+  const int w = blockIdx.x, lane = threadIdx.x;
   if (w >= nw) return;
-  r += 6 * w;
+  r += (long)kResidPerWake * w;
   dst += w * stride;
   if (slice_dev) dst += (long)(*slice_dev) * per_slice;
+  unsigned long long m[6] = {0, 0, 0, 0, 0, 0};
+  for (int s = lane; s < kResidSlots; s += 64)
+    for (int q = 0; q < 6; ++q) m[q] = m[q] > r[s * 8 + q] ? m[q] : r[s * 8 + q];
+  for (int o = 32; o > 0; o >>= 1)
+    for (int q = 0; q < 6; ++q) {
+      const unsigned long long v = __shfl_xor(m[q], o, 64);
+      m[q] = m[q] > v ? m[q] : v;
+    }
+  if (lane != 0) return;
   double err = 0.0;
   for (int c = 0; c < 3; ++c) {
-    double dif = __longlong_as_double((long long)r[c]);
-    double mx = __longlong_as_double((long long)r[3 + c]);
+    double dif = __longlong_as_double((long long)m[c]);
+    double mx = __longlong_as_double((long long)m[3 + c]);
     if (!(mx > 1e-5)) mx = (mx != mx) ? mx : 1e-5;
     double e = dif / mx;
     err = (e > err || e != e) ? e : err;
@@ -346,14 +357,15 @@ hipError_t launch_guess_update(qss_ctx *c, int first_layer) {
 hipError_t launch_iterate_update(qss_ctx *c, int final_pass) {
   SpecArgs a = make_args(c, final_pass);
   a.resid = c->d_resid;
-  const long n = c->ncell > 6 ? c->ncell : 6;   // threads for every mode and slot
+  // threads for every mode and every residual slot
+  const long n = c->ncell > kResidPerWake ? c->ncell : kResidPerWake;
   hipLaunchKernelGGL(k_iterate, grid1(n * c->nw), dim3(NT), 0, c->stream, a);
   return hipGetLastError();
 }
 
 hipError_t launch_residual_final(qss_ctx *c, double *dst) {
   // capturing: dst is the slot of slice 0 and the slice is read at replay
-  hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(((c->nw + 63) / 64) * 64), 0, c->stream,
+  hipLaunchKernelGGL(k_resid_final, dim3(c->nw), dim3(64), 0, c->stream,
                      c->d_resid, dst, c->nw, (long)c->p.nslices * c->p.nx_split,
                      c->capturing ? c->d_slice : nullptr, c->p.nx_split);
   return hipGetLastError();
