@@ -426,33 +426,40 @@ __global__ __launch_bounds__(NT) void k_combine(const double *win, int ntj, int 
 
 // new tile starts and group offsets: destination tile u receives the groups
 // of source tiles u - dir(d) for d = 0..8 in that order, each as the groups
-// of the source's parts 0..nsplit-1
-__global__ __launch_bounds__(NT) void k_tile_scan(const int32_t *tcount, int ntj, int ntk,
-                                                  int nsplit, int32_t *tile_start,
-                                                  int32_t *goff) {
-  __shared__ int sh[NT];
+// of the source's parts 0..nsplit-1.  One 1024-thread work-group per wake:
+// a thread holds one destination tile's 9 x nsplit counts in registers.
+constexpr int NTS = 1024;
+__global__ __launch_bounds__(NTS) void k_tile_scan(const int32_t *tcount, int ntj, int ntk,
+                                                   int nsplit, int32_t *tile_start,
+                                                   int32_t *goff) {
+  __shared__ int sh[NTS];
   const int nt = ntj * ntk;
   const int w = blockIdx.x;   // one work-group per wake
   tcount += (long)w * nt * nsplit * 9;
   goff += (long)w * nt * nsplit * 9;
   tile_start += (long)w * (nt + 1);
   int carry = 0;
-  for (int b0 = 0; b0 < nt; b0 += NT) {
+  for (int b0 = 0; b0 < nt; b0 += NTS) {
     const int u = b0 + threadIdx.x;
     int tot = 0;
-    int src[9];
+    int g[9][kMaxSplit];
     if (u < nt) {
       const int uj = u % ntj, uk = u / ntj;
+#pragma unroll
       for (int d = 0; d < 9; ++d) {
         const int dj = d % 3 - 1, dk = d / 3 - 1;
-        src[d] = wrapi(uk - dk, ntk) * ntj + wrapi(uj - dj, ntj);
-        for (int q = 0; q < nsplit; ++q) tot += tcount[(src[d] * nsplit + q) * 9 + d];
+        const int src = wrapi(uk - dk, ntk) * ntj + wrapi(uj - dj, ntj);
+#pragma unroll
+        for (int q = 0; q < kMaxSplit; ++q) {
+          g[d][q] = q < nsplit ? tcount[(src * nsplit + q) * 9 + d] : 0;
+          tot += g[d][q];
+        }
       }
     }
     // block exclusive scan of tot
     sh[threadIdx.x] = tot;
     __syncthreads();
-    for (int o = 1; o < NT; o <<= 1) {
+    for (int o = 1; o < NTS; o <<= 1) {
       const int v = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
       __syncthreads();
       sh[threadIdx.x] += v;
@@ -463,13 +470,16 @@ __global__ __launch_bounds__(NT) void k_tile_scan(const int32_t *tcount, int ntj
     if (u < nt) {
       tile_start[u] = start;
       int o = start;
+#pragma unroll
       for (int d = 0; d < 9; ++d)
-        for (int q = 0; q < nsplit; ++q) {
-          goff[(u * 9 + d) * nsplit + q] = o;
-          o += tcount[(src[d] * nsplit + q) * 9 + d];
-        }
+#pragma unroll
+        for (int q = 0; q < kMaxSplit; ++q)
+          if (q < nsplit) {
+            goff[(u * 9 + d) * nsplit + q] = o;
+            o += g[d][q];
+          }
     }
-    const int blk_tot = sh[NT - 1];
+    const int blk_tot = sh[NTS - 1];
     __syncthreads();
     carry += blk_tot;
   }
@@ -601,7 +611,7 @@ hipError_t launch_combine(qss_ctx *c, double *const *out) {
 
 hipError_t launch_tile_reorder(qss_ctx *c) {
   const int nb = c->cur ^ 1;
-  hipLaunchKernelGGL(k_tile_scan, dim3(c->nw), dim3(NT), 0, c->stream, c->tcount, c->ntj,
+  hipLaunchKernelGGL(k_tile_scan, dim3(c->nw), dim3(NTS), 0, c->stream, c->tcount, c->ntj,
                      c->ntk, c->nsplit, c->tile_start[nb], c->goff);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
