@@ -443,7 +443,8 @@ __device__ inline void split_bf16(double x, uint16_t *p) {
   }
 }
 
-template <int BM, int BN, int BK, int WGM, int WGN, int NPC, bool GUARD, bool TOUT>
+template <int BM, int BN, int BK, int WGM, int WGN, int NPC, bool GUARD, bool TOUT,
+          bool EPI = true>
 __global__ __launch_bounds__(WGM * WGN * 64) void k_gemm_bf16(GemmArgs a) {
   constexpr int NT = WGM * WGN * 64;
   constexpr int WTM = BM / WGM, WTN = BN / WGN;
@@ -555,9 +556,9 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_gemm_bf16(GemmArgs a) {
       compute);
 
   if constexpr (!TOUT) {
-    store_tiles<GUARD, TI, TJ, RowBF16>(
-        a, B, 0, m0 + wm * WTM, n0 + wn * WTN, lane,
-        [&](int i, int j, int r) { return (double)hi[i][j][r] + (double)lo[i][j][r]; });
+    auto val = [&](int i, int j, int r) { return (double)hi[i][j][r] + (double)lo[i][j][r]; };
+    store_tiles<GUARD, TI, TJ, RowBF16, decltype(val), EPI>(a, B, 0, m0 + wm * WTM,
+                                                            n0 + wn * WTN, lane, val);
   } else {
     // transposed split store: Ct[col][row], the 4 rows of a lane's registers
     // are consecutive -> one 8-byte store per piece
@@ -629,23 +630,27 @@ __global__ __launch_bounds__(256) void k_split(SplitArgs a) {
 typedef void (*GemmFn)(GemmArgs);
 struct Variant {
   int bm, bn, bk, nt;
-  GemmFn fn[2][2];   // [guarded][tout]; dense kernels: [guarded][epilogue]
+  // [guarded][store]: store 0 = plain, 1 = with epilogue work (beta, residual,
+  // push field), 2 = transposed split store (split-bf16 y pass)
+  GemmFn fn[2][3];
   const char *name;
 };
 
 #define QSS_GEMM_DENSE(MT, BM, BN, BK, WGM, WGN, KS)                                   \
   Variant{BM, BN, BK, Cfg<BM, BN, BK, WGM, WGN, KS>::NT,                               \
           {{k_gemm<MT, BM, BN, BK, WGM, WGN, KS, false, false>,                        \
-            k_gemm<MT, BM, BN, BK, WGM, WGN, KS, false, true>},                        \
+            k_gemm<MT, BM, BN, BK, WGM, WGN, KS, false, true>, nullptr},               \
            {k_gemm<MT, BM, BN, BK, WGM, WGN, KS, true, false>,                         \
-            k_gemm<MT, BM, BN, BK, WGM, WGN, KS, true, true>}},                        \
+            k_gemm<MT, BM, BN, BK, WGM, WGN, KS, true, true>, nullptr}},               \
           #BM "x" #BN "x" #BK "_w" #WGM "x" #WGN "_k" #KS}
 
 #define QSS_GEMM_BF16(NPC, BM, BN, BK, WGM, WGN)                                       \
   Variant{BM, BN, BK, WGM * WGN * 64,                                                  \
-          {{k_gemm_bf16<BM, BN, BK, WGM, WGN, NPC, false, false>,                      \
+          {{k_gemm_bf16<BM, BN, BK, WGM, WGN, NPC, false, false, false>,               \
+            k_gemm_bf16<BM, BN, BK, WGM, WGN, NPC, false, false, true>,                \
             k_gemm_bf16<BM, BN, BK, WGM, WGN, NPC, false, true>},                      \
-           {k_gemm_bf16<BM, BN, BK, WGM, WGN, NPC, true, false>,                       \
+           {k_gemm_bf16<BM, BN, BK, WGM, WGN, NPC, true, false, false>,                \
+            k_gemm_bf16<BM, BN, BK, WGM, WGN, NPC, true, false, true>,                 \
             k_gemm_bf16<BM, BN, BK, WGM, WGN, NPC, true, true>}},                      \
           #BM "x" #BN "x" #BK "_w" #WGM "x" #WGN}
 
@@ -745,15 +750,10 @@ hipError_t launch_gemm(const GemmArgs &a, hipStream_t s, int prec, int variant, 
   const bool exact = a.M % v.bm == 0 && a.N % v.bn == 0 && a.K % (2 * v.bk) == 0 &&
                      (!tout || a.M % 4 == 0);
   const dim3 grid(tiles * a.nbatch * nw), block(v.nt);
-  // dense kernels: second index = epilogue work present (beta, residual,
-  // push field); split-bf16 kernels: second index = transposed split store
-  int second = tout;
-  if (!gemm_pieces(prec)) {
-    bool epi = a.beta != 0.0;
-    for (int b = 0; b < a.nbatch; ++b) epi = epi || a.b[b].resid >= 0 || a.b[b].push >= 0;
-    second = epi ? 1 : 0;
-  }
-  const GemmFn fn = v.fn[exact ? 0 : 1][second];
+  bool epi = a.beta != 0.0;
+  for (int b = 0; b < a.nbatch; ++b) epi = epi || a.b[b].resid >= 0 || a.b[b].push >= 0;
+  if (tout && epi) return hipErrorInvalidValue;   // the split store has no epilogue
+  const GemmFn fn = v.fn[exact ? 0 : 1][tout ? 2 : (epi ? 1 : 0)];
   if (ev0 || ev1)
     hipExtLaunchKernelGGL(fn, grid, block, 0, s, ev0, ev1, 0, a);
   else
