@@ -135,9 +135,9 @@ def timed_run(mode, a):
     the same context: (1) launches bracketed by HIP events (per-kernel times
     for the roofline), (2) plain launches, (3) layer-step pairs replayed from
     a captured hipGraph (the production path; `value` is measured here)."""
-    from tertia_amd.params import PRECISIONS, config2
+    from tertia_amd.params import PRECISIONS, two_bunch_512
     from tertia_amd.mesh import Mesh
-    p = config2(precision=PRECISIONS[mode])
+    p = two_bunch_512(precision=PRECISIONS[mode])
     m = Mesh(p, history=())
     out = {}
     try:
@@ -194,9 +194,9 @@ def timed_run(mode, a):
 
 def sampled_history(mode, every):
     """Whole sweep (from the front) with Ex and psi stored; every `every`-th layer."""
-    from tertia_amd.params import PRECISIONS, config2
+    from tertia_amd.params import PRECISIONS, two_bunch_512
     from tertia_amd.mesh import Mesh
-    p = config2(precision=PRECISIONS[mode])
+    p = two_bunch_512(precision=PRECISIONS[mode])
     m = Mesh(p, history=("Ex", "Psi"))
     try:
         m.MoveAllSplitLayers()
@@ -215,8 +215,8 @@ def adjoint_run(fd_bins: int):
     differences (h = 1e-6) on `fd_bins` random bins."""
     from tertia_amd.loss import witness_loss_np
     from tertia_amd.mesh import Mesh
-    from tertia_amd.params import config4
-    p = config4()
+    from tertia_amd.params import profile_gradient_256
+    p = profile_gradient_256()
     nrun, every = 650, 32
     win = dict(xi0=6.0, xi1=6.5, rmax=0.2)
     m = Mesh(p, history=("Ex",))
@@ -293,8 +293,8 @@ def batch_run(nslices: int, per_size: int):
     per grid size advancing all its wakes with every launch; sequential: the
     same wakes one context at a time (same tile-shape tuning per size)."""
     from tertia_amd.mesh import Mesh
-    from tertia_amd.params import Params, config3
-    groups = config3(per_size=per_size, nslices=nslices)
+    from tertia_amd.params import Params, parameter_scan
+    groups = parameter_scan(per_size=per_size, nslices=nslices)
     out = {"workload": "%d wakes: %d each on 128^2, 192^2, 256^2, 384^2 (h = 1/16), 4 ppc, "
                        "%d layers of dxi = 0.02, FP64; sigma_r~U[0.2,0.8], "
                        "sigma_xi~U[0.5,1.5], nb~U[0.5,8]" % (4 * per_size, per_size, nslices),
@@ -389,9 +389,9 @@ def main():
         return 2
     if a.copy_calibration:
         return copy_calibration(a.copy_calibration)
-    from tertia_amd.params import config2
-    if a.warmup + a.steps > config2().nslices:
-        print("warmup + steps must be <= %d slices" % config2().nslices, file=sys.stderr)
+    from tertia_amd.params import two_bunch_512
+    if a.warmup + a.steps > two_bunch_512().nslices:
+        print("warmup + steps must be <= %d slices" % two_bunch_512().nslices, file=sys.stderr)
         return 2
     modes = [s for s in a.mixed.split(",") if s]
 
@@ -467,7 +467,7 @@ def main():
               flush=True)
     if a.cpu_slices > 0:
         nthr = a.cpu_threads or min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(config2(), a.cpu_slices, nthr)
+        line["cpu_baseline"] = cpu_baseline(two_bunch_512(), a.cpu_slices, nthr)
     print(json.dumps(line))
     return 0
 

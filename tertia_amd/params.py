@@ -6,9 +6,12 @@ The fields correspond to the reference's input-deck keys (``&Domain``:
 RadiusX/Y/Z``, elements.cpp:11-67,141-146).  The build uses exactly ``ny``
 cells over ``ly`` (the reference makes ``long(L/h)+1``, domain.cpp:141-145).
 
-Presets ``config0``..``config4`` are the five workloads the sweep is measured
-and checked on (uniform plasma n0=1, 2x2 particles per cell, immobile ions,
-frozen bi-Gaussian drivers, CIC shape, 2 corrector iterations).
+Presets (the five workloads the sweep is measured and checked on; uniform
+plasma n0=1, 2x2 particles per cell, immobile ions, frozen bi-Gaussian
+drivers, CIC shape, 2 corrector iterations): ``linear_wake_64`` (64^2, 200
+layers), ``blowout_256`` (256^2, 1024 layers), ``two_bunch_512`` (512^2,
+2048 layers, driver + witness), ``parameter_scan`` (64 wakes on 128^2 ..
+384^2) and ``profile_gradient_256`` (256^2, binned driver profile).
 """
 from __future__ import annotations
 
@@ -178,7 +181,7 @@ class Params:
         return c
 
 
-def config0(**kw) -> Params:
+def linear_wake_64(**kw) -> Params:
     """12x12 k_p^-1 on 64x64, 200 slices dxi=0.05, driver sr=0.5 sxi=1 nb=0.5 at xi=2."""
     p = Params(ny=64, nz=64, ly=12.0, lz=12.0, hx=0.05, nslices=200,
                bunches=[Bunch(nb=0.5, sigma_r=0.5, sigma_xi=1.0, xi0=2.0)])
@@ -187,7 +190,7 @@ def config0(**kw) -> Params:
     return p
 
 
-def config1(**kw) -> Params:
+def blowout_256(**kw) -> Params:
     """16x16 on 256x256, 1024 slices dxi=0.02, driver sr=0.3 sxi=0.8 nb=5, +-2% noise."""
     p = Params(ny=256, nz=256, ly=16.0, lz=16.0, hx=0.02, nslices=1024,
                bunches=[Bunch(nb=5.0, sigma_r=0.3, sigma_xi=0.8, xi0=2.0, noise=0.02)])
@@ -196,7 +199,7 @@ def config1(**kw) -> Params:
     return p
 
 
-def config2(**kw) -> Params:
+def two_bunch_512(**kw) -> Params:
     """20x20 on 512x512, 2048 slices dxi=0.01; driver nb=4 sr=0.25 sxi=0.7 at
     xi=2.1 plus witness nb=8 sr=0.1 sxi=0.1 placed 4.5 behind it."""
     xd = 2.1
@@ -208,7 +211,7 @@ def config2(**kw) -> Params:
     return p
 
 
-def config3(seed: int = 20261202, per_size: int = 16, nslices: int = 512,
+def parameter_scan(seed: int = 20261202, per_size: int = 16, nslices: int = 512,
             sizes=(128, 192, 256, 384)) -> List[Params]:
     """Parameter scan: per_size independent wakes on each of the transverse grids
     `sizes` (cell size 1/16 k_p^-1, so 8, 12, 16, 24 k_p^-1 boxes), 4 ppc,
@@ -230,7 +233,7 @@ def config3(seed: int = 20261202, per_size: int = 16, nslices: int = 512,
     return groups
 
 
-def config4(**kw) -> Params:
+def profile_gradient_256(**kw) -> Params:
     """256x256 over 16x16, 1024 slices dxi=0.01, FP64; driver longitudinal profile
     = 256 piecewise-constant bins over xi in [0,4] (initial Gaussian sxi=0.8,
     nb=3 centred at 2), sr=0.3."""
@@ -248,7 +251,7 @@ def config4(**kw) -> Params:
 
 
 def small(ny=32, nz=32, nslices=20, **kw) -> Params:
-    """A scaled-down config-0-like case that the oracle finishes in well under a second."""
+    """A scaled-down linear-wake case that the oracle finishes in well under a second."""
     p = Params(ny=ny, nz=nz, ly=12.0 * ny / 64, lz=12.0 * nz / 64, hx=0.05,
                nslices=nslices,
                bunches=[Bunch(nb=0.5, sigma_r=0.5, sigma_xi=0.4, xi0=0.6)])

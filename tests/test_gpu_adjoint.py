@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from tertia_amd.loss import witness_loss_np
-from tertia_amd.params import Bunch, Params, config4
+from tertia_amd.params import Bunch, Params, profile_gradient_256
 
 pytestmark = pytest.mark.gpu
 
@@ -119,7 +119,7 @@ def test_adjoint_vs_finite_differences_small(every):
 def test_adjoint_vs_finite_differences_256():
     """256x256 x 4 ppc, 650 layers (window xi in [6, 6.5], r < 0.2), 4 bins."""
     from tertia_amd.mesh import Mesh
-    p = config4()
+    p = profile_gradient_256()
     nrun = 650
     win = dict(xi0=6.0, xi1=6.5, rmax=0.2)
     m = Mesh(p, history=("Ex",))
@@ -132,7 +132,7 @@ def test_adjoint_vs_finite_differences_256():
         m.close()
     scale = np.abs(g).max()
     err = np.abs(g[idx] - gf) / scale
-    print("config4 loss %.6e |g|max %.3e adj %s fd %s err %.2e" %
+    print("profile_gradient_256 loss %.6e |g|max %.3e adj %s fd %s err %.2e" %
           (loss, scale, np.array2string(g[idx], precision=6),
            np.array2string(gf, precision=6), err.max()))
     assert err.max() < 1e-4, err

@@ -4,7 +4,7 @@ kernel launches) give each wake exactly what a single-wake context gives
 import numpy as np
 import pytest
 
-from tertia_amd.params import Bunch, PRECISIONS, config3, small
+from tertia_amd.params import Bunch, PRECISIONS, parameter_scan, small
 
 pytestmark = pytest.mark.gpu
 
@@ -47,10 +47,10 @@ def test_batch_equals_single_wakes(prec):
     assert not np.array_equal(out_b[0]["Ex"], out_b[1]["Ex"])
 
 
-def test_config3_groups_run():
+def test_parameter_scan_groups_run():
     """The parameter-scan layout: 4 grid sizes x 2 wakes, a few layers."""
     from tertia_amd.mesh import Mesh
-    for p in config3(per_size=2, nslices=6):
+    for p in parameter_scan(per_size=2, nslices=6):
         m = Mesh(p, history=())
         try:
             m.autotune(1)

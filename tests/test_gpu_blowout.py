@@ -12,7 +12,7 @@ of the final plasma particle tracks."""
 import numpy as np
 import pytest
 
-from tertia_amd.params import config1
+from tertia_amd.params import blowout_256
 
 pytestmark = pytest.mark.gpu
 
@@ -20,16 +20,16 @@ FIELDS = ("Ex", "Ey", "Ez", "Bz", "Psi")
 
 
 @pytest.mark.timeout(600)
-def test_config1_blowout_vs_oracle(oracle_mod):
+def test_blowout_256_vs_oracle(oracle_mod):
     from tertia_amd.mesh import Mesh
-    p = config1()
+    p = blowout_256()
     m = Mesh(p, history=FIELDS)
     o = oracle_mod.Oracle(p, use_fft=True, nthreads=16, history=FIELDS)
     try:
         rg = m.MoveAllSplitLayers()
         st, ro = o.sweep()
         assert st == 0
-        print("config1 residual max diff %.1e" % np.abs(rg - ro).max(), flush=True)
+        print("blowout_256 residual max diff %.1e" % np.abs(rg - ro).max(), flush=True)
         assert np.abs(rg - ro).max() < 1e-8
         stats = {}
         for f in FIELDS:
@@ -42,7 +42,7 @@ def test_config1_blowout_vs_oracle(oracle_mod):
         og, oo = np.argsort(pg["id"]), np.argsort(po["id"])
         track = {k: float(np.abs(pg[k][og] - po[k][oo]).max())
                  for k in ("y", "z", "px", "py", "pz")}
-        print("config1 rel-L2/max", {f: "%.1e/%.1e" % v for f, v in stats.items()},
+        print("blowout_256 rel-L2/max", {f: "%.1e/%.1e" % v for f, v in stats.items()},
               "tracks", {k: "%.1e" % v for k, v in track.items()}, flush=True)
         # measured on MI355X: rel-L2 <= 7e-12, max <= 1.1e-11, tracks <= 4.4e-9
         for f, (l2, mx) in stats.items():

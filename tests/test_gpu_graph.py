@@ -6,7 +6,7 @@ parity change and eager layers)."""
 import numpy as np
 import pytest
 
-from tertia_amd.params import Bunch, PRECISIONS, config0, small
+from tertia_amd.params import Bunch, PRECISIONS, linear_wake_64, small
 
 pytestmark = pytest.mark.gpu
 
@@ -55,8 +55,8 @@ def test_graph_sweep_is_the_eager_sweep(prec):
     same(sweep(p, False), sweep(p, True))
 
 
-def test_graph_with_odd_ranges_and_config0():
-    p = config0(nslices=40)
+def test_graph_with_odd_ranges_and_linear_wake():
+    p = linear_wake_64(nslices=40)
     eager = sweep(p, False)
     same(eager, sweep(p, True))
     # ranges of odd lengths: parity switches, eager layers between replays

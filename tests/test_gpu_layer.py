@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from tests.helpers import max_rel, random_particles, smooth_field
-from tertia_amd.params import Bunch, Params, config0, small
+from tertia_amd.params import Bunch, Params, linear_wake_64, small
 
 pytestmark = pytest.mark.gpu
 
@@ -31,7 +31,7 @@ def make_pair(oracle_mod, p):
 @pytest.mark.parametrize("cfg", ["c0", "two_bunch_noise", "split3", "bins"])
 def test_beam_layer(oracle_mod, cfg):
     if cfg == "c0":
-        p = config0(nslices=60)
+        p = linear_wake_64(nslices=60)
         slices = [0, 20, 35, 40, 59]
     elif cfg == "two_bunch_noise":
         p = small(ny=48, nz=40, nslices=30,

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from tests.helpers import max_rel
-from tertia_amd.params import PRECISIONS, config0, small
+from tertia_amd.params import PRECISIONS, linear_wake_64, small
 
 pytestmark = pytest.mark.gpu
 
@@ -83,10 +83,10 @@ def sweep_history(p, fields):
 GROUP = {"Ex": "E", "Ey": "E", "Ez": "E", "Bx": "B", "By": "B", "Bz": "B", "Psi": "Psi"}
 
 
-@pytest.mark.parametrize("cfg", ["small", "config0"])
+@pytest.mark.parametrize("cfg", ["small", "linear_wake_64"])
 def test_mixed_sweep_vs_fp64(cfg):
     fields = list(GROUP)
-    make = small if cfg == "small" else config0
+    make = small if cfg == "small" else linear_wake_64
     p64 = make()
     H64, P64, r64 = sweep_history(p64, fields)
     norm = {}

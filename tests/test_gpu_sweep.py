@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from tests.helpers import rel_l2
-from tertia_amd.params import Bunch, config0, small
+from tertia_amd.params import Bunch, linear_wake_64, small
 
 pytestmark = pytest.mark.gpu
 
@@ -75,12 +75,12 @@ def test_nonlinear_subcycle_split(oracle_mod):
     m.close()
 
 
-def test_config0_full(oracle_mod):
-    """Config 0 (64x64, 200 layers, nb=0.5 driver) end to end."""
-    p = config0()
+def test_linear_wake_64_full(oracle_mod):
+    """The 64x64 linear wake (200 layers, nb=0.5 driver) end to end."""
+    p = linear_wake_64()
     m, o, rg, ro = run_pair(oracle_mod, p, use_fft=True)
     assert np.abs(rg - ro).max() < 1e-7
-    compare(m, o, p, 1e-9, 1e-9, "config0")
+    compare(m, o, p, 1e-9, 1e-9, "linear_wake_64")
     m.close()
 
 
