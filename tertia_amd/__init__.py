@@ -4,9 +4,11 @@ Product path: ``libqss.so`` (hand-written HIP kernels for gfx950 behind the C
 ABI of ``include/qss.h``) and the host mirror :class:`tertia_amd.mesh.Mesh`
 of the reference's ``Mesh`` layer-step interface.
 """
-from .params import (Bunch, Params, linear_wake_64, blowout_256, two_bunch_512, profile_gradient_256, small,  # noqa: F401
-                     FIELDS, FIELD_KINDS, QSS_KIND_C, QSS_KIND_S, QSS_SHAPE_CIC,
-                     QSS_SHAPE_NGP_REF, QSS_PREC_FP64, QSS_PREC_MIXED32)
+from .params import (Bunch, Params, linear_wake_64, blowout_256, two_bunch_512,  # noqa: F401
+                     parameter_scan, profile_gradient_256, small, FIELDS, FIELD_KINDS,
+                     QSS_KIND_C, QSS_KIND_S, QSS_SHAPE_CIC, QSS_SHAPE_NGP_REF, QSS_PREC_FP64,
+                     QSS_PREC_MIXED32)
+from .deck import DeckError, load_deck  # noqa: F401
 
 
 def Mesh(*args, **kwargs):
