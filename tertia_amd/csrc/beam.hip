@@ -74,6 +74,14 @@ __device__ void node_value(const BeamArgs &a, const qss_bunch *bun, int i, int n
     const double xi = (double)(a.nslices - ci) * a.hx;
     for (int b = 0; b < a.nbunch; ++b) {
       const qss_bunch &B = bun[b];
+      // exact early out: arg = dx^2/Rx^2 + (non-negative terms) exceeds 16 as
+      // soon as its first term does (rounded sums of non-negative terms do
+      // not decrease), so a layer outside the bunch skips its cells
+      if (!(b == 0 && a.nbins > 0)) {
+        const double Rx = sqrt(2.0) * B.sigma_xi;
+        const double dx = xi - B.xi0;
+        if (dx * dx / Rx / Rx > 16) continue;
+      }
       for (int ck = nk - 1; ck <= nk; ++ck) {
         if (ck < 0) continue;
         const int dz_ = nk - ck;
