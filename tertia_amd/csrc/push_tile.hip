@@ -66,7 +66,16 @@ struct TileArgs {
   int32_t *counters;
   long npw;                // particle slots per wake (batched contexts)
   int nsplit;              // work-groups per tile (each takes every nsplit-th chunk)
+  uint64_t ny_magic;       // ceil(2^44 / ny): cell / ny = (cell * ny_magic) >> 44
 };
+
+// cell -> (j, k) without a division: for 0 <= c < 2^28 and ny <= 2^14,
+// c * ceil(2^44/ny) / 2^44 exceeds c/ny by less than 2^-16 < 1/ny, so the
+// floor is exact
+__device__ inline void cell_jk(int32_t c, int ny, uint64_t magic, int &j, int &k) {
+  k = (int)(((uint64_t)(uint32_t)c * magic) >> 44);
+  j = c - k * ny;
+}
 
 __device__ inline int wrapi(int a, int n) {
   a %= n;
@@ -156,8 +165,7 @@ void k_push_tile(TileArgs a0) {
     R djx = 0, djy = 0, djz = 0, drho = 0, dy = 0, dz = 0, partdx = 0, step = 1;
     if (active) {
       const int32_t c = a.cell[slot];
-      j = c % ny;
-      k = c / ny;
+      cell_jk(c, ny, a.ny_magic, j, k);
       yp = a.y[slot];
       zp = a.z[slot];
       if (!(yp >= 0 && yp <= 1 && zp >= 0 && zp <= 1)) atomicOr(&a.counters[1], ST_PARTICLE);
@@ -345,8 +353,10 @@ void k_push_tile(TileArgs a0) {
       int dtj = 0, dtk = 0;
       if (active) {
         const int ntj_ = a.ntj, ntk_ = a.ntk;
-        int d1 = wrapi(jj / T - tj, ntj_);
-        int d2 = wrapi(kk / T - tk, ntk_);
+        // |jj/T - tj| < ntj: one correction wraps it
+        int d1 = jj / T - tj, d2 = kk / T - tk;
+        d1 += d1 < 0 ? ntj_ : 0;
+        d2 += d2 < 0 ? ntk_ : 0;
         dtj = d1 == 0 ? 0 : (d1 == ntj_ - 1 && ntj_ > 2 ? -1 : 1);
         dtk = d2 == 0 ? 0 : (d2 == ntk_ - 1 && ntk_ > 2 ? -1 : 1);
         if (d1 > 1 && d1 < ntj_ - 1) atomicOr(&a.counters[1], ST_FAR);
@@ -499,7 +509,9 @@ __global__ __launch_bounds__(NT) void k_permute(TileArgs a0, const int32_t *goff
   const long pw = (long)w * a0.npw;
   y += pw; z += pw; px += pw; py += pw; pz += pw; cell += pw; pid += pw;
   const int32_t c = a.scell[s];
-  const int dst_tile = ((c / a.ny) / T) * a.ntj + (c % a.ny) / T;
+  int cj, ck;
+  cell_jk(c, a.ny, a.ny_magic, cj, ck);
+  const int dst_tile = (ck / T) * a.ntj + cj / T;
   const int sd = a.sdest[s];   // direction | part << 4 | rank << 8
   const long o = goff[(dst_tile * 9 + (sd & 15)) * a.nsplit + ((sd >> 4) & 15)] + (sd >> 8);
   y[o] = a.sy[s];
@@ -585,6 +597,7 @@ TileArgs make_args(qss_ctx *c, int full) {
   a.counters = c->d_counters;
   a.npw = c->np_cap;
   a.nsplit = c->nsplit;
+  a.ny_magic = ((1ull << 44) + (uint64_t)c->ny - 1) / (uint64_t)c->ny;
   return a;
 }
 
