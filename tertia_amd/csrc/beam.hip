@@ -13,6 +13,7 @@
 // i and i+1 (movepart_splitlayer.cpp:330-348, CUDA_WRAP/cuLayers.cu:416-435).
 
 #include "qss_internal.h"
+#include "split_bf16.h"
 
 namespace {
 
@@ -29,6 +30,8 @@ struct BeamArgs {
   int layer;          // reference layer index i = nslices-1-slice
   int nsplit, isplit;
   const int *slice_dev;   // graph replay: the slice is read from device memory
+  uint16_t *xs0, *xs1;    // split-bf16 modes: the planes as split transform operands
+  int npc;
 };
 
 __device__ inline uint64_t mix64(uint64_t x) {
@@ -126,16 +129,19 @@ __global__ __launch_bounds__(NT) void k_beam_plane(BeamArgs a) {
   const int layer = a.slice_dev ? a.nslices - 1 - *a.slice_dev : a.layer;
   double rc, jc;
   node_value(a, bun, layer, nj, nk, rc, jc);
-  if (a.nsplit == 1) {
-    rho[n] = rc;
-    jx[n] = jc;
-    return;
+  if (a.nsplit > 1) {
+    double rp, jp;
+    node_value(a, bun, layer + 1, nj, nk, rp, jp);
+    const int S = a.nsplit, s = a.isplit;
+    rc = (rc * (S - s) + rp * s) / S;
+    jc = (jc * (S - s) + jp * s) / S;
   }
-  double rp, jp;
-  node_value(a, bun, layer + 1, nj, nk, rp, jp);
-  const int S = a.nsplit, s = a.isplit;
-  rho[n] = (rc * (S - s) + rp * s) / S;
-  jx[n] = (jc * (S - s) + jp * s) / S;
+  rho[n] = rc;
+  jx[n] = jc;
+  if (a.npc) {   // single wake (split modes)
+    qss_split::store(a.xs0, nc, n, rc, a.npc);
+    qss_split::store(a.xs1, nc, n, jc, a.npc);
+  }
 }
 
 }  // namespace
@@ -156,6 +162,13 @@ hipError_t launch_beam_plane(qss_ctx *c, int slice, int isplit) {
   a.nsplit = c->p.nx_split;
   a.isplit = isplit;
   a.slice_dev = c->capturing ? c->d_slice : nullptr;
+  a.xs0 = c->xs[0];
+  a.xs1 = c->xs[1];
+  a.npc = c->npc;
+  if (c->npc) {
+    c->xs_src[0] = a.rho;
+    c->xs_src[1] = a.jx;
+  }
   hipLaunchKernelGGL(k_beam_plane, dim3((unsigned)((c->ncell + NT - 1) / NT), c->nw), dim3(NT),
                      0, c->stream, a);
   return hipGetLastError();

@@ -31,6 +31,7 @@
 #include <hip/hip_ext.h>
 
 #include "qss_internal.h"
+#include "split_bf16.h"
 
 namespace {
 
@@ -430,18 +431,8 @@ struct RowBF16 {
   __device__ static int row(int lk, int r) { return 4 * lk + r; }
 };
 
-// x ~ p[0] + ... + p[NPC-1]: each piece the round-to-nearest bf16 of the
-// remainder of fp32(x), which fp32 holds exactly
 template <int NPC>
-__device__ inline void split_bf16(double x, uint16_t *p) {
-  float r = (float)x;
-#pragma unroll
-  for (int q = 0; q < NPC; ++q) {
-    const __bf16 h = (__bf16)r;
-    p[q] = __builtin_bit_cast(uint16_t, h);
-    r -= (float)h;
-  }
-}
+__device__ inline void split_bf16(double x, uint16_t *p) { qss_split::split<NPC>(x, p); }
 
 template <int BM, int BN, int BK, int WGM, int WGN, int NPC, bool GUARD, bool TOUT,
           bool EPI = true>
@@ -510,16 +501,34 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_gemm_bf16(GemmArgs a) {
     }
   };
 
+  // hi: the products of the leading pieces, summed in fp32 over one K slab
+  // only and then added into fp64 (hid), so the fp32 rounding of the largest
+  // terms does not grow with K; lo: the cross terms (2^-8 smaller and
+  // below), summed in fp32 over all of K
   f4 hi[TI][TJ], lo[TI][TJ];
+  d4 hid[TI][TJ];
 #pragma unroll
   for (int i = 0; i < TI; ++i)
 #pragma unroll
     for (int j = 0; j < TJ; ++j) {
       hi[i][j] = f4{0, 0, 0, 0};
       lo[i][j] = f4{0, 0, 0, 0};
+      hid[i][j] = d4{0, 0, 0, 0};
     }
+  auto flush_hi = [&]() {
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) hid[i][j][r] += (double)hi[i][j][r];
+        hi[i][j] = f4{0, 0, 0, 0};
+      }
+  };
 
   auto compute = [&](int buf) {
+    // the previous slab's hi sums (their MFMAs finished behind the barrier)
+    flush_hi();
 #pragma unroll
     for (int kk = 0; kk < BK / 32; ++kk) {
       const int kc = kk * 32 + 8 * lk;
@@ -554,9 +563,10 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_gemm_bf16(GemmArgs a) {
       K, [&](int k0) { load(k0, ra0, rb0); }, [&](int k0) { load(k0, ra1, rb1); },
       [&](int buf) { store(buf, ra0, rb0); }, [&](int buf) { store(buf, ra1, rb1); },
       compute);
+  flush_hi();
 
   if constexpr (!TOUT) {
-    auto val = [&](int i, int j, int r) { return (double)hi[i][j][r] + (double)lo[i][j][r]; };
+    auto val = [&](int i, int j, int r) { return hid[i][j][r] + (double)lo[i][j][r]; };
     store_tiles<GUARD, TI, TJ, RowBF16, decltype(val), EPI>(a, B, 0, m0 + wm * WTM,
                                                             n0 + wn * WTN, lane, val);
   } else {
@@ -573,7 +583,7 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_gemm_bf16(GemmArgs a) {
         uint16_t pcs[4][NPC];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          split_bf16<NPC>(alpha * ((double)hi[i][j][r] + (double)lo[i][j][r]), pcs[r]);
+          split_bf16<NPC>(alpha * (hid[i][j][r] + (double)lo[i][j][r]), pcs[r]);
 #pragma unroll
         for (int p = 0; p < NPC; ++p) {
           uint16_t *dst = B.Cts + p * pc + (long)col * M + row0;

@@ -31,6 +31,7 @@
 #include <hip/hip_ext.h>
 
 #include "qss_internal.h"
+#include "split_bf16.h"
 #include "tile_sort.h"
 
 namespace {
@@ -403,9 +404,13 @@ void k_push_tile(TileArgs a0) {
 }
 
 // sum of the windows covering each node, fixed (dtk, dtj) order
+// xs0..xs2 (split-bf16 modes, one wake): Jx, Jy, Jz also written as the split
+// operands of the forward transform that follows
 __global__ __launch_bounds__(NT) void k_combine(const double *win, int ntj, int ntk, int ny,
                                                 int nz, int nw, int nsplit, double *J0,
-                                                double *J1, double *J2, double *J3) {
+                                                double *J1, double *J2, double *J3,
+                                                uint16_t *xs0, uint16_t *xs1, uint16_t *xs2,
+                                                int npc) {
   const long n = (long)blockIdx.x * NT + threadIdx.x;   // wake-major node index
   const long plane = (long)ny * nz;
   if (n >= plane * nw) return;
@@ -432,6 +437,11 @@ __global__ __launch_bounds__(NT) void k_combine(const double *win, int ntj, int 
     }
   }
   J0[n] = s[0]; J1[n] = s[1]; J2[n] = s[2]; J3[n] = s[3];
+  if (npc) {
+    qss_split::store(xs0, plane, n, s[0], npc);
+    qss_split::store(xs1, plane, n, s[1], npc);
+    qss_split::store(xs2, plane, n, s[2], npc);
+  }
 }
 
 // new tile starts and group offsets: destination tile u receives the groups
@@ -615,10 +625,13 @@ hipError_t launch_push_tile(qss_ctx *c, int full, hipEvent_t ev0, hipEvent_t ev1
 
 hipError_t launch_combine(qss_ctx *c, double *const *out) {
   double *const J[4] = {c->C[QSS_JX], c->C[QSS_JY], c->C[QSS_JZ], c->C[QSS_RHO]};
+  const bool split = c->npc && !out;   // sweep deposits only (not the tuning runs)
   if (!out) out = J;
+  if (split)
+    for (int b = 0; b < 3; ++b) c->xs_src[b] = J[b];
   hipLaunchKernelGGL(k_combine, grid1(c->ncell * c->nw), dim3(NT), 0, c->stream, c->win,
                      c->ntj, c->ntk, c->ny, c->nz, c->nw, c->nsplit, out[0], out[1], out[2],
-                     out[3]);
+                     out[3], c->xs[0], c->xs[1], c->xs[2], split ? c->npc : 0);
   return hipGetLastError();
 }
 

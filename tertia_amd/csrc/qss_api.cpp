@@ -160,15 +160,22 @@ int validate(const qss_params *p) {
 }
 
 // ---- transforms ------------------------------------------------------------
+// the recorded split forms no longer describe their planes
+void forget_splits(qss_ctx *c) {
+  for (auto &p : c->xs_src) p = nullptr;
+}
+
 inline int gemm_pick(const qss_ctx *c, int pass, int n) {
   return c->gemm_force >= 0 ? c->gemm_force : c->gemm_var[pass][n];
 }
 
 // out_b = alpha * T_kz[b] . in_b . T_ky[b]: y-pass into tmp, then z-pass
+// reuse: the split-bf16 operands may come from the producer of the inputs
+// (c->xs_src records which planes xs holds); false for caller-owned planes
 int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
                     const double *const *in, double *const *out, double alpha,
                     const int *resid, const int *push, const double *const *pplane,
-                    double beta = 0.0, bool one_wake = false) {
+                    double beta = 0.0, bool one_wake = false, bool reuse = false) {
   if (n < 1 || n > kGemmMaxBatch) {
     qss_set_error("transform batch size");
     return QSS_E_ARG;
@@ -185,10 +192,16 @@ int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
   g.Ff = c->Ff;
   g.resid = c->d_resid;
   if (c->npc) {
-    // split-bf16: the inputs become npc bf16 planes; the y pass leaves its
-    // result transposed and split for the z pass
-    Timed t(c, KC_GEMM, true);
-    HIPCHECK(launch_split_bf16(in, c->xs, n, c->ncell, c->npc, c->stream, t.e0, t.e1));
+    // split-bf16: the inputs become npc bf16 planes (unless their producer
+    // wrote them so); the y pass leaves its result transposed and split for
+    // the z pass
+    bool have = reuse;
+    for (int b = 0; b < n; ++b) have = have && c->xs_src[b] == in[b];
+    if (!have) {
+      Timed t(c, KC_GEMM, true);
+      HIPCHECK(launch_split_bf16(in, c->xs, n, c->ncell, c->npc, c->stream, t.e0, t.e1));
+      for (int b = 0; b < n; ++b) c->xs_src[b] = reuse ? in[b] : nullptr;
+    }
     g.tout = 1;
   }
   for (int b = 0; b < n; ++b) {
@@ -227,6 +240,10 @@ int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
     Timed t(c, KC_GEMM, true);
     HIPCHECK(launch_gemm(g, c->stream, c->p.precision, gemm_pick(c, 1, n), t.e0, t.e1));
   }
+  // the outputs changed: split forms recorded for them are stale
+  for (int b = 0; b < n; ++b)
+    for (int q = 0; q < kGemmMaxBatch; ++q)
+      if (c->xs_src[q] == out[b]) c->xs_src[q] = nullptr;
   if (c->timing) c->gemm_flops += 2.0 * g.M * g.N * g.K * n * g.nw;
   return QSS_OK;
 }
@@ -262,7 +279,8 @@ int do_guess(qss_ctx *c, int first) {
     const int kz[2] = {QSS_KIND_C, QSS_KIND_C}, ky[2] = {QSS_KIND_C, QSS_KIND_C};
     const double *in[2] = {c->C[QSS_RHOBEAM], c->C[QSS_JXBEAM]};
     double *out[2] = {c->S[SP_RHOB], c->S[SP_JXB]};
-    CHECK(transform_batch(c, 2, kz, ky, in, out, 1.0, nullptr, nullptr, nullptr));
+    CHECK(transform_batch(c, 2, kz, ky, in, out, 1.0, nullptr, nullptr, nullptr, 0.0, false,
+                          true));
   }
   {
     Timed t(c, KC_SPEC);
@@ -279,7 +297,7 @@ int do_guess(qss_ctx *c, int first) {
   const double *pp[6] = {c->P[QSS_EX], c->P[QSS_EY], c->P[QSS_EZ],
                          c->P[QSS_BX], c->P[QSS_BY], c->P[QSS_BZ]};
   const double alpha = 1.0 / (4.0 * c->ny * c->nz);
-  return transform_batch(c, 6, kz, ky, in, out, alpha, nullptr, push, pp);
+  return transform_batch(c, 6, kz, ky, in, out, alpha, nullptr, push, pp, 0.0, false, true);
 }
 
 int do_push_deposit(qss_ctx *c, int full) {
@@ -296,6 +314,7 @@ int do_push_deposit(qss_ctx *c, int full) {
     if (full) c->cur ^= 1;
     return QSS_OK;
   }
+  forget_splits(c);   // the global deposit writes J without its split form
   {
     Timed t(c, KC_SORT);
     HIPCHECK(launch_fill_i32(c->cnt, c->ncell, 0, c->stream));
@@ -325,7 +344,8 @@ int do_iterate(qss_ctx *c, int final_pass, double *res_dev) {
     const int ky[3] = {QSS_KIND_C, QSS_KIND_S, QSS_KIND_C};
     const double *in[3] = {c->C[QSS_JX], c->C[QSS_JY], c->C[QSS_JZ]};
     double *out[3] = {c->S[SP_JX], c->S[SP_JY], c->S[SP_JZ]};
-    CHECK(transform_batch(c, 3, kz, ky, in, out, 1.0, nullptr, nullptr, nullptr));
+    CHECK(transform_batch(c, 3, kz, ky, in, out, 1.0, nullptr, nullptr, nullptr, 0.0, false,
+                          true));
   }
   {
     Timed t(c, KC_SPEC);
@@ -350,7 +370,7 @@ int do_iterate(qss_ctx *c, int final_pass, double *res_dev) {
   // rho and psi are outputs only (the next layer starts from the kept
   // spectra); a sweep that stores neither leaves them to its end
   const int nb = final_pass && !c->skip_rho_psi ? 8 : 6;
-  CHECK(transform_batch(c, nb, kz, ky, in, out, alpha, resid, push, pp));
+  CHECK(transform_batch(c, nb, kz, ky, in, out, alpha, resid, push, pp, 0.0, false, true));
   if (res_dev) {
     Timed t(c, KC_SPEC);
     HIPCHECK(launch_residual_final(c, res_dev));
@@ -692,6 +712,7 @@ int check_status(qss_ctx *c) {
 }
 
 int do_reset(qss_ctx *c) {
+  forget_splits(c);
   for (int f = 0; f < QSS_NFIELD; ++f) {
     HIPCHECK(launch_fill_f64(c->C[f], c->ncell * c->nw, 0.0, c->stream));
     HIPCHECK(launch_fill_f64(c->P[f], c->ncell * c->nw, 0.0, c->stream));
@@ -1189,6 +1210,7 @@ int qss_iterate(qss_ctx *c, int final_pass, double *residual) {
 
 int qss_copy_c_to_p(qss_ctx *c) {
   if (!c) return QSS_E_ARG;
+  forget_splits(c);
   swap_c_p(c);
   c->push_dirty = true;
   return QSS_OK;
@@ -1233,6 +1255,7 @@ int qss_set_layer_field(qss_ctx *c, int which, int field, const double *host) {
   if (!c || !host || field < 0 || field >= QSS_NFIELD || (which != 0 && which != 1))
     return QSS_E_ARG;
   HIPCHECK(hipSetDevice(c->device));
+  forget_splits(c);
   double *dst = (which ? c->P[field] : c->C[field]) + (size_t)c->sel * c->ncell;
   HIPCHECK(hipMemcpyAsync(dst, host, c->ncell * sizeof(double), hipMemcpyHostToDevice,
                           c->stream));
@@ -1256,6 +1279,7 @@ int qss_layer_ptr(qss_ctx *c, int which, int field, double **dev) {
   if (!c || !dev || field < 0 || field >= QSS_NFIELD || (which != 0 && which != 1))
     return QSS_E_ARG;
   *dev = (which ? c->P[field] : c->C[field]) + (size_t)c->sel * c->ncell;
+  forget_splits(c);   // the caller may write the plane
   return QSS_OK;
 }
 
@@ -1278,6 +1302,7 @@ int qss_get_spectral(qss_ctx *c, int id, double *host) {
 int qss_set_spectral(qss_ctx *c, int id, const double *host) {
   if (!c || !host || id < 0 || id >= QSS_NSPECTRAL) return QSS_E_ARG;
   HIPCHECK(hipSetDevice(c->device));
+  forget_splits(c);
   HIPCHECK(hipMemcpyAsync(c->S[id] + (size_t)c->sel * c->ncell, host,
                           c->ncell * sizeof(double), hipMemcpyHostToDevice,
                           c->stream));
@@ -1418,6 +1443,7 @@ int qss_set_gemm_choice(qss_ctx *c, const int *chosen) {
 int qss_autotune(qss_ctx *c, int reps, float *table_ms) {
   if (!c || reps < 1) return QSS_E_ARG;
   graph_free(c);
+  forget_splits(c);   // the timing runs use the split buffers as scratch
   HIPCHECK(hipSetDevice(c->device));
   hipEvent_t e0, e1;
   HIPCHECK(hipEventCreate(&e0));

@@ -150,6 +150,11 @@ struct qss_ctx {
   uint16_t *Ts[4] = {};              // split tables (npc planes each)
   uint16_t *xs[kGemmMaxBatch] = {};  // split transform inputs
   uint16_t *ts[kGemmMaxBatch] = {};  // split, transposed y-pass results
+  // the plane whose split form xs[b] holds (null: none).  Kernels that
+  // produce transform inputs write the split form too and record it here;
+  // the next transform then skips its split pass.  Any other write to such a
+  // plane (host edits, transform outputs) clears the record.
+  const double *xs_src[kGemmMaxBatch] = {};
   double *d_bins = nullptr;
 
   // particles (SoA), ping-pong for the full-step reorder

@@ -14,6 +14,7 @@
 
 #include "qss_internal.h"
 #include "spectral_mode.h"
+#include "split_bf16.h"
 
 namespace {
 
@@ -37,6 +38,10 @@ struct SpecArgs {
   int ny, nz, nw;
   double ly, lz, hx, dens;
   int flag;  // iterate: final pass
+  // split-bf16 modes: the first nxs outputs (E, B, then rho, psi) also
+  // written as the split operands of the inverse transform that follows
+  uint16_t *xs[kGemmMaxBatch];
+  int npc, nxs;
   // iterate: residual slots (kResidPerWake per wake) cleared for the inverse-GEMM
   // epilogues that follow on the stream (null: untouched)
   unsigned long long *resid;
@@ -96,6 +101,12 @@ __global__ __launch_bounds__(NT) void k_guess(SpecArgs a) {
   S[SP_BX][n1] = out[GO_BX];
   S[SP_BY][n1] = out[GO_BY];
   S[SP_BZ][n1] = out[GO_BZ];
+  if (a.nxs) {   // single wake (split modes): n1 < nc
+    const double v[6] = {out[GO_EX], out[GO_EY], out[GO_EZ],
+                         out[GO_BX], out[GO_BY], out[GO_BZ]};
+#pragma unroll
+    for (int b = 0; b < 6; ++b) qss_split::store(a.xs[b], nc, n1, v[b], a.npc);
+  }
 }
 
 __global__ __launch_bounds__(NT) void k_iterate(SpecArgs a) {
@@ -130,6 +141,11 @@ __global__ __launch_bounds__(NT) void k_iterate(SpecArgs a) {
   S[SP_BY][n1] = out[IO_BY];
   S[SP_BZ][n1] = out[IO_BZ];
   if (a.flag) S[SP_PSI][n1] = out[IO_PSI];
+  if (a.nxs) {   // single wake (split modes): n1 < nc
+    const double v[8] = {out[IO_EX], out[IO_EY], out[IO_EZ], out[IO_BX],
+                         out[IO_BY], out[IO_BZ], out[IO_RHO], a.flag ? out[IO_PSI] : 0.0};
+    for (int b = 0; b < a.nxs; ++b) qss_split::store(a.xs[b], nc, n1, v[b], a.npc);
+  }
 }
 
 // ---- adjoints (A = adjoint of the spectral state, same ids) ---------------
@@ -281,6 +297,9 @@ SpecArgs make_args(qss_ctx *c, int flag) {
   a.hx = c->hxs; a.dens = c->p.dens;
   a.flag = flag;
   a.resid = nullptr;
+  for (int b = 0; b < kGemmMaxBatch; ++b) a.xs[b] = c->xs[b];
+  a.npc = c->npc;
+  a.nxs = 0;
   return a;
 }
 
@@ -349,13 +368,23 @@ hipError_t launch_fill_u64(unsigned long long *p, long n, unsigned long long v,
 }
 
 hipError_t launch_guess_update(qss_ctx *c, int first_layer) {
-  hipLaunchKernelGGL(k_guess, grid1(c->ncell * c->nw), dim3(NT), 0, c->stream,
-                     make_args(c, first_layer));
+  SpecArgs a = make_args(c, first_layer);
+  if (c->npc) {
+    a.nxs = 6;
+    const int ids[6] = {SP_EX, SP_EY, SP_EZ, SP_BX, SP_BY, SP_BZ};
+    for (int b = 0; b < 6; ++b) c->xs_src[b] = c->S[ids[b]];
+  }
+  hipLaunchKernelGGL(k_guess, grid1(c->ncell * c->nw), dim3(NT), 0, c->stream, a);
   return hipGetLastError();
 }
 
 hipError_t launch_iterate_update(qss_ctx *c, int final_pass) {
   SpecArgs a = make_args(c, final_pass);
+  if (c->npc) {
+    a.nxs = final_pass && !c->skip_rho_psi ? 8 : 6;
+    const int ids[8] = {SP_EX, SP_EY, SP_EZ, SP_BX, SP_BY, SP_BZ, SP_RHO, SP_PSI};
+    for (int b = 0; b < a.nxs; ++b) c->xs_src[b] = c->S[ids[b]];
+  }
   a.resid = c->d_resid;
   // threads for every mode and every residual slot
   const long n = c->ncell > kResidPerWake ? c->ncell : kResidPerWake;
