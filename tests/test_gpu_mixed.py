@@ -111,3 +111,42 @@ def test_mixed_needs_tiled_grid():
     from tertia_amd.mesh import Mesh
     with pytest.raises(QssError):
         Mesh(small(ny=24, nz=32, precision=PRECISIONS["mixed32"]), history=())
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "bf16x6"])
+def test_split_operands_from_producers(prec):
+    """In split-bf16 modes the kernels producing transform inputs (window
+    combine, Guess, Iterate, beam) also write the split operands and the
+    transform skips its split pass.  The same layer steps must give the same
+    bits as when every transform splits its own inputs, including after the
+    host overwrites a produced plane (the record of that split is dropped)."""
+    from tertia_amd.mesh import Mesh
+    p = small(ny=32, nz=32, nslices=6, precision=PRECISIONS[prec])
+    rng = np.random.default_rng(11)
+    jx = rng.standard_normal((32, 32)) * 1e-2
+    outs = []
+    for forget in (False, True):
+        m = Mesh(p, history=())
+        try:
+            m.reset()
+            m.InterpolateBeamLayers(p.nslices - 1, 0)
+            if forget:
+                m.layer_ptr("Jx")          # hands out a writable plane: records dropped
+            m.GuessFieldsHydroLinLayerSplit(p.nslices - 1)
+            m.MoveParticlesLayerSplit(p.nslices - 1, 0, 0)
+            if forget:
+                m.layer_ptr("Jx")
+            m.IterateFieldsHydroLinLayerSplit(p.nslices - 1, 0, 0)
+            m.MoveParticlesLayerSplit(p.nslices - 1, 0, 0)
+            m.set_field("Jx", jx)              # host edit after the deposit
+            m.IterateFieldsHydroLinLayerSplit(p.nslices - 1, 0, 1)
+            outs.append({f: m.get_field(f) for f in ("Ex", "Ey", "Bz", "Jx")})
+            # the corrector transformed the edited plane, not the deposit
+            ref = m.transform2d(jx, 0, 0)
+            got = m.get_spectral("Jx")
+            assert np.abs(got - ref).max() <= 1e-6 * np.abs(ref).max()
+        finally:
+            m.close()
+    for f in outs[0]:
+        assert np.array_equal(outs[0][f], outs[1][f]), f
+    assert np.array_equal(outs[0]["Jx"], jx)
