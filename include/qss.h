@@ -289,6 +289,71 @@ int  qss_copy_history(qss_ctx *c, int field, int first, int count, double *dst_d
 /* Bytes of device memory in use on `device` (all processes) and in total.  */
 int  qss_device_memory(int device, int64_t *used, int64_t *total);
 
+/* Writes history plane `slice` of field f (requires the history_mask bit);
+ * the wake an evolving driver is pushed through (qss_driver_push).         */
+int  qss_set_history(qss_ctx *c, int field, int slice, const double *host);
+
+/* ---- evolving driver: Domain::Step (step.cpp:23-105) -------------------- */
+/* The bunches above are a frozen driver: every sweep deposits their seeding
+ * lattice.  An evolving driver keeps the beam macro-particles in the 3-D
+ * cell grid (layer i = x-cell, transverse cell j, k, in-cell fractions
+ * x, y, z), pushes them between sweeps through the stored wake
+ * (Mesh::MoveBeamParticles, movebeam.cpp:14-660, the host path the
+ * reference links, run_control.h:22) and deposits rho_b and Jx_b
+ * trilinearly into 3-D node arrays (Mesh::MoveBeamInCell,
+ * movebeam.cpp:662-770) that the following sweeps interpolate per layer
+ * (movepart_splitlayer.cpp:330-348) instead of the frozen lattice.         */
+typedef struct qss_driver {
+  double ts;                    /* time step Ts (&Domain Ts)                  */
+  double b_ext[3];              /* external B (BxExternal.. / sf_DimFields)   */
+  double syn_min;               /* &Synchrotron SynMin: radiation reaction of
+                                   movebeam.cpp:448-501 where 1+p^2 > SynMin^2
+                                   (reference default 100)                    */
+  double wavelength;            /* &Domain Wavelength: photon energy scale
+                                   1.2398e-4 / Wavelength (default 0.001)     */
+  double p0[QSS_MAX_BUNCH][3];  /* seeded momentum Px0, Py0, Pz0 per bunch    */
+  double q2m[QSS_MAX_BUNCH];    /* charge/mass of each bunch's particles      */
+} qss_driver;
+
+/* Mesh::SeedBeamParticles (mesh.cpp:43-64, 108-276) for every bunch: 4
+ * particles per 3-D cell at (x = 1/2, {1/3,2/3}^2) of the cells i in
+ * [0, nslices), j, k with a positive profile at the cell corner, weight =
+ * profile/4 (times the bunch's noise factor), momentum p0.  From now on the
+ * sweeps read the driver's 3-D deposit, which is zero until the first
+ * qss_driver_push.  Needs history of QSS_EX..QSS_BZ, one wake, no profile
+ * bins.  n_particles (may be NULL) receives the count.                      */
+int  qss_driver_seed(qss_ctx *c, const qss_driver *d, int64_t *n_particles);
+/* Mesh::MoveBeamParticles: clears the 3-D deposit (Mesh::ClearBeamCurrents,
+ * mesh.cpp:648-662), then every particle in an interior cell (0 <= i <
+ * nslices, 0 <= j < Ny, 0 <= k < Nz) gathers E, B trilinearly from the
+ * stored wake (layer i = history slice nslices-1-i, layer nslices = the
+ * unperturbed front, transverse ghosts periodic), gets the reference's
+ * kick / Boris rotation / kick and radiation reaction, moves by
+ * ((Vx-1) ts/hx, Vy ts/hy, Vz ts/hz) with the reference's cell clamping
+ * to [-1, N], and deposits w*polarity and w*polarity*Vx on the 8 nodes of
+ * its new cell.  Particles in clamped (ghost) cells stay where they are.
+ * Deposits are summed in 64-bit fixed point (order-independent, so the
+ * result is bitwise reproducible): resolution 2^-e with e fixed at seeding
+ * from the total |w*polarity|, at most 2^-(e+1) rounding per contribution. */
+int  qss_driver_push(qss_ctx *c);
+/* Domain::Step (step.cpp:63,105): qss_driver_push, then qss_sweep.          */
+int  qss_time_step(qss_ctx *c, double *residuals);
+/* Driver particles (host arrays, any may be NULL): cell = (i, j, k) per
+ * particle, frac = in-cell (x, y, z) in [0,1], mom = (px, py, pz), weight,
+ * bunch index.                                                              */
+int  qss_driver_get(qss_ctx *c, int32_t *cell, double *frac, double *mom, double *weight,
+                    int32_t *bunch);
+/* Replaces the driver particles (n of them, same layout; needs a seeded
+ * driver for the push parameters).  Recomputes the fixed-point scale.       */
+int  qss_driver_set(qss_ctx *c, int64_t n, const int32_t *cell, const double *frac,
+                    const double *mom, const double *weight, const int32_t *bunch);
+/* 3-D deposit of reference layer `layer` in [0, nslices] (Ny*Nz doubles
+ * each, host; either may be NULL).                                           */
+int  qss_driver_density(qss_ctx *c, int layer, double *rho, double *jx);
+/* Kernel times of the last qss_driver_push (dispatch-stamped HIP events):
+ * the clear of the previous deposit's box and the push + deposit.          */
+int  qss_driver_timing(qss_ctx *c, double *ms_clear, double *ms_push);
+
 /* ---- timing of the last qss_sweep/qss_sweep_range (HIP events) --------- */
 /* Kernel classes: 0 push, 1 transform GEMM, 2 spectral update, 3 beam,
  * 4 counting sort (fill+scan+scatter), 5 deposit tile reduce.              */

@@ -18,6 +18,8 @@
 //   beam seeding/deposit  mesh.cpp:108-276; plasma.cpp:350-377;
 //                         movebeam.cpp:509-512, 662-770
 //   beam interpolation    movepart_splitlayer.cpp:330-348
+//   evolving driver       mesh.cpp:43-64 (seeding range), movebeam.cpp:14-660
+//                         (push), 662-770 (deposit), step.cpp:63,105 (step)
 //
 // Build conventions that differ from the reference are listed at the
 // function that makes them.  Compile with -ffp-contract=off: the oracle's
@@ -211,6 +213,13 @@ struct oracle {
   uint32_t hmask = 0;
   std::vector<double> hist[QSS_NFIELD];
   double last_residual = 0;
+  // evolving driver: beam particles in the 3-D cell grid and the 3-D
+  // deposit of nodes (layer 0..L, j, k)
+  bool driver = false;
+  qss_driver drv{};
+  std::vector<int> bi, bj, bk, bsp;
+  std::vector<double> bx, by, bz, bpx, bpy, bpz, bw;
+  std::vector<double> brho3, bjx3;
 };
 
 namespace {
@@ -738,17 +747,235 @@ void beam_layer(oracle *o, int slice, int isplit) {
   const int L = o->p.nslices, S = o->p.nx_split;
   int i = L - 1 - slice;
   std::vector<double> rc(o->ncell), jc(o->ncell);
-  beam_plane(o, i, rc.data(), jc.data());
+  if (o->driver) {   // the evolving driver's 3-D deposit, nodes of layer i
+    std::copy(o->brho3.begin() + (size_t)i * o->ncell,
+              o->brho3.begin() + (size_t)(i + 1) * o->ncell, rc.begin());
+    std::copy(o->bjx3.begin() + (size_t)i * o->ncell,
+              o->bjx3.begin() + (size_t)(i + 1) * o->ncell, jc.begin());
+  } else {
+    beam_plane(o, i, rc.data(), jc.data());
+  }
   if (S == 1) {
     o->C[QSS_RHOBEAM] = rc; o->C[QSS_JXBEAM] = jc;
     return;
   }
   std::vector<double> rp(o->ncell), jp(o->ncell);
-  beam_plane(o, i + 1, rp.data(), jp.data());
+  if (o->driver) {
+    std::copy(o->brho3.begin() + (size_t)(i + 1) * o->ncell,
+              o->brho3.begin() + (size_t)(i + 2) * o->ncell, rp.begin());
+    std::copy(o->bjx3.begin() + (size_t)(i + 1) * o->ncell,
+              o->bjx3.begin() + (size_t)(i + 2) * o->ncell, jp.begin());
+  } else {
+    beam_plane(o, i + 1, rp.data(), jp.data());
+  }
   for (long n = 0; n < o->ncell; ++n) {
     o->C[QSS_RHOBEAM][n] = (rc[n] * (S - isplit) + rp[n] * isplit) / S;
     o->C[QSS_JXBEAM][n] = (jc[n] * (S - isplit) + jp[n] * isplit) / S;
   }
+}
+
+// --------------------------------------------------------------------------
+// Evolving driver (test restatement of Mesh::SeedBeamParticles,
+// Mesh::MoveBeamParticles and Mesh::MoveBeamInCell).
+// --------------------------------------------------------------------------
+
+// Mesh::SeedBeamParticles over the cells of one rank (mesh.cpp:43-64: i in
+// [0, l_Mx-1], j, k interior) for the nseed == 4 lattice (mesh.cpp:148-152:
+// x = 1/2, y,z in {1/3,2/3}), weight = GetWeight()*dens = dens/4
+// (mesh.cpp:165), momentum (Px0, Py0, Pz0) with zero spreads.  Build
+// conventions: particles are stored in (i, k, j, bunch, iseed) order (the
+// reference keeps per-cell lists); the bunch noise factor and the density
+// at the cell corner are those of the frozen lattice (beam_plane).
+void driver_seed(oracle *o, const qss_driver &d) {
+  const qss_params &p = o->p;
+  const int ny = o->ny, nz = o->nz, L = p.nslices;
+  o->drv = d;
+  o->driver = true;
+  o->bi.clear(); o->bj.clear(); o->bk.clear(); o->bsp.clear();
+  o->bx.clear(); o->by.clear(); o->bz.clear();
+  o->bpx.clear(); o->bpy.clear(); o->bpz.clear(); o->bw.clear();
+  for (int ci = 0; ci < L; ++ci) {
+    const double xi = (double)(L - ci) * p.hx;
+    for (int ck = 0; ck < nz; ++ck)
+      for (int cj = 0; cj < ny; ++cj)
+        for (int b = 0; b < p.nbunch; ++b) {
+          const qss_bunch &B = p.bunch[b];
+          const double yc = o->hy * cj - p.ly / 2.0;
+          const double zc = o->hz * ck - p.lz / 2.0;
+          const double dn = bunch_density(o, b, xi, yc, zc);
+          if (!(dn > 0.0)) continue;
+          for (int is = 0; is < 4; ++is) {
+            double w = 0.25 * dn;
+            if (B.noise != 0.0) {
+              uint64_t cell = ((uint64_t)(ci + 1) * (uint64_t)nz + ck) * (uint64_t)ny + cj;
+              w *= 1.0 + B.noise * unif_pm1(p.seed ^ 0xB3A1ull, cell * 4 + is, (uint64_t)b + 7);
+            }
+            o->bi.push_back(ci); o->bj.push_back(cj); o->bk.push_back(ck);
+            o->bsp.push_back(b);
+            o->bx.push_back(0.5);
+            o->by.push_back(1.0 / 3.0 * (1.0 + is % 2));
+            o->bz.push_back(1.0 / 3.0 * (1.0 + is / 2));
+            o->bpx.push_back(d.p0[b][0]); o->bpy.push_back(d.p0[b][1]);
+            o->bpz.push_back(d.p0[b][2]);
+            o->bw.push_back(w);
+          }
+        }
+  }
+  o->brho3.assign((size_t)(L + 1) * o->ncell, 0.0);
+  o->bjx3.assign((size_t)(L + 1) * o->ncell, 0.0);
+}
+
+// Field f of the stored wake at 3-D cell (i, j, k): layer i < L is history
+// slice L-1-i (the layer stored at movepart_splitlayer.cpp:467-473), layer L
+// is the unperturbed front (zero); the stored transverse ghosts j = Ny,
+// k = Nz are the periodic copies of ExchangeFieldsSplit
+// (movepart_splitlayer.cpp:1659-1682).
+inline double wake_field(const oracle *o, int f, int i, int j, int k) {
+  const int L = o->p.nslices;
+  if (i < 0 || i >= L) return 0.0;
+  if (j == o->ny) j = 0;
+  if (k == o->nz) k = 0;
+  return o->hist[f][(size_t)(L - 1 - i) * o->ncell + j + (long)o->ny * k];
+}
+
+// Mesh::MoveBeamParticles (movebeam.cpp:14-660), one rank, no scattering
+// (ifscatter = 0), in the reference's operation order.  Particles outside
+// the interior cells are not visited (the loops of movebeam.cpp:63-69), so
+// they are neither pushed nor deposited.  The E kick is the interpolated
+// field itself: its q2m*ts/2 scaling is commented out in the reference
+// (movebeam.cpp:283-302).  Returns QSS_E_PARTICLE for an in-cell fraction
+// outside [0,1] (the reference's exit(-12), movebeam.cpp:200-206).
+int driver_push(oracle *o) {
+  const qss_params &p = o->p;
+  const qss_driver &d = o->drv;
+  const int ny = o->ny, nz = o->nz, L = p.nslices;
+  const double ts = d.ts, hx = p.hx, hy = o->hy, hz = o->hz;
+  const double PI_REF = 3.1415927;                       // domain.h:10
+  const double ElaserPhoton = 1.2398e-4 / d.wavelength;  // movebeam.cpp:19
+  std::fill(o->brho3.begin(), o->brho3.end(), 0.0);     // ClearBeamCurrents
+  std::fill(o->bjx3.begin(), o->bjx3.end(), 0.0);
+  const long np = (long)o->bx.size();
+  for (long n = 0; n < np; ++n) {
+    const int i = o->bi[n], j = o->bj[n], k = o->bk[n];
+    if (i < 0 || i >= L || j < 0 || j >= ny || k < 0 || k >= nz) continue;
+    const qss_bunch &B = p.bunch[o->bsp[n]];
+    const double weight = o->bw[n];
+    const double x = o->bx[n], y = o->by[n], z = o->bz[n];
+    double px = o->bpx[n], py = o->bpy[n], pz = o->bpz[n];
+    const double pxn = px, pyn = py, pzn = pz;
+    const double q2m = d.q2m[o->bsp[n]];
+    if (x < 0 || x > 1 || y < 0 || y > 1 || z < 0 || z > 1) return QSS_E_PARTICLE;
+    const double axc = 1. - x, axp = x, ayc = 1. - y, ayp = y, azc = 1. - z, azp = z;
+    const double accc = axc * ayc * azc, acpc = axc * ayp * azc;
+    const double accp = axc * ayc * azp, acpp = axc * ayp * azp;
+    const double apcc = axp * ayc * azc, appc = axp * ayp * azc;
+    const double apcp = axp * ayc * azp, appp = axp * ayp * azp;
+    double g[6];
+    for (int f = 0; f < 6; ++f)   // movebeam.cpp:232-272 term order
+      g[f] = accc * wake_field(o, f, i, j, k) + acpc * wake_field(o, f, i, j + 1, k) +
+             accp * wake_field(o, f, i, j, k + 1) + acpp * wake_field(o, f, i, j + 1, k + 1) +
+             apcc * wake_field(o, f, i + 1, j, k) + appc * wake_field(o, f, i + 1, j + 1, k) +
+             apcp * wake_field(o, f, i + 1, j, k + 1) +
+             appp * wake_field(o, f, i + 1, j + 1, k + 1);
+    const double ex = g[0], ey = g[1], ez = g[2];
+    double bx = g[3], by = g[4], bz = g[5];
+    bx += d.b_ext[0];                                    // movebeam.cpp:311-313
+    by += d.b_ext[1];
+    bz += d.b_ext[2];
+    px += ex;
+    py += ey;
+    pz += ez;
+    double gamma = sqrt(1. + px * px + py * py + pz * pz);
+    const double gamma_r = 1. / gamma;
+    bx = bx * gamma_r * q2m * ts / 2.;                   // movebeam.cpp:379-381
+    by = by * gamma_r * q2m * ts / 2.;
+    bz = bz * gamma_r * q2m * ts / 2.;
+    const double co = 2. / (1. + (bx * bx) + (by * by) + (bz * bz));
+    double p3x = py * bz - pz * by + px;
+    double p3y = pz * bx - px * bz + py;
+    double p3z = px * by - py * bx + pz;
+    p3x *= co;
+    p3y *= co;
+    p3z *= co;
+    const double px_new = p3y * bz - p3z * by;
+    const double py_new = p3z * bx - p3x * bz;
+    const double pz_new = p3x * by - p3y * bx;
+    px += (ex + px_new);
+    py += (ey + py_new);
+    pz += (ez + pz_new);
+    // radiation reaction (movebeam.cpp:448-501); the photon histogram
+    // (Synchrotron::AddParticle) is a diagnostic and not restated
+    const double p2 = px * px + py * py + pz * pz;
+    if (1. + p2 > d.syn_min * d.syn_min) {
+      double dpx = px - pxn, dpy = py - pyn, dpz = pz - pzn;
+      const double pdp = px * dpx + py * dpy + pz * dpz;
+      dpx -= px * (pdp / p2);
+      dpy -= py * (pdp / p2);
+      dpz -= pz * (pdp / p2);
+      const double p2absrev = 1.f / sqrt(p2);
+      const double nx = px * p2absrev, ny_ = py * p2absrev, nz_ = pz * p2absrev;
+      const double dpabs = sqrt(dpx * dpx + dpy * dpy + dpz * dpz);
+      const double Gamma2 = 1. + p2;
+      const double omega_c_omega_laser = 3. * Gamma2 * dpabs / (4. * PI_REF * ts);
+      const double Ephoton = ElaserPhoton * omega_c_omega_laser;
+      const double alfa = 1. / 137.;
+      const double nph = 4. / 9. * alfa * dpabs;
+      px -= (nph * Ephoton * nx / 5.11e5f);
+      py -= (nph * Ephoton * ny_ / 5.11e5f);
+      pz -= (nph * Ephoton * nz_ / 5.11e5f);
+    }
+    gamma = 1. / sqrt(1. + px * px + py * py + pz * pz);
+    const double Vx = px * gamma, Vy = py * gamma, Vz = pz * gamma;
+    o->bpx[n] = px; o->bpy[n] = py; o->bpz[n] = pz;
+    const double polarity = B.polarity;
+    const double djx = weight * polarity * Vx;
+    const double drho = weight * polarity;
+    // move (movebeam.cpp:520-583)
+    const double dx = (px * gamma - 1.) * ts / hx;
+    const double dy = py * gamma * ts / hy;
+    const double dz = pz * gamma * ts / hz;
+    // a non-finite or huge move would never leave the wrap loops below
+    if (!(std::fabs(dx) < 1e9 && std::fabs(dy) < 1e9 && std::fabs(dz) < 1e9))
+      return QSS_E_PARTICLE;
+    double xtmp = x + dx, ytmp = y + dy, ztmp = z + dz;
+    int i_jump = (int)xtmp, j_jump = (int)ytmp, k_jump = (int)ztmp;
+    if (xtmp < 0.) i_jump--;
+    if (ytmp < 0.) j_jump--;
+    if (ztmp < 0.) k_jump--;
+    int itmp = i + i_jump, jtmp = j + j_jump, ktmp = k + k_jump;
+    if (itmp < 0) itmp = -1;
+    if (itmp > L - 1) itmp = L;
+    if (jtmp < 0) jtmp = -1;
+    if (jtmp > ny - 1) jtmp = ny;
+    if (ktmp < 0) ktmp = -1;
+    if (ktmp > nz - 1) ktmp = nz;
+    xtmp -= i_jump;
+    ytmp -= j_jump;
+    ztmp -= k_jump;
+    while (xtmp < 0.) xtmp += 1.;
+    while (xtmp > 1.) xtmp -= 1.;
+    while (ytmp < 0.) ytmp += 1.;
+    while (ytmp > 1.) ytmp -= 1.;
+    while (ztmp < 0.) ztmp += 1.;
+    while (ztmp > 1.) ztmp -= 1.;
+    o->bi[n] = itmp; o->bj[n] = jtmp; o->bk[n] = ktmp;
+    o->bx[n] = xtmp; o->by[n] = ytmp; o->bz[n] = ztmp;
+    // Mesh::MoveBeamInCell (movebeam.cpp:662-770): rho_b and Jx_b on the 8
+    // nodes of the new cell; nodes outside layers [0, L] and the interior
+    // transverse range are dropped (beam currents are not ghost-folded)
+    const double ax[2] = {1. - xtmp, xtmp}, ay[2] = {1. - ytmp, ytmp},
+                 az[2] = {1. - ztmp, ztmp};
+    for (int a = 0; a < 2; ++a)
+      for (int c = 0; c < 2; ++c)
+        for (int b = 0; b < 2; ++b) {
+          const int ni = itmp + a, nj = jtmp + b, nk = ktmp + c;
+          if (ni < 0 || ni > L || nj < 0 || nj >= ny || nk < 0 || nk >= nz) continue;
+          const size_t m = (size_t)ni * o->ncell + nj + (size_t)ny * nk;
+          o->bjx3[m] += djx * ax[a] * ay[b] * az[c];
+          o->brho3[m] += drho * ax[a] * ay[b] * az[c];
+        }
+  }
+  return QSS_OK;
 }
 
 void copy_c_to_p(oracle *o) {
@@ -921,6 +1148,66 @@ int oracle_set_particles(oracle *o, long np, const int32_t *cj, const int32_t *c
   o->j.assign(cj, cj + np); o->k.assign(ck, ck + np);
   o->px.assign(px, px + np); o->py.assign(py, py + np); o->pz.assign(pz, pz + np);
   o->id.assign(id, id + np);
+  return QSS_OK;
+}
+
+int oracle_set_history(oracle *o, int field, int slice, const double *in) {
+  if (field < 0 || field >= QSS_NFIELD || o->hist[field].empty() || slice < 0 ||
+      slice >= o->p.nslices)
+    return QSS_E_ARG;
+  std::copy(in, in + o->ncell, o->hist[field].begin() + (size_t)slice * o->ncell);
+  return QSS_OK;
+}
+
+// evolving driver
+int oracle_driver_seed(oracle *o, const qss_driver *d, int64_t *n) {
+  for (int f = QSS_EX; f <= QSS_BZ; ++f)
+    if (o->hist[f].empty()) return QSS_E_STATE;
+  driver_seed(o, *d);
+  if (n) *n = (int64_t)o->bx.size();
+  return QSS_OK;
+}
+int oracle_driver_push(oracle *o) { return o->driver ? driver_push(o) : QSS_E_STATE; }
+// Domain::Step (step.cpp:63,105): beam push, then the sweep
+int oracle_time_step(oracle *o, double *residuals) {
+  if (!o->driver) return QSS_E_STATE;
+  int st = driver_push(o);
+  if (st != QSS_OK) return st;
+  reset(o);
+  return oracle_sweep_range(o, 0, o->p.nslices, residuals);
+}
+int64_t oracle_driver_count(const oracle *o) { return (int64_t)o->bx.size(); }
+int oracle_driver_get(oracle *o, int32_t *cell, double *frac, double *mom, double *weight,
+                      int32_t *bunch) {
+  const long np = (long)o->bx.size();
+  for (long n = 0; n < np; ++n) {
+    if (cell) { cell[3 * n] = o->bi[n]; cell[3 * n + 1] = o->bj[n]; cell[3 * n + 2] = o->bk[n]; }
+    if (frac) { frac[3 * n] = o->bx[n]; frac[3 * n + 1] = o->by[n]; frac[3 * n + 2] = o->bz[n]; }
+    if (mom) { mom[3 * n] = o->bpx[n]; mom[3 * n + 1] = o->bpy[n]; mom[3 * n + 2] = o->bpz[n]; }
+    if (weight) weight[n] = o->bw[n];
+    if (bunch) bunch[n] = o->bsp[n];
+  }
+  return QSS_OK;
+}
+int oracle_driver_set(oracle *o, int64_t n, const int32_t *cell, const double *frac,
+                      const double *mom, const double *weight, const int32_t *bunch) {
+  if (!o->driver || n < 0) return QSS_E_STATE;
+  o->bi.resize(n); o->bj.resize(n); o->bk.resize(n); o->bsp.resize(n);
+  o->bx.resize(n); o->by.resize(n); o->bz.resize(n);
+  o->bpx.resize(n); o->bpy.resize(n); o->bpz.resize(n); o->bw.resize(n);
+  for (int64_t q = 0; q < n; ++q) {
+    o->bi[q] = cell[3 * q]; o->bj[q] = cell[3 * q + 1]; o->bk[q] = cell[3 * q + 2];
+    o->bx[q] = frac[3 * q]; o->by[q] = frac[3 * q + 1]; o->bz[q] = frac[3 * q + 2];
+    o->bpx[q] = mom[3 * q]; o->bpy[q] = mom[3 * q + 1]; o->bpz[q] = mom[3 * q + 2];
+    o->bw[q] = weight[q]; o->bsp[q] = bunch[q];
+  }
+  return QSS_OK;
+}
+int oracle_driver_density(oracle *o, int layer, double *rho, double *jx) {
+  if (!o->driver || layer < 0 || layer > o->p.nslices) return QSS_E_ARG;
+  const size_t off = (size_t)layer * o->ncell;
+  if (rho) std::copy(o->brho3.begin() + off, o->brho3.begin() + off + o->ncell, rho);
+  if (jx) std::copy(o->bjx3.begin() + off, o->bjx3.begin() + off + o->ncell, jx);
   return QSS_OK;
 }
 

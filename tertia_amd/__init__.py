@@ -4,7 +4,7 @@ Product path: ``libqss.so`` (hand-written HIP kernels for gfx950 behind the C
 ABI of ``include/qss.h``) and the host mirror :class:`tertia_amd.mesh.Mesh`
 of the reference's ``Mesh`` layer-step interface.
 """
-from .params import (Bunch, Params, linear_wake_64, blowout_256, two_bunch_512,  # noqa: F401
+from .params import (Bunch, Driver, Params, linear_wake_64, blowout_256, two_bunch_512,  # noqa: F401
                      parameter_scan, profile_gradient_256, small, FIELDS, FIELD_KINDS,
                      QSS_KIND_C, QSS_KIND_S, QSS_SHAPE_CIC, QSS_SHAPE_NGP_REF, QSS_PREC_FP64,
                      QSS_PREC_MIXED32)

@@ -9,7 +9,7 @@ from __future__ import annotations
 import ctypes
 import os
 
-from .params import QssParams
+from .params import QssDriver, QssParams
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libqss.so")
@@ -45,6 +45,14 @@ SIGNATURES = {
     "qss_get_layer_field": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
     "qss_set_layer_field": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
     "qss_get_history": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "qss_set_history": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "qss_driver_seed": (ctypes.c_int, [_vp, ctypes.POINTER(QssDriver), _i64p]),
+    "qss_driver_push": (ctypes.c_int, [_vp]),
+    "qss_time_step": (ctypes.c_int, [_vp, _dp]),
+    "qss_driver_get": (ctypes.c_int, [_vp, _i32p, _dp, _dp, _dp, _i32p]),
+    "qss_driver_set": (ctypes.c_int, [_vp, ctypes.c_int64, _i32p, _dp, _dp, _dp, _i32p]),
+    "qss_driver_density": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
+    "qss_driver_timing": (ctypes.c_int, [_vp, _dp, _dp]),
     "qss_history_ptr": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp)]),
     "qss_layer_ptr": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "qss_get_particles": (ctypes.c_int, [_vp, _i32p, _i32p, _dp, _dp, _dp, _dp, _dp, _i64p]),

@@ -26,13 +26,14 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
           "-Wno-unused-command-line-argument"]
-NO_CONTRACT = {"particles.hip", "push_tile.hip", "beam.hip", "spectral.hip", "adjoint.hip"}
+NO_CONTRACT = {"particles.hip", "push_tile.hip", "beam.hip", "spectral.hip", "adjoint.hip",
+               "driver.hip"}
 # MFMA accumulators in VGPRs: with AGPR accumulators the compiler copies
 # every loop-carried accumulator AGPR->VGPR->AGPR at the K-loop back-edge,
 # draining the MFMA pipeline once per iteration
 MFMA_VGPR = {"gemm.hip"}
 SOURCES = ["gemm.hip", "spectral.hip", "beam.hip", "particles.hip", "push_tile.hip",
-           "adjoint.hip", "qss_api.cpp"]
+           "adjoint.hip", "driver.hip", "qss_api.cpp"]
 
 
 def _compile(src: str) -> str:

@@ -147,6 +147,7 @@ __global__ __launch_bounds__(NT) void k_beam_plane(BeamArgs a) {
 }  // namespace
 
 hipError_t launch_beam_plane(qss_ctx *c, int slice, int isplit) {
+  if (c->drv.on) return launch_driver_plane(c, slice, isplit);   // evolving driver
   BeamArgs a;
   a.ny = c->ny; a.nz = c->nz; a.nslices = c->p.nslices; a.nbunch = c->p.nbunch;
   a.nbins = c->p.nbins;

@@ -14,6 +14,7 @@
 #include "qss_internal.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -711,6 +712,70 @@ int check_status(qss_ctx *c) {
   return QSS_OK;
 }
 
+// ---- evolving driver: host side -------------------------------------------
+// splitmix64 draws of the frozen lattice's weight noise (the same integer
+// arithmetic as the beam kernels, so both lattices carry the same weights)
+uint64_t host_mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+double host_unif_pm1(uint64_t seed, uint64_t a, uint64_t b) {
+  const uint64_t h = host_mix64(seed ^ host_mix64(a ^ host_mix64(b)));
+  return (double)(h >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+}
+
+void driver_free(DriverState &d) {
+  dfree(d.ci); dfree(d.cj); dfree(d.ck); dfree(d.bunch);
+  dfree(d.x); dfree(d.y); dfree(d.z); dfree(d.px); dfree(d.py); dfree(d.pz); dfree(d.w);
+  d.np = 0;
+}
+
+// fixed-point scale 2^e of the deposit: |any node sum| <= W = sum |w*pol|
+// (trilinear weights in [0,1] summing to 1, |Vx| < 1), so W * 2^e < 2^61
+// leaves room for the per-contribution rounding in a signed 64-bit sum
+double driver_scale(const std::vector<double> &w, const std::vector<int32_t> &b,
+                    const qss_params &p) {
+  double W = 0.0;
+  for (size_t n = 0; n < w.size(); ++n) W += std::fabs(w[n] * p.bunch[b[n]].polarity);
+  if (!(W > 0.0)) return 1.0;
+  int ex = 0;
+  (void)std::frexp(W, &ex);   // W < 2^ex
+  return std::ldexp(1.0, 61 - ex);
+}
+
+// particle arrays of n entries from host vectors
+int driver_upload(qss_ctx *c, long n, const std::vector<int32_t> &cell,
+                  const std::vector<double> &frac, const std::vector<double> &mom,
+                  const std::vector<double> &w, const std::vector<int32_t> &bunch) {
+  DriverState &d = c->drv;
+  driver_free(d);
+  std::vector<int32_t> ci(n), cj(n), ck(n);
+  std::vector<double> x(n), y(n), z(n), px(n), py(n), pz(n);
+  for (long q = 0; q < n; ++q) {
+    ci[q] = cell[3 * q]; cj[q] = cell[3 * q + 1]; ck[q] = cell[3 * q + 2];
+    x[q] = frac[3 * q]; y[q] = frac[3 * q + 1]; z[q] = frac[3 * q + 2];
+    px[q] = mom[3 * q]; py[q] = mom[3 * q + 1]; pz[q] = mom[3 * q + 2];
+  }
+  int32_t **ip[4] = {&d.ci, &d.cj, &d.ck, &d.bunch};
+  const std::vector<int32_t> *iv[4] = {&ci, &cj, &ck, &bunch};
+  for (int a = 0; a < 4; ++a) {
+    CHECK(dalloc(ip[a], (size_t)n));
+    if (n) HIPCHECK(hipMemcpy(*ip[a], iv[a]->data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  double **dp[7] = {&d.x, &d.y, &d.z, &d.px, &d.py, &d.pz, &d.w};
+  const std::vector<double> *dv[7] = {&x, &y, &z, &px, &py, &pz, &w};
+  for (int a = 0; a < 7; ++a) {
+    CHECK(dalloc(dp[a], (size_t)n));
+    if (n) HIPCHECK(hipMemcpy(*dp[a], dv[a]->data(), n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  d.np = n;
+  d.scale = driver_scale(w, bunch, c->p);
+  graph_free(c);   // a captured beam-plane launch holds the old scale
+  return QSS_OK;
+}
+
 int do_reset(qss_ctx *c) {
   forget_splits(c);
   for (int f = 0; f < QSS_NFIELD; ++f) {
@@ -903,6 +968,10 @@ int qss_destroy(qss_ctx *c) {
   graph_free(c);
   dfree(c->d_slice);
   adj_free(c);
+  driver_free(c->drv);
+  dfree(c->drv.acc_rho); dfree(c->drv.acc_jx); dfree(c->drv.box); dfree(c->drv.status);
+  for (auto &e : c->drv.ev)
+    if (e) (void)hipEventDestroy(e);
   for (auto &p : c->d_T) dfree(p);
   for (int f = 0; f < QSS_NFIELD; ++f) { dfree(c->C[f]); dfree(c->P[f]); dfree(c->hist[f]); }
   for (auto &p : c->S) dfree(p);
@@ -1564,6 +1633,227 @@ int qss_timing(qss_ctx *c, double *ms_total, double *ms_kernel, int64_t *launche
     if (launches) launches[k] = (int64_t)c->ev_used[k];
   }
   if (gemm_flops) *gemm_flops = c->gemm_flops;
+  return QSS_OK;
+}
+
+int qss_set_history(qss_ctx *c, int field, int slice, const double *host) {
+  if (!c || !host || field < 0 || field >= QSS_NFIELD || !c->hist[field] || slice < 0 ||
+      slice >= c->p.nslices) {
+    qss_set_error("set_history: no history for this field or slice out of range");
+    return QSS_E_ARG;
+  }
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipMemcpyAsync(c->hist[field] + ((size_t)c->sel * c->p.nslices + slice) * c->ncell,
+                          host, c->ncell * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return QSS_OK;
+}
+
+// Mesh::SeedBeamParticles over cells i in [0, nslices) (mesh.cpp:43-64),
+// nseed == 4 lattice (mesh.cpp:148-152), weight = profile/4 (mesh.cpp:165).
+// Particles are stored in (i, k, j, bunch, iseed) order.  Cells outside a
+// bunch's support are skipped by exact early outs: the profile is zero when
+// arg = dx^2/Rx^2 + dy^2/R^2 + dz^2/R^2 > 16, and a rounded sum of
+// non-negative terms is at least each of its terms.
+int qss_driver_seed(qss_ctx *c, const qss_driver *d, int64_t *n_particles) {
+  if (!c || !d) return QSS_E_ARG;
+  if (c->nw != 1 || c->p.nbins > 0) {
+    qss_set_error("driver: one wake with Gaussian bunches (no profile bins)");
+    return QSS_E_ARG;
+  }
+  for (int f = QSS_EX; f <= QSS_BZ; ++f)
+    if (!c->hist[f]) {
+      qss_set_error("driver: needs the history of Ex, Ey, Ez, Bx, By, Bz");
+      return QSS_E_STATE;
+    }
+  if (!(std::isfinite(d->ts) && d->wavelength > 0.0)) {
+    qss_set_error("driver: ts must be finite and wavelength positive");
+    return QSS_E_ARG;
+  }
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  const qss_params &p = c->p;
+  const int L = p.nslices, ny = c->ny, nz = c->nz;
+  std::vector<int32_t> cell, bunch;
+  std::vector<double> frac, mom, w;
+  for (int ci = 0; ci < L; ++ci) {
+    const double xi = (double)(L - ci) * p.hx;
+    for (int ck = 0; ck < nz; ++ck) {
+      const double zc = c->hz * ck - p.lz / 2.0;
+      for (int cj = 0; cj < ny; ++cj) {
+        const double yc = c->hy * cj - p.ly / 2.0;
+        for (int b = 0; b < p.nbunch; ++b) {
+          const qss_bunch &B = p.bunch[b];
+          const double Rx = std::sqrt(2.0) * B.sigma_xi, Ry = std::sqrt(2.0) * B.sigma_r;
+          const double dx = xi - B.xi0, dy = yc - B.y0, dz = zc - B.z0;
+          if (dx * dx / Rx / Rx > 16) continue;
+          if (dz * dz / Ry / Ry > 16) continue;
+          const double arg = dx * dx / Rx / Rx + dy * dy / Ry / Ry + dz * dz / Ry / Ry;
+          if (arg > 16) continue;
+          const double dn = B.nb * std::exp(-arg);
+          if (!(dn > 0.0)) continue;
+          for (int is = 0; is < 4; ++is) {
+            double wt = 0.25 * dn;
+            if (B.noise != 0.0) {
+              const uint64_t cl = ((uint64_t)(ci + 1) * (uint64_t)nz + ck) * (uint64_t)ny + cj;
+              wt *= 1.0 + B.noise * host_unif_pm1(p.seed ^ 0xB3A1ull, cl * 4 + is,
+                                                   (uint64_t)b + 7);
+            }
+            cell.insert(cell.end(), {ci, cj, ck});
+            frac.insert(frac.end(), {0.5, 1.0 / 3.0 * (1.0 + is % 2), 1.0 / 3.0 * (1.0 + is / 2)});
+            mom.insert(mom.end(), {d->p0[b][0], d->p0[b][1], d->p0[b][2]});
+            w.push_back(wt);
+            bunch.push_back(b);
+          }
+        }
+      }
+    }
+  }
+  DriverState &ds = c->drv;
+  ds.par = *d;
+  CHECK(driver_upload(c, (long)w.size(), cell, frac, mom, w, bunch));
+  const size_t nodes = (size_t)(L + 1) * c->ncell;
+  if (!ds.acc_rho) CHECK(dalloc(&ds.acc_rho, nodes));
+  else HIPCHECK(hipMemset(ds.acc_rho, 0, nodes * sizeof(unsigned long long)));
+  if (!ds.acc_jx) CHECK(dalloc(&ds.acc_jx, nodes));
+  else HIPCHECK(hipMemset(ds.acc_jx, 0, nodes * sizeof(unsigned long long)));
+  if (!ds.box) CHECK(dalloc(&ds.box, 6));
+  if (!ds.status) CHECK(dalloc(&ds.status, 1));
+  for (auto &e : ds.ev)
+    if (!e) HIPCHECK(hipEventCreate(&e));
+  ds.box_lo[0] = 1; ds.box_hi[0] = 0;   // nothing deposited yet
+  ds.on = true;
+  graph_free(c);   // a captured graph holds the frozen lattice's kernel
+  if (n_particles) *n_particles = (int64_t)ds.np;
+  return QSS_OK;
+}
+
+int qss_driver_push(qss_ctx *c) {
+  if (!c) return QSS_E_ARG;
+  DriverState &d = c->drv;
+  if (!d.on) { qss_set_error("driver: not seeded"); return QSS_E_STATE; }
+  HIPCHECK(hipSetDevice(c->device));
+  // Mesh::ClearBeamCurrents: only the nodes the previous push touched are
+  // non-zero
+  const bool clear = d.box_lo[0] <= d.box_hi[0];
+  if (clear) HIPCHECK(hipEventRecord(d.ev[0], c->stream));
+  HIPCHECK(launch_driver_clear(c));
+  if (clear) HIPCHECK(hipEventRecord(d.ev[1], c->stream));
+  const int32_t empty[6] = {INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN};
+  HIPCHECK(hipMemcpyAsync(d.box, empty, sizeof(empty), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(launch_fill_i32(d.status, 1, 0, c->stream));
+  HIPCHECK(launch_driver_push(c, d.ev[2], d.ev[3]));
+  int32_t box[6], st = 0;
+  HIPCHECK(hipMemcpyAsync(box, d.box, sizeof(box), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipMemcpyAsync(&st, d.status, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  d.ms_clear = 0.f;
+  if (clear) HIPCHECK(hipEventElapsedTime(&d.ms_clear, d.ev[0], d.ev[1]));
+  d.ms_push = 0.f;
+  if (d.np) HIPCHECK(hipEventElapsedTime(&d.ms_push, d.ev[2], d.ev[3]));
+  if (box[0] <= box[1]) {
+    for (int q = 0; q < 3; ++q) {
+      const int lim = q == 0 ? c->p.nslices : (q == 1 ? c->ny - 1 : c->nz - 1);
+      d.box_lo[q] = std::max(0, box[2 * q]);
+      d.box_hi[q] = std::min(lim, box[2 * q + 1]);
+    }
+  } else {
+    d.box_lo[0] = 1; d.box_hi[0] = 0;
+  }
+  if (st & 2) { qss_set_error("driver: non-finite or unbounded move"); return QSS_E_PARTICLE; }
+  if (st & 1) {
+    qss_set_error("driver: in-cell fraction outside [0,1] (reference exit(-12))");
+    return QSS_E_PARTICLE;
+  }
+  return QSS_OK;
+}
+
+int qss_time_step(qss_ctx *c, double *residuals) {
+  CHECK(qss_driver_push(c));
+  return qss_sweep(c, residuals);
+}
+
+int qss_driver_get(qss_ctx *c, int32_t *cell, double *frac, double *mom, double *weight,
+                   int32_t *bunch) {
+  if (!c) return QSS_E_ARG;
+  DriverState &d = c->drv;
+  if (!d.on) { qss_set_error("driver: not seeded"); return QSS_E_STATE; }
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  const long n = d.np;
+  if (n == 0) return QSS_OK;
+  std::vector<int32_t> iv(n);
+  std::vector<double> dv(n);
+  int32_t *ia[3] = {d.ci, d.cj, d.ck};
+  double *fa[3] = {d.x, d.y, d.z}, *ma[3] = {d.px, d.py, d.pz};
+  for (int a = 0; a < 3; ++a) {
+    if (cell) {
+      HIPCHECK(hipMemcpy(iv.data(), ia[a], n * sizeof(int32_t), hipMemcpyDeviceToHost));
+      for (long q = 0; q < n; ++q) cell[3 * q + a] = iv[q];
+    }
+    if (frac) {
+      HIPCHECK(hipMemcpy(dv.data(), fa[a], n * sizeof(double), hipMemcpyDeviceToHost));
+      for (long q = 0; q < n; ++q) frac[3 * q + a] = dv[q];
+    }
+    if (mom) {
+      HIPCHECK(hipMemcpy(dv.data(), ma[a], n * sizeof(double), hipMemcpyDeviceToHost));
+      for (long q = 0; q < n; ++q) mom[3 * q + a] = dv[q];
+    }
+  }
+  if (weight) HIPCHECK(hipMemcpy(weight, d.w, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (bunch) HIPCHECK(hipMemcpy(bunch, d.bunch, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return QSS_OK;
+}
+
+int qss_driver_set(qss_ctx *c, int64_t n, const int32_t *cell, const double *frac,
+                   const double *mom, const double *weight, const int32_t *bunch) {
+  if (!c || n < 0 || (n > 0 && (!cell || !frac || !mom || !weight || !bunch)))
+    return QSS_E_ARG;
+  DriverState &d = c->drv;
+  if (!d.on) { qss_set_error("driver: not seeded"); return QSS_E_STATE; }
+  const int lim[3] = {c->p.nslices, c->ny, c->nz};
+  for (int64_t q = 0; q < n; ++q) {
+    if (bunch[q] < 0 || bunch[q] >= c->p.nbunch) {
+      qss_set_error("driver_set: bunch index out of range");
+      return QSS_E_ARG;
+    }
+    for (int a = 0; a < 3; ++a)
+      if (cell[3 * q + a] < -1 || cell[3 * q + a] > lim[a]) {
+        qss_set_error("driver_set: cell index outside [-1, N]");
+        return QSS_E_ARG;
+      }
+  }
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  std::vector<int32_t> cv(cell, cell + 3 * n), bv(bunch, bunch + n);
+  std::vector<double> fv(frac, frac + 3 * n), mv(mom, mom + 3 * n), wv(weight, weight + n);
+  return driver_upload(c, (long)n, cv, fv, mv, wv, bv);
+}
+
+int qss_driver_density(qss_ctx *c, int layer, double *rho, double *jx) {
+  if (!c) return QSS_E_ARG;
+  DriverState &d = c->drv;
+  if (!d.on) { qss_set_error("driver: not seeded"); return QSS_E_STATE; }
+  if (layer < 0 || layer > c->p.nslices) { qss_set_error("driver: layer out of range"); return QSS_E_ARG; }
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  std::vector<unsigned long long> v(c->ncell);
+  const double inv = 1.0 / d.scale;
+  unsigned long long *src[2] = {d.acc_rho, d.acc_jx};
+  double *dst[2] = {rho, jx};
+  for (int a = 0; a < 2; ++a) {
+    if (!dst[a]) continue;
+    HIPCHECK(hipMemcpy(v.data(), src[a] + (size_t)layer * c->ncell,
+                       c->ncell * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (long n = 0; n < c->ncell; ++n) dst[a][n] = (double)(long long)v[n] * inv;
+  }
+  return QSS_OK;
+}
+
+int qss_driver_timing(qss_ctx *c, double *ms_clear, double *ms_push) {
+  if (!c) return QSS_E_ARG;
+  if (ms_clear) *ms_clear = c->drv.ms_clear;
+  if (ms_push) *ms_push = c->drv.ms_push;
   return QSS_OK;
 }
 

@@ -119,6 +119,24 @@ struct AdjState {
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
+// evolving driver (qss_driver_*): beam particles in the 3-D cell grid and
+// the fixed-point 3-D deposit the sweeps read
+struct DriverState {
+  bool on = false;
+  qss_driver par{};
+  long np = 0;
+  int32_t *ci = nullptr, *cj = nullptr, *ck = nullptr, *bunch = nullptr;
+  double *x = nullptr, *y = nullptr, *z = nullptr;
+  double *px = nullptr, *py = nullptr, *pz = nullptr, *w = nullptr;
+  unsigned long long *acc_rho = nullptr, *acc_jx = nullptr;   // (nslices+1) x ncell
+  double scale = 1.0;            // 2^e: fixed-point resolution 2^-e
+  int32_t *box = nullptr;        // device: min/max node i, j, k of the last push
+  int32_t *status = nullptr;     // device status bits of the last push
+  int box_lo[3] = {1, 0, 0}, box_hi[3] = {0, -1, -1};   // host copy (empty)
+  hipEvent_t ev[4] = {};         // dispatch-stamped clear and push
+  float ms_clear = 0.f, ms_push = 0.f;
+};
+
 struct qss_ctx {
   qss_params p{};
   std::vector<double> bins;
@@ -196,6 +214,7 @@ struct qss_ctx {
   double *hist[QSS_NFIELD] = {};
 
   AdjState adj;
+  DriverState drv;
 
   // hipGraph replay of two layer steps (the C/P and particle ping-pong
   // pointers return to the same state every two layers); the layer index
@@ -229,6 +248,9 @@ hipError_t launch_fill_f64(double *p, long n, double v, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long *p, long n, unsigned long long v, hipStream_t s);
 hipError_t launch_seed_plasma(qss_ctx *c);
 hipError_t launch_beam_plane(qss_ctx *c, int slice, int isplit);
+hipError_t launch_driver_clear(qss_ctx *c);
+hipError_t launch_driver_push(qss_ctx *c, hipEvent_t ev0, hipEvent_t ev1);
+hipError_t launch_driver_plane(qss_ctx *c, int slice, int isplit);
 hipError_t launch_guess_update(qss_ctx *c, int first_layer);
 hipError_t launch_iterate_update(qss_ctx *c, int final_pass);
 hipError_t launch_push(qss_ctx *c, int full_step);

@@ -31,7 +31,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import QssError, check
-from .params import FIELD_ID, SPECTRAL_ID, Params
+from .params import FIELD_ID, SPECTRAL_ID, Driver, Params
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -191,6 +191,69 @@ class Mesh:
         check(self.L.qss_get_history(self.h, FIELD_ID[name], slice_, _p(out)),
               "qss_get_history")
         return out
+
+    def set_history(self, name: str, slice_: int, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.nz, self.ny)
+        check(self.L.qss_set_history(self.h, FIELD_ID[name], slice_, _p(a)),
+              "qss_set_history")
+
+    # -- evolving driver: Domain::Step (step.cpp:63,105) ---------------------
+    def SeedBeamParticles(self, driver: Driver) -> int:
+        """Mesh::SeedBeamParticles (mesh.cpp:43-64,108-276) of every bunch;
+        from now on the sweeps read the driver's 3-D deposit.  Returns the
+        number of beam macro-particles."""
+        self._driver_c = driver.to_c()
+        n = ctypes.c_int64()
+        check(self.L.qss_driver_seed(self.h, ctypes.byref(self._driver_c), ctypes.byref(n)),
+              "qss_driver_seed")
+        self._driver_n = n.value
+        return n.value
+
+    def MoveBeamParticles(self):
+        """Mesh::MoveBeamParticles (movebeam.cpp:14-660): push through the
+        stored wake and deposit rho_b, Jx_b into the 3-D node arrays."""
+        check(self.L.qss_driver_push(self.h), "qss_driver_push")
+
+    def Step(self) -> np.ndarray:
+        """Domain::Step (step.cpp:23-105): MoveBeamParticles, then
+        MoveAllSplitLayers; returns the sweep's residuals."""
+        res = np.zeros(self.params.nslices * self.params.nx_split)
+        check(self.L.qss_time_step(self.h, _p(res)), "qss_time_step")
+        return res
+
+    def driver_particles(self) -> dict:
+        n = self._driver_n
+        d = {"cell": np.zeros((n, 3), dtype=np.int32), "frac": np.zeros((n, 3)),
+             "mom": np.zeros((n, 3)), "weight": np.zeros(n),
+             "bunch": np.zeros(n, dtype=np.int32)}
+        if n:
+            check(self.L.qss_driver_get(self.h, d["cell"].ctypes.data_as(_i32p),
+                                        _p(d["frac"]), _p(d["mom"]), _p(d["weight"]),
+                                        d["bunch"].ctypes.data_as(_i32p)), "qss_driver_get")
+        return d
+
+    def driver_set(self, cell, frac, mom, weight, bunch):
+        cell = np.ascontiguousarray(cell, dtype=np.int32).reshape(-1, 3)
+        frac = np.ascontiguousarray(frac, dtype=np.float64).reshape(-1, 3)
+        mom = np.ascontiguousarray(mom, dtype=np.float64).reshape(-1, 3)
+        weight = np.ascontiguousarray(weight, dtype=np.float64)
+        bunch = np.ascontiguousarray(bunch, dtype=np.int32)
+        check(self.L.qss_driver_set(self.h, len(weight), cell.ctypes.data_as(_i32p), _p(frac),
+                                    _p(mom), _p(weight), bunch.ctypes.data_as(_i32p)),
+              "qss_driver_set")
+        self._driver_n = len(weight)
+
+    def driver_density(self, layer: int):
+        rho, jx = self._plane(), self._plane()
+        check(self.L.qss_driver_density(self.h, int(layer), _p(rho), _p(jx)),
+              "qss_driver_density")
+        return rho, jx
+
+    def driver_timing(self) -> dict:
+        c, p = ctypes.c_double(), ctypes.c_double()
+        check(self.L.qss_driver_timing(self.h, ctypes.byref(c), ctypes.byref(p)),
+              "qss_driver_timing")
+        return {"ms_clear": c.value, "ms_push": p.value}
 
     def history_ptr(self, name: str) -> int:
         p = ctypes.c_void_p()

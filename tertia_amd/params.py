@@ -181,6 +181,49 @@ class Params:
         return c
 
 
+class QssDriver(ctypes.Structure):
+    _fields_ = [
+        ("ts", ctypes.c_double),
+        ("b_ext", ctypes.c_double * 3),
+        ("syn_min", ctypes.c_double),
+        ("wavelength", ctypes.c_double),
+        ("p0", (ctypes.c_double * 3) * QSS_MAX_BUNCH),
+        ("q2m", ctypes.c_double * QSS_MAX_BUNCH),
+    ]
+
+
+@dataclass
+class Driver:
+    """Push parameters of an evolving driver (``qss_driver``): the time step
+    ``Ts``, external B, the radiation-reaction threshold ``SynMin`` and
+    ``Wavelength`` (&Domain / &Synchrotron keys, domain.cpp:50-74,
+    synchrotron.cpp:13), and per bunch the seeded momentum and q2m
+    (&Specie<i> ``Px0/Py0/Pz0``, q2m = Polarity/(MassAE*1836),
+    elements.cpp:141-185)."""
+    ts: float = 1.0
+    p0: Sequence[Sequence[float]] = ((0.0, 0.0, 0.0),)
+    q2m: Sequence[float] = (-1.0,)
+    b_ext: Sequence[float] = (0.0, 0.0, 0.0)
+    syn_min: float = 1e2
+    wavelength: float = 1e-3
+
+    def to_c(self) -> QssDriver:
+        if len(self.p0) > QSS_MAX_BUNCH or len(self.q2m) > QSS_MAX_BUNCH:
+            raise ValueError("at most %d bunches" % QSS_MAX_BUNCH)
+        c = QssDriver()
+        c.ts = self.ts
+        c.syn_min = self.syn_min
+        c.wavelength = self.wavelength
+        for a in range(3):
+            c.b_ext[a] = self.b_ext[a]
+        for b, mom in enumerate(self.p0):
+            for a in range(3):
+                c.p0[b][a] = mom[a]
+        for b, q in enumerate(self.q2m):
+            c.q2m[b] = q
+        return c
+
+
 def linear_wake_64(**kw) -> Params:
     """12x12 k_p^-1 on 64x64, 200 slices dxi=0.05, driver sr=0.5 sxi=1 nb=0.5 at xi=2."""
     p = Params(ny=64, nz=64, ly=12.0, lz=12.0, hx=0.05, nslices=200,
