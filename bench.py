@@ -31,6 +31,10 @@ rocprofv3 FETCH_SIZE/WRITE_SIZE summary.  roofline_push: the push+deposit
 kernel against the 8 TB/s HBM peak, its algorithmic bytes per launch being
 Np*52 B of particle state + 16 N^2 planes of field reads / deposit writes.
 
+driver: the evolving driver (3-D beam push between sweeps) on the same
+workload: seeding, one time step (push + sweep) and the push kernel's time
+per push with its HBM roofline (see driver_run).
+
 cpu_baseline: the FP64 oracle (FFT-form DCT-IV, OpenMP) on the host cores,
 on the first slices of the same workload, reported in the same unit.
 """
@@ -78,6 +82,11 @@ def parse():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched parameter scan (64 wakes on 4 grid sizes)")
     ap.add_argument("--batch-slices", type=int, default=512)
+    ap.add_argument("--driver-pushes", type=int, default=10,
+                    help="evolving-driver pushes timed after one time step (0: skip)")
+    ap.add_argument("--driver-only", action="store_true",
+                    help="only the evolving-driver pushes, through an empty wake (no sweep: "
+                         "a short run for hardware counters; the push reads the same bytes)")
     ap.add_argument("--batch-wakes", type=int, default=16, help="wakes per grid size")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a kernel of known "
@@ -379,6 +388,49 @@ def batch_run(nslices: int, per_size: int):
     return out
 
 
+def driver_run(pushes: int, sweep: bool = True):
+    """Evolving driver on the 512^2 workload: the driver and witness seeded
+    as 3-D beam macro-particles (4 per cell of their support, px0 = 2000,
+    electrons, Ts = 2, radiation reaction above gamma = 100), one time step
+    (push through the empty wake + a full FP64 sweep that stores Ex..Bz of
+    all 2048 layers), then `pushes` pushes through that wake.  The push
+    kernel is HBM-bound; its algorithmic bytes are the particle state it
+    reads (cell 3 x int32, fractions and momenta 6 x f64, weight f64, bunch
+    int32: 72 B) and writes (60 B) per particle."""
+    from tertia_amd.mesh import Mesh
+    from tertia_amd.params import Driver, two_bunch_512
+    p = two_bunch_512()
+    m = Mesh(p, history=("Ex", "Ey", "Ez", "Bx", "By", "Bz"))
+    try:
+        drv = Driver(ts=2.0, p0=[(2000.0, 0.0, 0.0)] * len(p.bunches),
+                     q2m=[-1.0] * len(p.bunches))
+        t0 = time.perf_counter()
+        n = m.SeedBeamParticles(drv)
+        t_seed = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        if sweep:
+            m.Step()
+        t_step = time.perf_counter() - t0 if sweep else None
+        push, clear = [], []
+        for _ in range(pushes):
+            m.MoveBeamParticles()
+            t = m.driver_timing()
+            push.append(t["ms_push"])
+            clear.append(t["ms_clear"])
+        ms_push = float(np.mean(push))
+        bytes_state = 132.0 * n
+        gbs = bytes_state / (ms_push * 1e-3) / 1e9
+        return {"particles": n, "pushes": pushes, "ms_push": ms_push,
+                "ms_clear": float(np.mean(clear)), "seed_s": t_seed,
+                "time_step_s": t_step, "particle_pushes_per_s": n / (ms_push * 1e-3),
+                "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s",
+                             "frac": gbs / 8000.0, "bytes_per_launch": bytes_state,
+                             "traffic": profiled_traffic("k_driver_push_512x512"),
+                             "kernel": "k_driver_push (3-D beam push + deposit)"}}
+    finally:
+        m.close()
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -389,6 +441,9 @@ def main():
         return 2
     if a.copy_calibration:
         return copy_calibration(a.copy_calibration)
+    if a.driver_only:
+        print(json.dumps({"driver": driver_run(a.driver_pushes or 10, sweep=False)}))
+        return 0
     from tertia_amd.params import two_bunch_512
     if a.warmup + a.steps > two_bunch_512().nslices:
         print("warmup + steps must be <= %d slices" % two_bunch_512().nslices, file=sys.stderr)
@@ -461,6 +516,11 @@ def main():
     if not a.no_batch:
         line["batch"] = batch_run(a.batch_slices, a.batch_wakes)
         print("batch speedup %.2f" % line["batch"]["speedup"], file=sys.stderr, flush=True)
+    if a.driver_pushes > 0:
+        line["driver"] = driver_run(a.driver_pushes)
+        print("driver push %.3f ms (%d particles)" % (line["driver"]["ms_push"],
+                                                    line["driver"]["particles"]),
+              file=sys.stderr, flush=True)
     if not a.no_adjoint:
         line["adjoint"] = adjoint_run(a.fd_bins)
         print("adjoint/forward %.2f" % line["adjoint"]["adjoint_over_forward"], file=sys.stderr,

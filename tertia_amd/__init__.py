@@ -8,7 +8,7 @@ from .params import (Bunch, Driver, Params, linear_wake_64, blowout_256, two_bun
                      parameter_scan, profile_gradient_256, small, FIELDS, FIELD_KINDS,
                      QSS_KIND_C, QSS_KIND_S, QSS_SHAPE_CIC, QSS_SHAPE_NGP_REF, QSS_PREC_FP64,
                      QSS_PREC_MIXED32)
-from .deck import DeckError, load_deck  # noqa: F401
+from .deck import DeckError, load_deck, load_driver  # noqa: F401
 
 
 def Mesh(*args, **kwargs):

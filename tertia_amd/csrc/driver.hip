@@ -19,8 +19,10 @@
 // with integer atomics.  Integer addition is associative, so the 3-D
 // deposit is bitwise independent of the order in which particles arrive
 // (the reference's CUDA twin races on Jy/Jz, assign_currents.cu:92-104).
-// Each work-group also reduces the bounding box of the nodes it touched;
-// the next push clears only that box instead of the whole 3-D array.
+// Lanes of a wave whose particles share a cell add their contributions
+// first, so one atomic per node and run of lanes reaches L2.  Each
+// work-group also reduces the bounding box of the nodes it touched; the
+// next push clears only that box instead of the whole 3-D array.
 
 #include <hip/hip_ext.h>
 
@@ -82,6 +84,10 @@ __global__ __launch_bounds__(NT) void k_driver_push(DriverArgs a) {
   }
   const bool bad = interior && (x < 0 || x > 1 || y < 0 || y > 1 || z < 0 || z > 1);
   if (bad) atomicOr(a.status, 1);
+  // deposit data of this lane (dep: the particle was moved)
+  int it = 0, jt = 0, kt = 0;
+  double xt = 0, yt = 0, zt = 0, djx = 0, drho = 0;
+  bool dep = false;
   if (interior && !bad) {
     const int b = a.bunch[n];
     const double weight = a.w[n];
@@ -147,8 +153,8 @@ __global__ __launch_bounds__(NT) void k_driver_push(DriverArgs a) {
     gamma = 1. / sqrt(1. + px * px + py * py + pz * pz);
     const double Vx = px * gamma;
     const double pol = a.pol[b];
-    const double djx = weight * pol * Vx;
-    const double drho = weight * pol;
+    djx = weight * pol * Vx;
+    drho = weight * pol;
     const double dx = (px * gamma - 1.) * a.ts / a.hx;
     const double dy = py * gamma * a.ts / a.hy;
     const double dz = pz * gamma * a.ts / a.hz;
@@ -157,12 +163,16 @@ __global__ __launch_bounds__(NT) void k_driver_push(DriverArgs a) {
     // error and the particle stays
     const bool far = !(fabs(dx) < 1e9 && fabs(dy) < 1e9 && fabs(dz) < 1e9);
     if (far) atomicOr(a.status, 2);
-    double xt = far ? x : x + dx, yt = far ? y : y + dy, zt = far ? z : z + dz;
+    xt = far ? x : x + dx;
+    yt = far ? y : y + dy;
+    zt = far ? z : z + dz;
     int ij = (int)xt, jj = (int)yt, kj = (int)zt;
     if (xt < 0.) ij--;
     if (yt < 0.) jj--;
     if (zt < 0.) kj--;
-    int it = i + ij, jt = j + jj, kt = k + kj;
+    it = i + ij;
+    jt = j + jj;
+    kt = k + kj;
     it = it < 0 ? -1 : (it > a.L - 1 ? a.L : it);
     jt = jt < 0 ? -1 : (jt > a.ny - 1 ? a.ny : jt);
     kt = kt < 0 ? -1 : (kt > a.nz - 1 ? a.nz : kt);
@@ -184,29 +194,67 @@ __global__ __launch_bounds__(NT) void k_driver_push(DriverArgs a) {
     a.x[n] = xt;
     a.y[n] = yt;
     a.z[n] = zt;
-    const double ax[2] = {1. - xt, xt}, ay[2] = {1. - yt, yt}, az[2] = {1. - zt, zt};
-    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    dep = !far;
+  }
+  // Deposit, with every lane of the wave active.  Lanes whose particles
+  // ended in the same cell form runs (particles are stored cell by cell and
+  // mostly stay); all 8 node slots of a lane share its cell, so one head
+  // mask serves every slot.  Each slot's fixed-point values are summed over
+  // the run by a segmented suffix scan and the run's first lane issues the
+  // atomics: integer sums, so the result does not depend on the grouping.
+  const int lane = threadIdx.x & 63;
+  const long key = dep ? ((long)(it + 1) * (a.ny + 2) + (jt + 1)) * (long)(a.nz + 2) + (kt + 1)
+                       : -1;
+  const long prev = __shfl_up(key, 1);
+  const bool head = lane == 0 || prev != key;
+  const unsigned long long hm = __ballot(head);
+  // cont[t]: lanes l whose lanes l+1 .. l+2^t are in l's run
+  const unsigned long long nonhead = ~hm;
+  const double ax[2] = {1. - xt, xt}, ay[2] = {1. - yt, yt}, az[2] = {1. - zt, zt};
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int da = q >> 2, dc = (q >> 1) & 1, db = q & 1;
-      const int ni = it + da, nj = jt + db, nk = kt + dc;
-      if (far || ni < 0 || ni > a.L || nj < 0 || nj >= a.ny || nk < 0 || nk >= a.nz) continue;
-      const long m = (long)ni * a.ncell + nj + (long)a.ny * nk;
-      // djx*axc*ayc*azc, ... (movebeam.cpp:697-764), then fixed point
-      const long long qj = (long long)rint(djx * ax[da] * ay[db] * az[dc] * a.scale);
-      const long long qr = (long long)rint(drho * ax[da] * ay[db] * az[dc] * a.scale);
-      atomicAdd(a.acc_jx + m, (unsigned long long)qj);
-      atomicAdd(a.acc_rho + m, (unsigned long long)qr);
-      lo[0] = min(lo[0], ni); hi[0] = max(hi[0], ni);
-      lo[1] = min(lo[1], nj); hi[1] = max(hi[1], nj);
-      lo[2] = min(lo[2], nk); hi[2] = max(hi[2], nk);
-    }
-    if (lo[0] <= hi[0]) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        atomicMin(&sbox[2 * d], lo[d]);
-        atomicMax(&sbox[2 * d + 1], hi[d]);
+  for (int q = 0; q < 8; ++q) {
+    const int da = q >> 2, dc = (q >> 1) & 1, db = q & 1;
+    const int ni = it + da, nj = jt + db, nk = kt + dc;
+    const bool valid =
+        dep && ni >= 0 && ni <= a.L && nj >= 0 && nj < a.ny && nk >= 0 && nk < a.nz;
+    // djx*axc*ayc*azc, ... (movebeam.cpp:697-764), then fixed point
+    long long vj = valid ? (long long)rint(djx * ax[da] * ay[db] * az[dc] * a.scale) : 0;
+    long long vr = valid ? (long long)rint(drho * ax[da] * ay[db] * az[dc] * a.scale) : 0;
+    unsigned long long y = nonhead >> 1;   // bit l: lane l+1 continues l's run
+    for (int d = 1; d < 64 && y; d <<= 1) {
+      const long long oj = __shfl_down(vj, d), orr = __shfl_down(vr, d);
+      if ((y >> lane) & 1ull) {   // lanes l+1 .. l+d are in l's run
+        vj += oj;
+        vr += orr;
       }
+      y &= y >> d;                // now: lanes l+1 .. l+2d
+    }
+    if (head && valid) {
+      const long m = (long)ni * a.ncell + nj + (long)a.ny * nk;
+      atomicAdd(a.acc_jx + m, (unsigned long long)vj);
+      atomicAdd(a.acc_rho + m, (unsigned long long)vr);
+    }
+  }
+  // bounding box of the touched nodes: wave min/max, then LDS, then global
+  int lo[3], hi[3];
+  lo[0] = max(it, 0); hi[0] = min(it + 1, a.L);
+  lo[1] = max(jt, 0); hi[1] = min(jt + 1, a.ny - 1);
+  lo[2] = max(kt, 0); hi[2] = min(kt + 1, a.nz - 1);
+  const bool any = dep && lo[0] <= hi[0] && lo[1] <= hi[1] && lo[2] <= hi[2];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (!any) { lo[d] = INT_MAX; hi[d] = INT_MIN; }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      lo[d] = min(lo[d], __shfl_xor(lo[d], o));
+      hi[d] = max(hi[d], __shfl_xor(hi[d], o));
+    }
+  }
+  if (lane == 0 && lo[0] != INT_MAX) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      atomicMin(&sbox[2 * d], lo[d]);
+      atomicMax(&sbox[2 * d + 1], hi[d]);
     }
   }
   __syncthreads();

@@ -1678,9 +1678,25 @@ int qss_driver_seed(qss_ctx *c, const qss_driver *d, int64_t *n_particles) {
   std::vector<double> frac, mom, w;
   for (int ci = 0; ci < L; ++ci) {
     const double xi = (double)(L - ci) * p.hx;
+    bool any_x = false;
+    for (int b = 0; b < p.nbunch; ++b) {
+      const double Rx = std::sqrt(2.0) * p.bunch[b].sigma_xi, dx = xi - p.bunch[b].xi0;
+      any_x = any_x || !(dx * dx / Rx / Rx > 16);
+    }
+    if (!any_x) continue;
     for (int ck = 0; ck < nz; ++ck) {
       const double zc = c->hz * ck - p.lz / 2.0;
-      for (int cj = 0; cj < ny; ++cj) {
+      // candidate cells of this row: the union over bunches of the cells
+      // within 4 R of the bunch centre plus 2 cells (outside it
+      // dy^2/R^2 > 16, so arg > 16); every candidate gets the exact test
+      int jlo = ny, jhi = -1;
+      for (int b = 0; b < p.nbunch; ++b) {
+        const double Ry = std::sqrt(2.0) * p.bunch[b].sigma_r;
+        const double ctr = (p.bunch[b].y0 + p.ly / 2.0) / c->hy, half = 4.0 * Ry / c->hy + 2.0;
+        jlo = std::min(jlo, (int)std::max(0.0, std::floor(ctr - half)));
+        jhi = std::max(jhi, (int)std::min((double)(ny - 1), std::ceil(ctr + half)));
+      }
+      for (int cj = jlo; cj <= jhi; ++cj) {
         const double yc = c->hy * cj - p.ly / 2.0;
         for (int b = 0; b < p.nbunch; ++b) {
           const qss_bunch &B = p.bunch[b];

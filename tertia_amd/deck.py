@@ -50,7 +50,7 @@ from typing import Dict, List, Optional, Union
 
 import numpy as np
 
-from .params import QSS_SHAPE_NGP_REF, Bunch, Params
+from .params import QSS_SHAPE_NGP_REF, Bunch, Driver, Params
 
 
 class DeckError(ValueError):
@@ -230,3 +230,47 @@ def load_deck(deck: Union[str, os.PathLike], shape: int = QSS_SHAPE_NGP_REF,
     return Params(ny=ny, nz=nz, ly=ny * hy, lz=nz * hz, hx=hx, nx_split=nx_split,
                   nslices=nslices, n_iter=niter, dens=dens, ppc_y=2, ppc_z=2, jitter=0.0,
                   shape=shape, fix_subcycle=0, bunches=bunches)
+
+
+def load_driver(deck: Union[str, os.PathLike]) -> Driver:
+    """Push parameters of the evolving driver (Mesh::MoveBeamParticles,
+    movebeam.cpp:14-660) for a reference deck:
+
+    * ``&Domain`` ``Ts`` (default 1), ``BxExternal/ByExternal/BzExternal``
+      (default 0; Cell::sf_DimFields = 1, domain.cpp:204) and ``Wavelength``
+      (default 0.001), domain.cpp:50-74;
+    * ``&Synchrotron`` ``SynMin`` (default 100, synchrotron.cpp:13): the
+      radiation-reaction threshold on gamma;
+    * per beam ``&Specie<i>`` (in the order of :func:`load_deck`'s bunches):
+      ``Px0, Py0, Pz0`` (elements.cpp:59) and q2m = Polarity/(MassAE*1836)
+      (elements.cpp:141-185, the ion species a beam is seeded as by default,
+      ``Type`` = 1, mesh.cpp:246-255).  Other species types are refused: the
+      reference seeds them as Electron objects of species 0."""
+    text = str(deck)
+    if "\n" not in text and os.path.exists(text):
+        with open(text) as f:
+            text = f.read()
+    raw = read_nlist(text, "Domain")
+    if raw is None:
+        raise DeckError("no &Domain list")
+    dom = _List(raw, "Domain")
+    raw = read_nlist(text, "Synchrotron")
+    syn = _List(raw, "Synchrotron").d("SynMin", 1e2) if raw is not None else 1e2
+    p0, q2m = [], []
+    for isp in range(1, dom.i("Nspecies", 1)):
+        nm = "Specie%d" % isp
+        raw = read_nlist(text, nm)
+        if raw is None:
+            raise DeckError("no &%s list" % nm)
+        sp = _List(raw, nm)
+        if sp.i("Beam", 0) == 0:
+            continue
+        if sp.i("Type", 1) != 1:
+            raise DeckError("&%s Type: only beam species seeded as ions (Type = 1) have "
+                            "their own q2m" % nm)
+        p0.append((sp.d("Px0", 0.0), sp.d("Py0", 0.0), sp.d("Pz0", 0.0)))
+        q2m.append(sp.d("Polarity", 1.0) / (sp.d("MassAE", 1.0) * 1836))
+    return Driver(ts=dom.d("Ts", 1.0), p0=p0 or [(0.0, 0.0, 0.0)], q2m=q2m or [0.0],
+                  b_ext=(dom.d("BxExternal", 0.0), dom.d("ByExternal", 0.0),
+                         dom.d("BzExternal", 0.0)),
+                  syn_min=syn, wavelength=dom.d("Wavelength", 1e-3))

@@ -126,3 +126,22 @@ def test_plasma_must_be_uniform_and_2x2():
         load_deck(DECK.replace("P_perCell = 4\n End", "P_perCell = 9\n End"))
     with pytest.raises(DeckError):
         load_deck(DECK.replace(" End = 1e6\n PlateauEnd = 1e6\n", ""))   # plateau ends at 0
+
+
+def test_deck_to_driver():
+    """Push parameters of the evolving driver: &Domain Ts / B*External /
+    Wavelength, &Synchrotron SynMin, q2m = Polarity/(MassAE*1836) of the
+    beam species (elements.cpp:141-185), all read as floats."""
+    from tertia_amd.deck import load_driver
+    d = load_driver(DECK)   # no Ts, no &Synchrotron: reference defaults
+    assert d.ts == 1.0 and d.syn_min == 100.0 and d.wavelength == 1e-3
+    assert list(d.b_ext) == [0.0, 0.0, 0.0]
+    assert d.p0 == [(1000.0, 0.0, 0.0)]
+    assert d.q2m == [f32(-1.0) / (f32(0.000544) * 1836)]
+    deck = DECK.replace(" Nspecies = 2\n", " Nspecies = 2\n Ts = 2.5\n ByExternal = 0.1\n"
+                        " Wavelength = 0.0008\n") + "&Synchrotron\n SynMin = 1e9\n/\n"
+    d = load_driver(deck)
+    assert d.ts == 2.5 and d.syn_min == f32(1e9) and d.wavelength == f32(0.0008)
+    assert list(d.b_ext) == [0.0, f32(0.1), 0.0]
+    with pytest.raises(DeckError):
+        load_driver(DECK.replace(" Beam = 1\n", " Beam = 1\n Type = 0\n"))
