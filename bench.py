@@ -84,6 +84,8 @@ def parse():
     ap.add_argument("--batch-slices", type=int, default=512)
     ap.add_argument("--driver-pushes", type=int, default=10,
                     help="evolving-driver pushes timed after one time step (0: skip)")
+    ap.add_argument("--library-gemm", action="store_true",
+                    help="only time the platform FP64 GEMM on the transform shapes")
     ap.add_argument("--driver-only", action="store_true",
                     help="only the evolving-driver pushes, through an empty wake (no sweep: "
                          "a short run for hardware counters; the push reads the same bytes)")
@@ -388,6 +390,36 @@ def batch_run(nslices: int, per_size: int):
     return out
 
 
+def library_gemm(reps: int = 50):
+    """The platform's FP64 GEMM (torch.matmul -> rocBLAS/hipBLASLt) on the
+    transform's shapes: b fields of 512x512 times a 512x512 matrix, as the
+    y pass (X.T, M = 512 b) and the z pass (T.X batched), timed with events
+    on torch's stream.  A reference point for the transform kernel's
+    fraction of peak, not part of the sweep."""
+    import torch
+    torch.manual_seed(0)
+    dev = "cuda:0"
+    T = torch.randn(512, 512, dtype=torch.float64, device=dev)
+    out = {}
+    for b in (1, 2, 3, 6, 8):
+        X = torch.randn(b * 512, 512, dtype=torch.float64, device=dev)
+        Xb = X.view(b, 512, 512)
+        for name, fn in (("y_pass", lambda: X @ T), ("z_pass", lambda: torch.matmul(T, Xb))):
+            for _ in range(5):
+                fn()
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) / reps * 1e3
+            out["%s_b%d" % (name, b)] = {"us": us, "tflops": 2.0 * b * 512 ** 3 / us / 1e6}
+    return out
+
+
 def driver_run(pushes: int, sweep: bool = True):
     """Evolving driver on the 512^2 workload: the driver and witness seeded
     as 3-D beam macro-particles (4 per cell of their support, px0 = 2000,
@@ -441,6 +473,9 @@ def main():
         return 2
     if a.copy_calibration:
         return copy_calibration(a.copy_calibration)
+    if a.library_gemm:
+        print(json.dumps({"library_gemm": library_gemm()}))
+        return 0
     if a.driver_only:
         print(json.dumps({"driver": driver_run(a.driver_pushes or 10, sweep=False)}))
         return 0

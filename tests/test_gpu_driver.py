@@ -202,3 +202,43 @@ def test_driver_sweep_in_graph_mode_is_bitwise_eager(oracle_mod):
         finally:
             m.close()
     assert np.array_equal(outs[0], outs[1])
+
+
+def test_time_steps_reference_shape_match_oracle(oracle_mod):
+    """The same time steps with the reference's linked NGP plasma shape (the
+    input-deck default)."""
+    from tertia_amd.params import QSS_SHAPE_NGP_REF
+    p = grid(shape=QSS_SHAPE_NGP_REF)
+    m, o = pair(oracle_mod, p)
+    try:
+        drv = Driver(ts=2.0, p0=[(30.0, 0.0, 0.0), (60.0, 0.0, 0.0)], q2m=[-1.0, -1.0])
+        m.SeedBeamParticles(drv)
+        o.driver_seed(drv)
+        for step in range(2):
+            m.Step()
+            o.time_step()
+            ex_g = np.stack([m.history("Ex", s) for s in range(p.nslices)])
+            ex_o = np.stack([o.history("Ex", s) for s in range(p.nslices)])
+            assert np.linalg.norm(ex_g - ex_o) / np.linalg.norm(ex_o) < 1e-9, step
+    finally:
+        m.close(); o.close()
+
+
+def test_driver_in_split_bf16_is_close_to_fp64():
+    """bf16x6 transforms with the driver's deposit as the beam source: the
+    producer of the beam planes writes their split operands too."""
+    from tertia_amd.mesh import Mesh
+    from tertia_amd.params import QSS_PREC_MIXED_BF16X6
+    outs = []
+    for prec in (0, QSS_PREC_MIXED_BF16X6):
+        m = Mesh(grid(precision=prec), history=FIELDS6)
+        try:
+            m.SeedBeamParticles(Driver(ts=2.0, p0=[(30.0, 0, 0), (60.0, 0, 0)],
+                                       q2m=[-1.0, -1.0]))
+            for _ in range(2):
+                m.Step()
+            outs.append(np.stack([m.history("Ex", s) for s in range(m.params.nslices)]))
+        finally:
+            m.close()
+    err = np.linalg.norm(outs[1] - outs[0]) / np.linalg.norm(outs[0])
+    assert err < 1e-5, err
