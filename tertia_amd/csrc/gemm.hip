@@ -280,9 +280,15 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_gemm(GemmArgs a) {
   const int m0 = it.m0, n0 = it.n0;
 
   // A stored k-major so that the 16 lanes of an MFMA operand read
-  // consecutive elements
-  __shared__ T As[2][BK][BM + MT::PAD];
-  __shared__ T Bs[2][BK][BN + MT::PAD];
+  // consecutive elements.  The split-K partial sums reuse the slab storage
+  // after the K loop (less LDS per work-group: more work-groups per CU).
+  constexpr size_t kSlabA = sizeof(T) * 2 * BK * (BM + MT::PAD);
+  constexpr size_t kSlabB = sizeof(T) * 2 * BK * (BN + MT::PAD);
+  constexpr size_t kRed = KS > 1 ? sizeof(ACC) * (KS - 1) * C::NWT * C::TI * C::TJ * 64 : 0;
+  constexpr size_t kSmem = kSlabA + kSlabB > kRed ? kSlabA + kSlabB : kRed;
+  __shared__ __attribute__((aligned(32))) char smem[kSmem];
+  T(*As)[BK][BM + MT::PAD] = reinterpret_cast<T(*)[BK][BM + MT::PAD]>(smem);
+  T(*Bs)[BK][BN + MT::PAD] = reinterpret_cast<T(*)[BK][BN + MT::PAD]>(smem + kSlabA);
 
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int ks = w / C::NWT, wl = w - ks * C::NWT;
@@ -392,8 +398,11 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_gemm(GemmArgs a) {
       compute);
 
   if constexpr (KS > 1) {
-    // partial sums of waves ks = 1..KS-1 through LDS, added in ks order
-    __shared__ ACC red[KS - 1][C::NWT][C::TI * C::TJ][64];
+    // partial sums of waves ks = 1..KS-1 through LDS (the slab storage, once
+    // every wave has finished reading its last slab), added in ks order
+    ACC(*red)[C::NWT][C::TI * C::TJ][64] =
+        reinterpret_cast<ACC(*)[C::NWT][C::TI * C::TJ][64]>(smem);
+    __syncthreads();
     if (ks > 0) {
 #pragma unroll
       for (int i = 0; i < C::TI; ++i)
