@@ -289,6 +289,22 @@ int  qss_copy_history(qss_ctx *c, int field, int first, int count, double *dst_d
 /* Bytes of device memory in use on `device` (all processes) and in total.  */
 int  qss_device_memory(int device, int64_t *used, int64_t *total);
 
+/* ---- layer hand-off between xi chunks ----------------------------------- */
+/* The reference pipelines a sweep over MPI ranks that own consecutive xi
+ * chunks: a rank receives the boundary layer from the rank in front, sweeps
+ * its chunk and sends its last layer on (PackLayer / SendLayer /
+ * ReceiveLayer, para.cpp:94-412; movepart_splitlayer.cpp:125,545).  These
+ * calls are that hand-off's wire format: the state a sweep continues from
+ * (particles with their tile starts, the P-layer planes and the spectral
+ * state) as one flat buffer of qss_layer_state_bytes bytes, device memory
+ * (what a device-to-device send carries) or host memory.  A context that
+ * unpacks the state of a context of the same parameters after layer s-1
+ * and sweeps layers [s, ...) with qss_sweep_range gives the layers of one
+ * unbroken sweep bitwise.  Single-wake contexts.                            */
+int  qss_layer_state_bytes(const qss_ctx *c, size_t *bytes);
+int  qss_pack_layer(qss_ctx *c, void *dst);
+int  qss_unpack_layer(qss_ctx *c, const void *src);
+
 /* Writes history plane `slice` of field f (requires the history_mask bit);
  * the wake an evolving driver is pushed through (qss_driver_push).         */
 int  qss_set_history(qss_ctx *c, int field, int slice, const double *host);

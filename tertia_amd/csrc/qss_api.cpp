@@ -420,9 +420,12 @@ int copy_particles(qss_ctx *c, const PartView &v, int to_ctx) {
                   c->pid[b], c->tile_start[b]};
   void *snap[8] = {v.y, v.z, v.px, v.py, v.pz, v.cell, v.pid, v.tstart};
   const size_t bytes[8] = {n * 8, n * 8, n * 8, n * 8, n * 8, n * 4, n * 8, nt * 4};
+  // hipMemcpyDefault: the snapshot may be device memory (checkpoints, a
+  // layer sent to another chunk) or host memory
   for (int i = 0; i < 8; ++i)
-    HIPCHECK(hipMemcpyAsync(to_ctx ? ctx[i] : snap[i], to_ctx ? snap[i] : ctx[i], bytes[i],
-                            hipMemcpyDeviceToDevice, c->stream));
+    if (ctx[i])   // no tile starts without the tiled particle path
+      HIPCHECK(hipMemcpyAsync(to_ctx ? ctx[i] : snap[i], to_ctx ? snap[i] : ctx[i], bytes[i],
+                              hipMemcpyDefault, c->stream));
   return QSS_OK;
 }
 
@@ -472,11 +475,11 @@ int checkpoint(qss_ctx *c, char *base, int restore) {
   for (int f = 0; f < QSS_NFIELD; ++f, p += pl)
     HIPCHECK(hipMemcpyAsync(restore ? (void *)c->P[f] : (void *)p,
                             restore ? (const void *)p : (const void *)c->P[f], bytes,
-                            hipMemcpyDeviceToDevice, c->stream));
+                            hipMemcpyDefault, c->stream));
   for (int s = 0; s < SP_N; ++s, p += pl)
     HIPCHECK(hipMemcpyAsync(restore ? (void *)c->S[s] : (void *)p,
                             restore ? (const void *)p : (const void *)c->S[s], bytes,
-                            hipMemcpyDeviceToDevice, c->stream));
+                            hipMemcpyDefault, c->stream));
   return QSS_OK;
 }
 
@@ -1633,6 +1636,39 @@ int qss_timing(qss_ctx *c, double *ms_total, double *ms_kernel, int64_t *launche
     if (launches) launches[k] = (int64_t)c->ev_used[k];
   }
   if (gemm_flops) *gemm_flops = c->gemm_flops;
+  return QSS_OK;
+}
+
+// ---- layer hand-off between xi chunks ----------------------------------------
+// The layer state a sweep continues from: the particles (current buffer and
+// tile starts), the P-layer planes and the spectral state; the layout of the
+// adjoint's checkpoints.
+int qss_layer_state_bytes(const qss_ctx *c, size_t *bytes) {
+  if (!c || !bytes) return QSS_E_ARG;
+  if (c->nw != 1) { qss_set_error("layer hand-off: one wake"); return QSS_E_ARG; }
+  *bytes = chk_bytes(c);
+  return QSS_OK;
+}
+
+int qss_pack_layer(qss_ctx *c, void *dst) {
+  if (!c || !dst) return QSS_E_ARG;
+  if (c->nw != 1) { qss_set_error("layer hand-off: one wake"); return QSS_E_ARG; }
+  HIPCHECK(hipSetDevice(c->device));
+  CHECK(checkpoint(c, static_cast<char *>(dst), 0));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return QSS_OK;
+}
+
+int qss_unpack_layer(qss_ctx *c, const void *src) {
+  if (!c || !src) return QSS_E_ARG;
+  if (c->nw != 1) { qss_set_error("layer hand-off: one wake"); return QSS_E_ARG; }
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  forget_splits(c);   // the planes change under any recorded split operands
+  CHECK(checkpoint(c, const_cast<char *>(static_cast<const char *>(src)), 1));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->np = c->np_cap;
+  c->push_dirty = true;
   return QSS_OK;
 }
 

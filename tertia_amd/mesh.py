@@ -197,6 +197,22 @@ class Mesh:
         check(self.L.qss_set_history(self.h, FIELD_ID[name], slice_, _p(a)),
               "qss_set_history")
 
+    # -- layer hand-off between xi chunks (para.cpp:94-412) -------------------
+    def pack_layer(self) -> np.ndarray:
+        """The state a sweep continues from (after the last swept layer) as
+        a flat byte buffer: the wire format of a chunk boundary."""
+        n = ctypes.c_size_t()
+        check(self.L.qss_layer_state_bytes(self.h, ctypes.byref(n)), "qss_layer_state_bytes")
+        buf = np.empty(n.value, dtype=np.uint8)
+        check(self.L.qss_pack_layer(self.h, buf.ctypes.data_as(ctypes.c_void_p)),
+              "qss_pack_layer")
+        return buf
+
+    def unpack_layer(self, buf: np.ndarray):
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        check(self.L.qss_unpack_layer(self.h, buf.ctypes.data_as(ctypes.c_void_p)),
+              "qss_unpack_layer")
+
     # -- evolving driver: Domain::Step (step.cpp:63,105) ---------------------
     def SeedBeamParticles(self, driver: Driver) -> int:
         """Mesh::SeedBeamParticles (mesh.cpp:43-64,108-276) of every bunch;
