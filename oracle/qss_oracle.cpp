@@ -21,6 +21,12 @@
 //   evolving driver       mesh.cpp:43-64 (seeding range), movebeam.cpp:14-660
 //                         (push), 662-770 (deposit), step.cpp:63,105 (step)
 //
+// Parity unpinned: the absolute prefactor of the driver's radiation reaction
+// (movebeam.cpp:448-501: 4/9/137, 1.2398e-4/Wavelength, 3/(4 PI), 5.11e5);
+// its direction (along -p), its absence for dp parallel to p and its
+// scaling with dp_perp^2, 1+p^2, 1/ts and 1/Wavelength are pinned
+// (tests/test_oracle_driver.py), the constant itself only by this text.
+//
 // Build conventions that differ from the reference are listed at the
 // function that makes them.  Compile with -ffp-contract=off: the oracle's
 // push/deposit arithmetic is evaluated in the reference's operation order.
