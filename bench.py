@@ -84,6 +84,8 @@ def parse():
     ap.add_argument("--batch-slices", type=int, default=512)
     ap.add_argument("--driver-pushes", type=int, default=10,
                     help="evolving-driver pushes timed after one time step (0: skip)")
+    ap.add_argument("--particle-split", type=int, default=0, choices=(0, 1, 2, 4),
+                    help="work-groups per tile of the push (0: the autotuner's choice)")
     ap.add_argument("--library-gemm", action="store_true",
                     help="only time the platform FP64 GEMM on the transform shapes")
     ap.add_argument("--driver-only", action="store_true",
@@ -153,6 +155,11 @@ def timed_run(mode, a):
     out = {}
     try:
         choice = m.autotune(3)
+        if a.particle_split:
+            # work-groups per push tile fixed instead of tuned (the tuned
+            # choice can flip between runs where two splits time alike)
+            m.particle_split = a.particle_split
+            choice["particle_split"] = m.particle_split
         npart = m.n_particles
 
         def window(events, graph):
