@@ -84,6 +84,10 @@ def parse():
     ap.add_argument("--batch-slices", type=int, default=512)
     ap.add_argument("--driver-pushes", type=int, default=10,
                     help="evolving-driver pushes timed after one time step (0: skip)")
+    ap.add_argument("--gemm-tiles", default="", metavar="Y/Z",
+                    help="FP64 transform tile variants per batch size 1..8 for the y and z "
+                         "passes, e.g. 11,12,11,13,11,14,0,0/11,12,12,13,11,15,0,2 "
+                         "(default: the autotuner's choice)")
     ap.add_argument("--particle-split", type=int, default=0, choices=(0, 1, 2, 4),
                     help="work-groups per tile of the push (0: the autotuner's choice)")
     ap.add_argument("--library-gemm", action="store_true",
@@ -155,6 +159,12 @@ def timed_run(mode, a):
     out = {}
     try:
         choice = m.autotune(3)
+        if a.gemm_tiles and mode == "fp64":
+            # a fixed tile table (y-pass and z-pass variant per batch size 1..8)
+            # instead of the tuned one, e.g. to repeat a run's kernels exactly
+            y, z = ([int(v) for v in part.split(",")] for part in a.gemm_tiles.split("/"))
+            m.set_gemm_choice({"y_pass": y, "z_pass": z})
+            choice.update(m.gemm_choice())
         if a.particle_split:
             # work-groups per push tile fixed instead of tuned (the tuned
             # choice can flip between runs where two splits time alike)
