@@ -92,3 +92,42 @@ def test_params_layout_matches_c():
         assert vals[2 + i] == getattr(QssParams, f).offset, f
     for i, f in enumerate(bfields):
         assert vals[2 + len(fields) + i] == getattr(QssBunch, f).offset, f
+
+
+def test_driver_layout_matches_c():
+    from tertia_amd.params import QssDriver
+    fields = [f for f, _ in QssDriver._fields_]
+    src = ["#include <stdio.h>", "#include <stddef.h>", '#include "qss.h"', "int main(void){",
+           'printf("%zu\\n", sizeof(qss_driver));']
+    for f in fields:
+        src.append('printf("%%zu\\n", offsetof(qss_driver, %s));' % f)
+    src.append("return 0;}")
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "l.c")
+        exe = os.path.join(d, "l")
+        open(c, "w").write("\n".join(src))
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    vals = [int(v) for v in out]
+    assert vals[0] == ctypes.sizeof(QssDriver)
+    for i, f in enumerate(fields):
+        assert vals[1 + i] == getattr(QssDriver, f).offset, f
+
+
+def test_driver_entry_points_refuse_before_seeding_without_a_gpu():
+    """The driver and hand-off entry points validate their arguments first."""
+    from tertia_amd import _lib
+    L = _lib.load()
+    assert L.qss_driver_push(None) == -1
+    assert L.qss_time_step(None, None) == -1
+    assert L.qss_driver_seed(None, None, None) == -1
+    n = ctypes.c_size_t()
+    assert L.qss_layer_state_bytes(None, ctypes.byref(n)) == -1
+    assert L.qss_pack_layer(None, None) == -1
+    assert L.qss_unpack_layer(None, None) == -1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run(["python", os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
