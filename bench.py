@@ -35,7 +35,8 @@ driver: the evolving driver (3-D beam push between sweeps) on the same
 workload: seeding, one time step (push + sweep) and the push kernel's time
 per push with its HBM roofline (see driver_run).
 
-cpu_baseline: the FP64 oracle (FFT-form DCT-IV, OpenMP) on the host cores,
+cpu_baseline: the FP64 oracle (FFT-form DCT-IV, OpenMP) on the host cores
+(cpu_baseline_1core: the same on one core, a shorter sample),
 on the first slices of the same workload, reported in the same unit.
 """
 from __future__ import annotations
@@ -580,6 +581,9 @@ def main():
     if a.cpu_slices > 0:
         nthr = a.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(two_bunch_512(), a.cpu_slices, nthr)
+        # one core, as the reference runs one rank (FFTW with at most 2 threads)
+        line["cpu_baseline_1core"] = cpu_baseline(two_bunch_512(), max(1, a.cpu_slices // 12),
+                                                  1)
     print(json.dumps(line))
     return 0
 
