@@ -139,6 +139,7 @@ void k_push_tile(TileArgs a0) {
   const double q2m = -1.0, djx0 = -1.0, djy0 = -1.0, djz0 = -1.0, drho0 = -1.0;
   const double weight = a.weight;
   const bool cic = a.shape == QSS_SHAPE_CIC;
+  int status = 0;   // ST_* bits met by this thread, reported once at the end
 
   // 1. field window: nodes (j0..j0+T, k0..k0+T), periodic
   for (int i = t; i < NF; i += NT) {
@@ -169,7 +170,7 @@ void k_push_tile(TileArgs a0) {
       cell_jk(c, ny, a.ny_magic, j, k);
       yp = a.y[slot];
       zp = a.z[slot];
-      if (!(yp >= 0 && yp <= 1 && zp >= 0 && zp <= 1)) atomicOr(&a.counters[1], ST_PARTICLE);
+      if (!(yp >= 0 && yp <= 1 && zp >= 0 && zp <= 1)) status |= (ST_PARTICLE);
       px = a.px[slot]; py = a.py[slot]; pz = a.pz[slot];
       const R rpx = (R)px, rpy = (R)py, rpz = (R)pz;
       const R gammap = sqrt(R(1) + rpx * rpx + rpy * rpy + rpz * rpz);
@@ -178,7 +179,7 @@ void k_push_tile(TileArgs a0) {
       // gather from the staged window: local node (j - j0, k - k0)
       int lj = j - j0, lk = k - k0;
       if (lj < 0 || lj >= T || lk < 0 || lk >= T) {   // not in this tile: never expected
-        atomicOr(&a.counters[1], ST_FAR);
+        status |= (ST_FAR);
         lj = min(max(lj, 0), T - 1);
         lk = min(max(lk, 0), T - 1);
       }
@@ -246,7 +247,7 @@ void k_push_tile(TileArgs a0) {
       }
       if (partdx < R(1)) partdx = step = R(1);
       if (!(step < (R)(kMaxSub - 1))) {
-        atomicOr(&a.counters[1], ST_SUBSTEPS);
+        status |= (ST_SUBSTEPS);
         partdx = step = R(1);
         dy = R(0);
         dz = R(0);
@@ -266,7 +267,7 @@ void k_push_tile(TileArgs a0) {
         ytmp += (double)(dy * part_step) + j;
         ztmp += (double)(dz * part_step) + k;
         if (!(ytmp > -2.0 * ny && ytmp < 3.0 * ny && ztmp > -2.0 * nz && ztmp < 3.0 * nz)) {
-          atomicOr(&a.counters[1], ST_PARTICLE);
+          status |= (ST_PARTICLE);
           ytmp = 0.0;
           ztmp = 0.0;
         }
@@ -294,7 +295,7 @@ void k_push_tile(TileArgs a0) {
           ev_w[t] = make_double4((double)(djx * part_step), (double)(djy * part_step),
                                  (double)(djz * part_step), (double)(drho * part_step));
         } else {
-          atomicOr(&a.counters[1], ST_FAR);
+          status |= (ST_FAR);
         }
       }
       // segmented reduction of this round's events
@@ -360,14 +361,16 @@ void k_push_tile(TileArgs a0) {
         d2 += d2 < 0 ? ntk_ : 0;
         dtj = d1 == 0 ? 0 : (d1 == ntj_ - 1 && ntj_ > 2 ? -1 : 1);
         dtk = d2 == 0 ? 0 : (d2 == ntk_ - 1 && ntk_ > 2 ? -1 : 1);
-        if (d1 > 1 && d1 < ntj_ - 1) atomicOr(&a.counters[1], ST_FAR);
-        if (d2 > 1 && d2 < ntk_ - 1) atomicOr(&a.counters[1], ST_FAR);
+        if (d1 > 1 && d1 < ntj_ - 1) status |= (ST_FAR);
+        if (d2 > 1 && d2 < ntk_ - 1) status |= (ST_FAR);
       }
       const int dir = (dtj + 1) + 3 * (dtk + 1);
       // rank among this tile's particles with the same direction, slot order
       const int lane = t & 63, wv = t >> 6;
       const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
       int myrank_w = 0;
+      // kept rolled: nine live ballot masks would spill scalar registers
+#pragma unroll 1
       for (int d = 0; d < 9; ++d) {
         const unsigned long long m = __ballot(active && dir == d);
         if (lane == 0) wave_cnt[wv][d] = __popcll(m);
@@ -395,6 +398,7 @@ void k_push_tile(TileArgs a0) {
       __syncthreads();
     }
   }
+  if (status) atomicOr(&a.counters[1], status);
   // 4. window to HBM
   double *wout = a.win + (long)blockIdx.x * NWIN * 4;
   reinterpret_cast<double4 *>(wout)[n0] = make_double4(acc0[0], acc0[1], acc0[2], acc0[3]);
