@@ -62,7 +62,6 @@ struct TileArgs {
   // full step staging (indexed by the old slot)
   double *sy, *sz, *spx, *spy, *spz;
   int32_t *scell, *sdest;  // sdest = direction | rank << 4
-  int64_t *spid;
   int32_t *tcount;         // ntiles x nsplit x 9
   int32_t *counters;
   long npw;                // particle slots per wake (batched contexts)
@@ -96,7 +95,7 @@ __host__ __device__ inline TileArgs wake_view(const TileArgs &a0, int w) {
   if (a.Ff) a.Ff += w * kPushComp * plane;
   a.win += w * nt * a0.nsplit * NWIN * 4;
   a.sy += pw; a.sz += pw; a.spx += pw; a.spy += pw; a.spz += pw;
-  a.scell += pw; a.sdest += pw; a.spid += pw;
+  a.scell += pw; a.sdest += pw;
   a.tcount += w * nt * a0.nsplit * 9;
   return a;
 }
@@ -120,6 +119,7 @@ template <typename R, typename FT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
 void k_push_tile(TileArgs a0) {
   const TileArgs a = wake_view(a0, blockIdx.y);   // wake of a batch
+  const long pw = (long)blockIdx.y * a0.npw;
   __shared__ FT Fw[NF * 6];
   __shared__ int keys_sh[NT];
   __shared__ int sorted[NT];
@@ -161,17 +161,18 @@ void k_push_tile(TileArgs a0) {
   const int np = a.tile_start[tile + 1] - base;
   for (int c0 = part * NT; c0 < np; c0 += a.nsplit * NT) {
     const int slot = base + c0 + t;
+    const long gs = pw + slot;   // the wake's particle arrays, unmoved pointers
     const bool active = c0 + t < np;
     int j = 0, k = 0;
     double yp = 0, zp = 0, px = 0, py = 0, pz = 0;
     R djx = 0, djy = 0, djz = 0, drho = 0, dy = 0, dz = 0, partdx = 0, step = 1;
     if (active) {
-      const int32_t c = a.cell[slot];
+      const int32_t c = a0.cell[gs];
       cell_jk(c, ny, a.ny_magic, j, k);
-      yp = a.y[slot];
-      zp = a.z[slot];
+      yp = a0.y[gs];
+      zp = a0.z[gs];
       if (!(yp >= 0 && yp <= 1 && zp >= 0 && zp <= 1)) status |= (ST_PARTICLE);
-      px = a.px[slot]; py = a.py[slot]; pz = a.pz[slot];
+      px = a0.px[gs]; py = a0.py[gs]; pz = a0.pz[gs];
       const R rpx = (R)px, rpy = (R)py, rpz = (R)pz;
       const R gammap = sqrt(R(1) + rpx * rpx + rpy * rpy + rpz * rpz);
       R Vx = rpx / gammap;
@@ -380,14 +381,13 @@ void k_push_tile(TileArgs a0) {
       int off = dir_total[dir];
       for (int q = 0; q < wv; ++q) off += wave_cnt[q][dir];
       if (active) {
-        a.sy[slot] = ytmp;
-        a.sz[slot] = ztmp;
-        a.spx[slot] = px;
-        a.spy[slot] = py;
-        a.spz[slot] = pz;
-        a.scell[slot] = jj + ny * kk;
-        a.spid[slot] = a.pid[slot];
-        a.sdest[slot] = dir | (part << 4) | ((off + myrank_w) << 8);
+        a0.sy[gs] = ytmp;
+        a0.sz[gs] = ztmp;
+        a0.spx[gs] = px;
+        a0.spy[gs] = py;
+        a0.spz[gs] = pz;
+        a0.scell[gs] = jj + ny * kk;
+        a0.sdest[gs] = dir | (part << 4) | ((off + myrank_w) << 8);
       }
       __syncthreads();
       if (t < 9) {
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(NT) void k_permute(TileArgs a0, const int32_t *goff
   py[o] = a.spy[s];
   pz[o] = a.spz[s];
   cell[o] = c;
-  pid[o] = a.spid[s];
+  pid[o] = a.pid[s];   // ids are not staged: the old storage still holds them
 }
 
 // seeding in tile-major slot order: tile, local cell (j fastest), seed
@@ -606,7 +606,6 @@ TileArgs make_args(qss_ctx *c, int full) {
   a.sy = c->ev_y; a.sz = c->ev_z; a.spx = c->ev_p; a.spy = c->ev_p + npall;
   a.spz = c->ev_p + 2 * npall;
   a.scell = c->ev_cell; a.sdest = c->ev_rank;
-  a.spid = reinterpret_cast<int64_t *>(c->ov_key);   // ov_cap >= np_cap
   a.tcount = c->tcount;
   a.counters = c->d_counters;
   a.npw = c->np_cap;
