@@ -410,29 +410,31 @@ void k_push_tile(TileArgs a0) {
 // sum of the windows covering each node, fixed (dtk, dtj) order
 // xs0..xs2 (split-bf16 modes, one wake): Jx, Jy, Jz also written as the split
 // operands of the forward transform that follows
+// grid: (node blocks of one plane, wakes); 32-bit node index (ny, nz <= 2^14)
 __global__ __launch_bounds__(NT) void k_combine(const double *win, int ntj, int ntk, int ny,
-                                                int nz, int nw, int nsplit, double *J0,
-                                                double *J1, double *J2, double *J3,
-                                                uint16_t *xs0, uint16_t *xs1, uint16_t *xs2,
-                                                int npc) {
-  const long n = (long)blockIdx.x * NT + threadIdx.x;   // wake-major node index
-  const long plane = (long)ny * nz;
-  if (n >= plane * nw) return;
-  win += (n / plane) * ntj * ntk * nsplit * NWIN * 4;   // the wake's windows
-  const long n1 = n % plane;
-  const int nj = (int)(n1 % ny), nk = (int)(n1 / ny);
+                                                int nz, int nsplit, uint64_t ny_magic,
+                                                double *J0, double *J1, double *J2,
+                                                double *J3, uint16_t *xs0, uint16_t *xs1,
+                                                uint16_t *xs2, int npc) {
+  const int plane = ny * nz;
+  const int n1 = blockIdx.x * NT + threadIdx.x;   // node of the wake's plane
+  if (n1 >= plane) return;
+  win += (long)blockIdx.y * ntj * ntk * nsplit * NWIN * 4;   // the wake's windows
+  const long n = (long)blockIdx.y * plane + n1;               // wake-major node index
+  int nj, nk;
+  cell_jk(n1, ny, ny_magic, nj, nk);
   const int tj = nj / T, tk = nk / T;
   double s[4] = {0, 0, 0, 0};
   for (int dtk = -1; dtk <= 1; ++dtk) {
-    const int utk = tk + dtk;                 // unwrapped tile row
+    const int utk = tk + dtk;                 // unwrapped tile row, in [-1, ntk]
     const int lk = nk - (utk * T - H);
     if (lk < 0 || lk >= W) continue;
-    const int wtk = wrapi(utk, ntk);
+    const int wtk = utk < 0 ? utk + ntk : (utk >= ntk ? utk - ntk : utk);
     for (int dtj = -1; dtj <= 1; ++dtj) {
       const int utj = tj + dtj;
       const int lj = nj - (utj * T - H);
       if (lj < 0 || lj >= W) continue;
-      const int wtj = wrapi(utj, ntj);
+      const int wtj = utj < 0 ? utj + ntj : (utj >= ntj ? utj - ntj : utj);
       for (int q = 0; q < nsplit; ++q) {
         const long wi = (long)(wtk * ntj + wtj) * nsplit + q;
         const double4 v = reinterpret_cast<const double4 *>(win + wi * NWIN * 4)[lk * W + lj];
@@ -632,9 +634,10 @@ hipError_t launch_combine(qss_ctx *c, double *const *out) {
   if (!out) out = J;
   if (split)
     for (int b = 0; b < 3; ++b) c->xs_src[b] = J[b];
-  hipLaunchKernelGGL(k_combine, grid1(c->ncell * c->nw), dim3(NT), 0, c->stream, c->win,
-                     c->ntj, c->ntk, c->ny, c->nz, c->nw, c->nsplit, out[0], out[1], out[2],
-                     out[3], c->xs[0], c->xs[1], c->xs[2], split ? c->npc : 0);
+  const uint64_t ny_magic = ((1ull << 44) + (uint64_t)c->ny - 1) / (uint64_t)c->ny;
+  hipLaunchKernelGGL(k_combine, dim3(grid1(c->ncell).x, c->nw), dim3(NT), 0, c->stream,
+                     c->win, c->ntj, c->ntk, c->ny, c->nz, c->nsplit, ny_magic, out[0], out[1],
+                     out[2], out[3], c->xs[0], c->xs[1], c->xs[2], split ? c->npc : 0);
   return hipGetLastError();
 }
 
