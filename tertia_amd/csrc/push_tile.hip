@@ -407,41 +407,70 @@ void k_push_tile(TileArgs a0) {
   if (a.full && t < 9) a.tcount[blockIdx.x * 9 + t] = dir_total[t];
 }
 
-// sum of the windows covering each node, fixed (dtk, dtj) order
+// sum of the windows covering each node, fixed (dtk, dtj, part) order.  Per
+// axis a node lies in its own tile's window (dt = 0) and in at most one
+// neighbour's (dt = -1 within W - T - H nodes of the tile's low edge, +1 within
+// H of its high edge; the two ranges are disjoint), so at most 4 x NS windows
+// cover it: all their loads are issued before the first add.  An absent window
+// reads the own tile's entry and adds +0.0 instead, which leaves the sum bitwise
+// unchanged (the sum starts at +0.0, so it is never -0.0).
 // xs0..xs2 (split-bf16 modes, one wake): Jx, Jy, Jz also written as the split
-// operands of the forward transform that follows
+// operands of the forward transform that follows.
 // grid: (node blocks of one plane, wakes); 32-bit node index (ny, nz <= 2^14)
+static_assert(W - T - H <= T - H, "at most two tiles per axis cover a node");
+template <int NS>
 __global__ __launch_bounds__(NT) void k_combine(const double *win, int ntj, int ntk, int ny,
-                                                int nz, int nsplit, uint64_t ny_magic,
-                                                double *J0, double *J1, double *J2,
-                                                double *J3, uint16_t *xs0, uint16_t *xs1,
-                                                uint16_t *xs2, int npc) {
+                                                int nz, uint64_t ny_magic, double *J0,
+                                                double *J1, double *J2, double *J3,
+                                                uint16_t *xs0, uint16_t *xs1, uint16_t *xs2,
+                                                int npc) {
   const int plane = ny * nz;
   const int n1 = blockIdx.x * NT + threadIdx.x;   // node of the wake's plane
   if (n1 >= plane) return;
-  win += (long)blockIdx.y * ntj * ntk * nsplit * NWIN * 4;   // the wake's windows
-  const long n = (long)blockIdx.y * plane + n1;               // wake-major node index
+  const double4 *w4 = reinterpret_cast<const double4 *>(win) +
+                      (long)blockIdx.y * ntj * ntk * NS * NWIN;   // the wake's windows
+  const long n = (long)blockIdx.y * plane + n1;                   // wake-major node index
   int nj, nk;
   cell_jk(n1, ny, ny_magic, nj, nk);
   const int tj = nj / T, tk = nk / T;
-  double s[4] = {0, 0, 0, 0};
-  for (int dtk = -1; dtk <= 1; ++dtk) {
-    const int utk = tk + dtk;                 // unwrapped tile row, in [-1, ntk]
-    const int lk = nk - (utk * T - H);
-    if (lk < 0 || lk >= W) continue;
+  const int rj = nj - tj * T, rk = nk - tk * T;   // position in the tile, [0, T)
+  // covering tiles per axis, in increasing order: (d0, d0 + 1), the second
+  // present when the node is near either edge
+  const int dk0 = rk < W - T - H ? -1 : 0, dj0 = rj < W - T - H ? -1 : 0;
+  const bool hk = rk < W - T - H || rk >= T - H, hj = rj < W - T - H || rj >= T - H;
+  long idx[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int utk = tk + dk0 + a;                 // unwrapped tile row, in [-1, ntk]
     const int wtk = utk < 0 ? utk + ntk : (utk >= ntk ? utk - ntk : utk);
-    for (int dtj = -1; dtj <= 1; ++dtj) {
-      const int utj = tj + dtj;
-      const int lj = nj - (utj * T - H);
-      if (lj < 0 || lj >= W) continue;
+    const int lk = nk - (utk * T - H);
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int utj = tj + dj0 + b;
       const int wtj = utj < 0 ? utj + ntj : (utj >= ntj ? utj - ntj : utj);
-      for (int q = 0; q < nsplit; ++q) {
-        const long wi = (long)(wtk * ntj + wtj) * nsplit + q;
-        const double4 v = reinterpret_cast<const double4 *>(win + wi * NWIN * 4)[lk * W + lj];
-        s[0] += v.x; s[1] += v.y; s[2] += v.z; s[3] += v.w;
-      }
+      const int lj = nj - (utj * T - H);
+      idx[a][b] = (long)(wtk * ntj + wtj) * NS * NWIN + lk * W + lj;
     }
   }
+  const bool ok[2][2] = {{true, hj}, {hk, hk && hj}};
+  double4 v[2][2][NS];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < NS; ++q)
+        v[a][b][q] = w4[(ok[a][b] ? idx[a][b] : idx[0][0]) + (long)q * NWIN];
+  double s[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        const double4 u = ok[a][b] ? v[a][b][q] : make_double4(0.0, 0.0, 0.0, 0.0);
+        s[0] += u.x; s[1] += u.y; s[2] += u.z; s[3] += u.w;
+      }
   J0[n] = s[0]; J1[n] = s[1]; J2[n] = s[2]; J3[n] = s[3];
   if (npc) {
     qss_split::store(xs0, plane, n, s[0], npc);
@@ -635,9 +664,11 @@ hipError_t launch_combine(qss_ctx *c, double *const *out) {
   if (split)
     for (int b = 0; b < 3; ++b) c->xs_src[b] = J[b];
   const uint64_t ny_magic = ((1ull << 44) + (uint64_t)c->ny - 1) / (uint64_t)c->ny;
-  hipLaunchKernelGGL(k_combine, dim3(grid1(c->ncell).x, c->nw), dim3(NT), 0, c->stream,
-                     c->win, c->ntj, c->ntk, c->ny, c->nz, c->nsplit, ny_magic, out[0], out[1],
-                     out[2], out[3], c->xs[0], c->xs[1], c->xs[2], split ? c->npc : 0);
+  auto fn = c->nsplit == 1 ? k_combine<1> : c->nsplit == 2 ? k_combine<2> : k_combine<4>;
+  if (c->nsplit != 1 && c->nsplit != 2 && c->nsplit != 4) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(fn, dim3(grid1(c->ncell).x, c->nw), dim3(NT), 0, c->stream, c->win,
+                     c->ntj, c->ntk, c->ny, c->nz, ny_magic, out[0], out[1], out[2], out[3],
+                     c->xs[0], c->xs[1], c->xs[2], split ? c->npc : 0);
   return hipGetLastError();
 }
 
