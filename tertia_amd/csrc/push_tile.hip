@@ -303,8 +303,22 @@ void k_push_tile(TileArgs a0) {
       for (int i = t; i < NWIN; i += NT) { cstart[i] = 0; cend[i] = 0; }
       const int sk = sort256(key, keys_sh);
       sorted[t] = sk;
+      // the event at sorted position t, moved there in place: gathered before
+      // the barrier (the sort's barriers made every event visible), stored after
+      // it, so the node sums below read events without the sorted[] indirection
+      double gy = 0, gz = 0;
+      double4 gw = make_double4(0.0, 0.0, 0.0, 0.0);
+      if (sk != kInvalid) {
+        const int e = sk & 255;
+        gy = ev_y[e];
+        gz = ev_z[e];
+        gw = ev_w[e];
+      }
       __syncthreads();
       if (sk != kInvalid) {
+        ev_y[t] = gy;
+        ev_z[t] = gz;
+        ev_w[t] = gw;
         const int cellw = sk >> 8;
         const int prev = t > 0 ? sorted[t - 1] : kInvalid;
         const int next = t + 1 < NT ? sorted[t + 1] : kInvalid;
@@ -327,7 +341,7 @@ void k_push_tile(TileArgs a0) {
           const int cw = ck * W + cj;
           const int e1 = cend[cw];
           for (int i = cstart[cw]; i < e1; ++i) {
-            const int e = sorted[i] & 255;
+            const int e = i;   // events are stored in sorted order
             const double4 wv = ev_w[e];
             const double wq[4] = {wv.x, wv.y, wv.z, wv.w};
             if (cic) {
