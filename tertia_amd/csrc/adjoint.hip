@@ -143,8 +143,22 @@ __global__ __launch_bounds__(NT) void k_push_tile_adj(AdjTileArgs a) {
     for (int i = t; i < T * T; i += NT) { cstart[i] = 0; cend[i] = 0; }
     const int sk = qss_tile::sort256(key, keys_sh);
     sorted[t] = sk;
+    // the event at sorted position t, moved there in place: gathered before the
+    // barrier (the sort's barriers made every event visible), stored after it
+    double gf[6], gw[4];
+    if (sk != kInvalid) {
+      const int e = sk & 255;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) gf[q] = ev_f[e * 6 + q];
+#pragma unroll
+      for (int cn = 0; cn < 4; ++cn) gw[cn] = ev_w[e * 4 + cn];
+    }
     __syncthreads();
     if (sk != kInvalid) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) ev_f[t * 6 + q] = gf[q];
+#pragma unroll
+      for (int cn = 0; cn < 4; ++cn) ev_w[t * 4 + cn] = gw[cn];
       const int cl = sk >> 8;
       const int prev = t > 0 ? sorted[t - 1] : kInvalid;
       const int next = t + 1 < NT ? sorted[t + 1] : kInvalid;
@@ -165,7 +179,7 @@ __global__ __launch_bounds__(NT) void k_push_tile_adj(AdjTileArgs a) {
         const int cl = ck * T + cj;
         const int e1 = cend[cl];
         for (int i = cstart[cl]; i < e1; ++i) {
-          const int e = sorted[i] & 255;
+          const int e = i;   // events are stored in sorted order
           const double wv = ev_w[e * 4 + cn];
 #pragma unroll
           for (int q = 0; q < 6; ++q) accn[q] += wv * ev_f[e * 6 + q];
