@@ -429,6 +429,19 @@ int copy_particles(qss_ctx *c, const PartView &v, int to_ctx) {
   return QSS_OK;
 }
 
+// the layer-start particles into a device tape slot, all arrays in one launch
+int tape_particles(qss_ctx *c, const PartView &v) {
+  const int b = c->cur;
+  const long n = c->np_cap, nt = (long)c->ntj * c->ntk + 1;
+  CopyBufs cb{{c->py_[b], c->pz_[b], c->ppx[b], c->ppy[b], c->ppz[b], c->pcell[b], c->pid[b],
+               c->tile_start[b]},
+              {v.y, v.z, v.px, v.py, v.pz, v.cell, v.pid, v.tstart},
+              {n * 8, n * 8, n * 8, n * 8, n * 8, n * 4, n * 8, nt * 4}};
+  if (!cb.src[7]) cb.nbytes[7] = 0;   // no tile starts without the tiled particle path
+  HIPCHECK(launch_copy_bufs(cb, c->stream));
+  return QSS_OK;
+}
+
 // tape slot of one layer: particles at the layer start + the push field of
 // each of the n_iter+1 passes
 size_t tape_bytes(const qss_ctx *c) {
@@ -442,7 +455,7 @@ double *tape_field(const qss_ctx *c, char *slot, int pass) {
 // tape != null: record the layer's start particles and every pass's push
 // field (the reverse sweep replays the pushes from them)
 int do_layer_step(qss_ctx *c, int slice, int isplit, double *res_dev, char *tape = nullptr) {
-  if (tape) CHECK(copy_particles(c, part_view(c, tape), 0));
+  if (tape) CHECK(tape_particles(c, part_view(c, tape)));
   auto rec = [&](int pass) -> int {
     if (tape)
       HIPCHECK(hipMemcpyAsync(tape_field(c, tape, pass), c->F,

@@ -245,6 +245,15 @@ enum { KC_PUSH = 0, KC_GEMM = 1, KC_SPEC = 2, KC_BEAM = 3, KC_SORT = 4, KC_REDUC
 // launchers (tertia_amd/csrc/*.hip); all enqueue on c->stream
 hipError_t launch_fill_i32(int32_t *p, long n, int32_t v, hipStream_t s);
 hipError_t launch_fill_f64(double *p, long n, double v, hipStream_t s);
+// device-to-device copies of up to kCopyBufs buffers in one launch (unused
+// slots: nbytes = 0); 16-byte aligned buffers, sizes multiples of 4 bytes
+constexpr int kCopyBufs = 8;
+struct CopyBufs {
+  const void *src[kCopyBufs];
+  void *dst[kCopyBufs];
+  long nbytes[kCopyBufs];
+};
+hipError_t launch_copy_bufs(const CopyBufs &a, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long *p, long n, unsigned long long v, hipStream_t s);
 hipError_t launch_seed_plasma(qss_ctx *c);
 hipError_t launch_beam_plane(qss_ctx *c, int slice, int isplit);

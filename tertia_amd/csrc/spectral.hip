@@ -32,6 +32,23 @@ __global__ void k_fill_u64(unsigned long long *p, long n, unsigned long long v) 
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+// up to kCopyBufs device-to-device copies in one launch: blockIdx.y picks the
+// buffer, a thread moves 16 bytes (4-byte steps for the last partial 16 bytes).
+// Buffers are 16-byte aligned and their sizes multiples of 4 bytes.
+__global__ void k_copy_bufs(CopyBufs a) {
+  const int b = blockIdx.y;
+  const long nb = a.nbytes[b];
+  const long i = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 16;
+  if (i >= nb) return;
+  const char *src = static_cast<const char *>(a.src[b]);
+  char *dst = static_cast<char *>(a.dst[b]);
+  if (i + 16 <= nb) {
+    *reinterpret_cast<uint4 *>(dst + i) = *reinterpret_cast<const uint4 *>(src + i);
+  } else {
+    for (long k = i; k < nb; k += 4)
+      *reinterpret_cast<uint32_t *>(dst + k) = *reinterpret_cast<const uint32_t *>(src + k);
+  }
+}
 
 struct SpecArgs {
   double *S[SP_N];
@@ -358,6 +375,17 @@ hipError_t launch_fill_i32(int32_t *p, long n, int32_t v, hipStream_t s) {
 hipError_t launch_fill_f64(double *p, long n, double v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill_f64, grid1(n), dim3(NT), 0, s, p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_copy_bufs(const CopyBufs &a, hipStream_t s) {
+  long mx = 0;
+  for (int b = 0; b < kCopyBufs; ++b) {
+    if (a.nbytes[b] % 4 || ((uintptr_t)a.src[b] | (uintptr_t)a.dst[b]) % 16)
+      return hipErrorInvalidValue;
+    mx = a.nbytes[b] > mx ? a.nbytes[b] : mx;
+  }
+  if (mx == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_bufs, dim3(grid1((mx + 15) / 16).x, kCopyBufs), dim3(NT), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_fill_u64(unsigned long long *p, long n, unsigned long long v,
