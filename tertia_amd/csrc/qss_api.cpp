@@ -591,7 +591,9 @@ int adj_deposit_transform(qss_ctx *c) {
   double *out[3] = {s.jbar, s.jbar + c->ncell, s.jbar + 2 * c->ncell};
   CHECK(transform_batch(c, 3, kz, ky, in, out, 1.0, nullptr, nullptr, nullptr));
   Timed t(c, KC_SPEC);
-  for (int q = 0; q < 3; ++q) HIPCHECK(launch_fill_f64(s.A[SP_JX + q], c->ncell, 0.0, c->stream));
+  HIPCHECK(launch_zero_bufs(ZeroBufs{{s.A[SP_JX], s.A[SP_JY], s.A[SP_JZ], nullptr},
+                                      {c->ncell, c->ncell, c->ncell, 0}},
+                             c->stream));
   return QSS_OK;
 }
 
@@ -643,8 +645,9 @@ int adj_layer_step(qss_ctx *c, int slice, char *slot, const double *exbar, int &
     CHECK(transform_batch(c, 2, kz, ky, in, out, 1.0, nullptr, nullptr, nullptr));
     Timed t(c, KC_BEAM);
     HIPCHECK(launch_beam_adj(c, slice, s.bbar, s.bbar + nc, s.qbeam));
-    HIPCHECK(launch_fill_f64(s.A[SP_RHOB], nc, 0.0, c->stream));
-    HIPCHECK(launch_fill_f64(s.A[SP_JXB], nc, 0.0, c->stream));
+    HIPCHECK(launch_zero_bufs(ZeroBufs{{s.A[SP_RHOB], s.A[SP_JXB], nullptr, nullptr},
+                                        {nc, nc, 0, 0}},
+                               c->stream));
   }
   // this layer's beam P spectrum is the previous layer's beam spectrum
   std::swap(s.A[SP_RHOB], s.A[SP_RHOBP]);

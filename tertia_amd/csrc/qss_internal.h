@@ -254,6 +254,13 @@ struct CopyBufs {
   long nbytes[kCopyBufs];
 };
 hipError_t launch_copy_bufs(const CopyBufs &a, hipStream_t s);
+// zero fills of up to kFillBufs double buffers in one launch (unused slots: n = 0)
+constexpr int kFillBufs = 4;
+struct ZeroBufs {
+  double *p[kFillBufs];
+  long n[kFillBufs];
+};
+hipError_t launch_zero_bufs(const ZeroBufs &a, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long *p, long n, unsigned long long v, hipStream_t s);
 hipError_t launch_seed_plasma(qss_ctx *c);
 hipError_t launch_beam_plane(qss_ctx *c, int slice, int isplit);

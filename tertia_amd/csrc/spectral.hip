@@ -32,6 +32,13 @@ __global__ void k_fill_u64(unsigned long long *p, long n, unsigned long long v) 
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+// zero fills of up to kFillBufs double buffers in one launch (blockIdx.y
+// picks the buffer)
+__global__ void k_zero_bufs(ZeroBufs a) {
+  const int b = blockIdx.y;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.n[b]) a.p[b][i] = 0.0;
+}
 // up to kCopyBufs device-to-device copies in one launch: blockIdx.y picks the
 // buffer, a thread moves 16 bytes (4-byte steps for the last partial 16 bytes).
 // Buffers are 16-byte aligned and their sizes multiples of 4 bytes.
@@ -375,6 +382,13 @@ hipError_t launch_fill_i32(int32_t *p, long n, int32_t v, hipStream_t s) {
 hipError_t launch_fill_f64(double *p, long n, double v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill_f64, grid1(n), dim3(NT), 0, s, p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_zero_bufs(const ZeroBufs &a, hipStream_t s) {
+  long mx = 0;
+  for (int b = 0; b < kFillBufs; ++b) mx = a.n[b] > mx ? a.n[b] : mx;
+  if (mx == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_bufs, dim3(grid1(mx).x, kFillBufs), dim3(NT), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_copy_bufs(const CopyBufs &a, hipStream_t s) {
