@@ -5,12 +5,25 @@ and the ``cpu_baseline`` leg of ``bench.py``.  It is the checker, never the
 thing measured or shipped; the product path (``tertia_amd``) never imports it.
 
 Every function of ``qss_oracle.cpp`` restates a reference routine; the
-file:line it follows is given at each function there.  Pinning: the
-transforms are pinned by scipy type-4 fixtures, brute force and the
-involution T.T = 2N.I; the spectral updates by single-mode hand-computed
-answers and linearity; the push by Boris-rotation invariants and closed
-forms; the sweep by the psi/Ex consistency and linear-wake convergence
-(see tests/test_oracle_*.py).
+file:line it follows is given at each function there.  Pinning (each by a
+check that does not restate the oracle's formula):
+
+* transforms: scipy type-4 fixtures, brute force, the involution T.T = 2N.I
+  (tests/test_oracle_transforms.py);
+* Guess and Iterate, every output field: the continuous equations they
+  discretise (continuity, Gauss, the cold-fluid current, Ampere, the psi
+  Poisson equation), checked in real space with scipy-built derivatives on
+  random planes, plus the predictor's plasma rotation
+  (tests/test_oracle_fields.py, tests/test_oracle_spectral.py);
+* push: Boris invariants, closed-form kicks, exact bilinear gather, deposit
+  conservation (tests/test_oracle_push.py);
+* frozen driver planes: charge, centroid and width of the analytic
+  bi-Gaussian and of the binned profile (tests/test_oracle_beam.py);
+* the whole sweep: the psi/Ex identity (tests/test_oracle_sweep.py) and
+  second-order convergence to 3-D linear wake theory
+  (tests/test_oracle_linear_wake.py);
+* tests/test_oracle_mutants.py plants 23 plausible mistakes in copies of the
+  source and checks that these pins catch each of them.
 """
 from __future__ import annotations
 
@@ -39,52 +52,63 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
+def _configure(L):
+    """Declare the argument types of every exported oracle function on L."""
+    L.oracle_create.restype = ctypes.c_void_p
+    L.oracle_create.argtypes = [ctypes.POINTER(QssParams), ctypes.c_int,
+                                ctypes.c_int, ctypes.c_uint32]
+    L.oracle_destroy.argtypes = [ctypes.c_void_p]
+    for name in ("oracle_reset", "oracle_n_particles"):
+        getattr(L, name).argtypes = [ctypes.c_void_p]
+    L.oracle_layer_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+    L.oracle_sweep_range.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+    L.oracle_sweep.argtypes = [ctypes.c_void_p, _dp]
+    L.oracle_beam_layer.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.oracle_beam_plane.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+    L.oracle_guess.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.oracle_push_deposit.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.oracle_iterate.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    L.oracle_copy_c_to_p.argtypes = [ctypes.c_void_p]
+    L.oracle_get_layer_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+    L.oracle_set_layer_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+    L.oracle_get_spectral.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    L.oracle_set_spectral.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    L.oracle_get_history.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+    L.oracle_get_particles.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp,
+                                       _i64p, _i32p, _i32p, _dp, _dp]
+    L.oracle_set_particles.argtypes = [ctypes.c_void_p, ctypes.c_long, _i32p, _i32p,
+                                       _dp, _dp, _dp, _dp, _dp, _i64p]
+    L.oracle_set_history.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+    L.oracle_driver_seed.argtypes = [ctypes.c_void_p, ctypes.POINTER(QssDriver), _i64p]
+    L.oracle_driver_push.argtypes = [ctypes.c_void_p]
+    L.oracle_time_step.argtypes = [ctypes.c_void_p, _dp]
+    L.oracle_driver_count.restype = ctypes.c_int64
+    L.oracle_driver_count.argtypes = [ctypes.c_void_p]
+    L.oracle_driver_get.argtypes = [ctypes.c_void_p, _i32p, _dp, _dp, _dp, _i32p]
+    L.oracle_driver_set.argtypes = [ctypes.c_void_p, ctypes.c_int64, _i32p, _dp, _dp,
+                                    _dp, _i32p]
+    L.oracle_driver_density.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+    L.oracle_transform1d.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int]
+    L.oracle_transform2d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, _dp, _dp, ctypes.c_double,
+                                     ctypes.c_int, ctypes.c_int]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        L.oracle_create.restype = ctypes.c_void_p
-        L.oracle_create.argtypes = [ctypes.POINTER(QssParams), ctypes.c_int,
-                                    ctypes.c_int, ctypes.c_uint32]
-        L.oracle_destroy.argtypes = [ctypes.c_void_p]
-        for name in ("oracle_reset", "oracle_n_particles"):
-            getattr(L, name).argtypes = [ctypes.c_void_p]
-        L.oracle_layer_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
-        L.oracle_sweep_range.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
-        L.oracle_sweep.argtypes = [ctypes.c_void_p, _dp]
-        L.oracle_beam_layer.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
-        L.oracle_beam_plane.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
-        L.oracle_guess.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        L.oracle_push_deposit.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        L.oracle_iterate.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
-        L.oracle_copy_c_to_p.argtypes = [ctypes.c_void_p]
-        L.oracle_get_layer_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
-        L.oracle_set_layer_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
-        L.oracle_get_spectral.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
-        L.oracle_set_spectral.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
-        L.oracle_get_history.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
-        L.oracle_get_particles.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp,
-                                           _i64p, _i32p, _i32p, _dp, _dp]
-        L.oracle_set_particles.argtypes = [ctypes.c_void_p, ctypes.c_long, _i32p, _i32p,
-                                           _dp, _dp, _dp, _dp, _dp, _i64p]
-        L.oracle_set_history.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
-        L.oracle_driver_seed.argtypes = [ctypes.c_void_p, ctypes.POINTER(QssDriver), _i64p]
-        L.oracle_driver_push.argtypes = [ctypes.c_void_p]
-        L.oracle_time_step.argtypes = [ctypes.c_void_p, _dp]
-        L.oracle_driver_count.restype = ctypes.c_int64
-        L.oracle_driver_count.argtypes = [ctypes.c_void_p]
-        L.oracle_driver_get.argtypes = [ctypes.c_void_p, _i32p, _dp, _dp, _dp, _i32p]
-        L.oracle_driver_set.argtypes = [ctypes.c_void_p, ctypes.c_int64, _i32p, _dp, _dp,
-                                        _dp, _i32p]
-        L.oracle_driver_density.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
-        L.oracle_transform1d.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int]
-        L.oracle_transform2d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                         ctypes.c_int, _dp, _dp, ctypes.c_double,
-                                         ctypes.c_int, ctypes.c_int]
-        _lib = L
+        _lib = _configure(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+def load_library(path: str):
+    """The oracle's functions from another build of its source: the tests that
+    plant a mistake in a copy of qss_oracle.cpp load the copy with this and
+    check that the pins fail on it."""
+    return _configure(ctypes.CDLL(path))
 
 
 def _p(a):
@@ -113,13 +137,14 @@ class Oracle:
 
     def __init__(self, params: Params, use_fft: bool = False, nthreads: int = 1,
                  history=("Ex", "Ey", "Ez", "Bx", "By", "Bz", "Jx", "Jy", "Jz",
-                          "Rho", "RhoBeam", "JxBeam", "Psi")):
+                          "Rho", "RhoBeam", "JxBeam", "Psi"), library=None):
+        self._L = library if library is not None else lib()
         self.params = params
         self._c = params.to_c()
         mask = 0
         for f in history or ():
             mask |= 1 << FIELD_ID[f]
-        self.h = lib().oracle_create(ctypes.byref(self._c), int(use_fft), int(nthreads),
+        self.h = self._L.oracle_create(ctypes.byref(self._c), int(use_fft), int(nthreads),
                                      mask)
         if not self.h:
             raise ValueError("oracle_create rejected the parameters")
@@ -127,7 +152,7 @@ class Oracle:
 
     def close(self):
         if self.h:
-            lib().oracle_destroy(self.h)
+            self._L.oracle_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -138,76 +163,76 @@ class Oracle:
 
     @property
     def n_particles(self):
-        return lib().oracle_n_particles(self.h)
+        return self._L.oracle_n_particles(self.h)
 
     def _plane(self):
         return np.zeros((self.nz, self.ny), dtype=np.float64)
 
     # -- sweep ---------------------------------------------------------
     def reset(self):
-        lib().oracle_reset(self.h)
+        self._L.oracle_reset(self.h)
 
     def sweep(self):
         n = self.params.nslices * self.params.nx_split
         res = np.zeros(n)
-        st = lib().oracle_sweep(self.h, _p(res))
+        st = self._L.oracle_sweep(self.h, _p(res))
         return st, res
 
     def sweep_range(self, first, count):
         res = np.zeros(count * self.params.nx_split)
-        st = lib().oracle_sweep_range(self.h, first, count, _p(res))
+        st = self._L.oracle_sweep_range(self.h, first, count, _p(res))
         return st, res
 
     def layer_step(self, slice_, isplit=0):
         r = np.zeros(1)
-        st = lib().oracle_layer_step(self.h, slice_, isplit, _p(r))
+        st = self._L.oracle_layer_step(self.h, slice_, isplit, _p(r))
         return st, float(r[0])
 
     # -- pieces ----------------------------------------------------------
     def beam_layer(self, slice_, isplit=0):
-        return lib().oracle_beam_layer(self.h, slice_, isplit)
+        return self._L.oracle_beam_layer(self.h, slice_, isplit)
 
     def beam_plane(self, layer):
         r, j = self._plane(), self._plane()
-        lib().oracle_beam_plane(self.h, layer, _p(r), _p(j))
+        self._L.oracle_beam_plane(self.h, layer, _p(r), _p(j))
         return r, j
 
     def guess(self, first_layer):
-        return lib().oracle_guess(self.h, int(first_layer))
+        return self._L.oracle_guess(self.h, int(first_layer))
 
     def push_deposit(self, full_step):
-        return lib().oracle_push_deposit(self.h, int(full_step))
+        return self._L.oracle_push_deposit(self.h, int(full_step))
 
     def iterate(self, final_pass):
         r = np.zeros(1)
-        lib().oracle_iterate(self.h, int(final_pass), _p(r))
+        self._L.oracle_iterate(self.h, int(final_pass), _p(r))
         return float(r[0])
 
     def copy_c_to_p(self):
-        return lib().oracle_copy_c_to_p(self.h)
+        return self._L.oracle_copy_c_to_p(self.h)
 
     # -- data ------------------------------------------------------------
     def get_field(self, name, which=0):
         out = self._plane()
-        lib().oracle_get_layer_field(self.h, which, FIELD_ID[name], _p(out))
+        self._L.oracle_get_layer_field(self.h, which, FIELD_ID[name], _p(out))
         return out
 
     def set_field(self, name, arr, which=0):
         a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.nz, self.ny)
-        lib().oracle_set_layer_field(self.h, which, FIELD_ID[name], _p(a))
+        self._L.oracle_set_layer_field(self.h, which, FIELD_ID[name], _p(a))
 
     def get_spectral(self, name):
         out = self._plane()
-        lib().oracle_get_spectral(self.h, SPECTRAL_ID[name], _p(out))
+        self._L.oracle_get_spectral(self.h, SPECTRAL_ID[name], _p(out))
         return out
 
     def set_spectral(self, name, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.nz, self.ny)
-        lib().oracle_set_spectral(self.h, SPECTRAL_ID[name], _p(a))
+        self._L.oracle_set_spectral(self.h, SPECTRAL_ID[name], _p(a))
 
     def history(self, name, slice_):
         out = self._plane()
-        st = lib().oracle_get_history(self.h, FIELD_ID[name], slice_, _p(out))
+        st = self._L.oracle_get_history(self.h, FIELD_ID[name], slice_, _p(out))
         if st != 0:
             raise ValueError("no history for %s" % name)
         return out
@@ -218,7 +243,7 @@ class Oracle:
         d["id"] = np.zeros(n, dtype=np.int64)
         d["cj"] = np.zeros(n, dtype=np.int32)
         d["ck"] = np.zeros(n, dtype=np.int32)
-        lib().oracle_get_particles(self.h, _p(d["y"]), _p(d["z"]), _p(d["px"]),
+        self._L.oracle_get_particles(self.h, _p(d["y"]), _p(d["z"]), _p(d["px"]),
                                    _p(d["py"]), _p(d["pz"]),
                                    d["id"].ctypes.data_as(_i64p),
                                    d["cj"].ctypes.data_as(_i32p),
@@ -230,42 +255,42 @@ class Oracle:
         arrs = [np.ascontiguousarray(a, dtype=np.int32) for a in (cj, ck)]
         f = [np.ascontiguousarray(a, dtype=np.float64) for a in (fy, fz, px, py, pz)]
         pid = np.ascontiguousarray(pid, dtype=np.int64)
-        lib().oracle_set_particles(self.h, len(pid), arrs[0].ctypes.data_as(_i32p),
+        self._L.oracle_set_particles(self.h, len(pid), arrs[0].ctypes.data_as(_i32p),
                                    arrs[1].ctypes.data_as(_i32p), _p(f[0]), _p(f[1]),
                                    _p(f[2]), _p(f[3]), _p(f[4]),
                                    pid.ctypes.data_as(_i64p))
 
     def set_history(self, name, slice_, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.nz, self.ny)
-        if lib().oracle_set_history(self.h, FIELD_ID[name], slice_, _p(a)) != 0:
+        if self._L.oracle_set_history(self.h, FIELD_ID[name], slice_, _p(a)) != 0:
             raise ValueError("no history for %s" % name)
 
     # -- evolving driver -----------------------------------------------
     def driver_seed(self, driver: Driver) -> int:
         self._drv = driver.to_c()
         n = ctypes.c_int64(0)
-        if lib().oracle_driver_seed(self.h, ctypes.byref(self._drv), ctypes.byref(n)) != 0:
+        if self._L.oracle_driver_seed(self.h, ctypes.byref(self._drv), ctypes.byref(n)) != 0:
             raise ValueError("the driver needs the history of Ex..Bz")
         return n.value
 
     def driver_push(self):
-        st = lib().oracle_driver_push(self.h)
+        st = self._L.oracle_driver_push(self.h)
         if st != 0:
             raise ValueError("oracle_driver_push: status %d" % st)
 
     def time_step(self):
         res = np.zeros(self.params.nslices * self.params.nx_split)
-        st = lib().oracle_time_step(self.h, _p(res))
+        st = self._L.oracle_time_step(self.h, _p(res))
         if st != 0:
             raise ValueError("oracle_time_step: status %d" % st)
         return res
 
     def driver_particles(self):
-        n = lib().oracle_driver_count(self.h)
+        n = self._L.oracle_driver_count(self.h)
         d = {"cell": np.zeros((n, 3), dtype=np.int32), "frac": np.zeros((n, 3)),
              "mom": np.zeros((n, 3)), "weight": np.zeros(n),
              "bunch": np.zeros(n, dtype=np.int32)}
-        lib().oracle_driver_get(self.h, d["cell"].ctypes.data_as(_i32p), _p(d["frac"]),
+        self._L.oracle_driver_get(self.h, d["cell"].ctypes.data_as(_i32p), _p(d["frac"]),
                                 _p(d["mom"]), _p(d["weight"]),
                                 d["bunch"].ctypes.data_as(_i32p))
         return d
@@ -276,13 +301,13 @@ class Oracle:
         mom = np.ascontiguousarray(mom, dtype=np.float64).reshape(-1, 3)
         weight = np.ascontiguousarray(weight, dtype=np.float64)
         bunch = np.ascontiguousarray(bunch, dtype=np.int32)
-        if lib().oracle_driver_set(self.h, len(weight), cell.ctypes.data_as(_i32p),
+        if self._L.oracle_driver_set(self.h, len(weight), cell.ctypes.data_as(_i32p),
                                    _p(frac), _p(mom), _p(weight),
                                    bunch.ctypes.data_as(_i32p)) != 0:
             raise ValueError("seed the driver first")
 
     def driver_density(self, layer):
         rho, jx = self._plane(), self._plane()
-        if lib().oracle_driver_density(self.h, int(layer), _p(rho), _p(jx)) != 0:
+        if self._L.oracle_driver_density(self.h, int(layer), _p(rho), _p(jx)) != 0:
             raise ValueError("no driver deposit for layer %d" % layer)
         return rho, jx

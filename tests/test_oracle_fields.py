@@ -1,0 +1,147 @@
+"""Pins every field the oracle's predictor (Guess,
+movefieldshydrolinlayersplit.cpp:807-901) and corrector (Iterate, :1373-1478)
+produce, by the continuous equations they discretise, checked in real space
+with derivatives taken by scipy's type-4 DCT/DST (tests/halfint.py), not by
+the oracle's own transforms or wavenumbers.
+
+The inputs are random planes (every mode excited) on a non-square grid with
+ny != nz and Ly != Lz, a plasma density dens != 1 and NxSplit = 2, so a
+swapped y/z derivative, a transposed kind, a flipped sign, a dropped source
+term or a wrong factor each break at least one identity at O(1).  Notation:
+P-layer fields carry a p suffix, rho_b/rho_bp are the beam densities of the C
+layer and of the previous layer, bars are trapezoid means (C + P)/2,
+h = Hx/NxSplit:
+
+Predictor (Guess)
+  G1  (rho - rho_p)/h            = (dens - lap) Ex_bar          (continuity + Gauss)
+  G2  (dens - lap)(Ex - Ex_p)/h  = -dens (rho_bar + rho_b_bar)  (plasma response)
+  G3  curl_x(E_perp + E_perp_p)  = 0                            (E_perp_bar is a gradient)
+  G4  (dens - lap) div E_perp_bar = -lap (rho_bar + rho_b_bar)  (screened Gauss law)
+  G5  J - J_p = h dens E_bar, for x, y and z, cell by cell      (cold-fluid current)
+  G6  -lap Bx = dJz/dy - dJy/dz                                 (Ampere, x)
+  G7  -lap By =  d(Jx + Jxb)/dz + dens Ez                       (Ampere, y)
+  G8  -lap Bz = -d(Jx + Jxb)/dy - dens Ey                       (Ampere, z)
+Corrector (Iterate), with the fresh deposit J:
+  I1  rho - rho_p = Jx - Jx_p - h/2 div(J_perp + J_perp_p)      (continuity)
+  I2  lap Ex = div J_perp
+  I3  curl_x(E_perp + E_perp_p) = 0
+  I4  (dens - lap) div E_perp_bar = -lap (rho_prev + rho_p + rho_b + rho_bp)/2,
+      rho_prev = the density before this corrector pass (the reference reads
+      fft_of_Rho before overwriting it, :1423-1424)
+  I5-I7  G6-G8 with the deposit J and the new E
+  I8  -lap psi = rho - Jx                                       (psi: this build's plane)
+  I9  the returned residual = max over x, y, z of max|E_new - E_old| / max(1e-5, max|E_new|)
+      (:1584-1621)
+"""
+import numpy as np
+import pytest
+import scipy.fft as sf
+
+from tertia_amd.params import FIELD_KINDS as K, small
+from tests.halfint import Grid
+
+PFIELDS = ("Rho", "Ex", "Ey", "Ez", "Jx", "Jy", "Jz", "Bx", "By", "Bz")
+EYK, EZK, CC, SS = K["Ey"], K["Ez"], K["Rho"], K["Bx"]
+
+
+def params(dens=1.7):
+    return small(ny=24, nz=20, nslices=3, ly=9.0, lz=6.5, hx=0.11, nx_split=2, dens=dens)
+
+
+def fwd2(x):
+    """The reference's forward 2-D REDFT11 (FFTW convention: 2 sum per axis)."""
+    return sf.dct(sf.dct(x, type=4, axis=1), type=4, axis=0)
+
+
+def close(lhs, rhs, what, tol=1e-11):
+    scale = max(np.abs(lhs).max(), np.abs(rhs).max(), 1e-300)
+    err = np.abs(lhs - rhs).max() / scale
+    assert err <= tol, "%s: %.3e" % (what, err)
+
+
+def run_guess(make, p, seed=0):
+    rng = np.random.default_rng(seed)
+    o = make(p)
+    shape = (p.nz, p.ny)
+    P = {f: rng.standard_normal(shape) for f in PFIELDS}
+    for f, v in P.items():
+        o.set_field(f, v, which=1)
+    B = {k: rng.standard_normal(shape) for k in ("rb", "jxb", "rbp", "jxbp")}
+    o.set_field("RhoBeam", B["rb"])
+    o.set_field("JxBeam", B["jxb"])
+    o.set_spectral("RhoBeam", fwd2(B["rbp"]))    # the previous layer's beam spectra
+    o.set_spectral("JxBeam", fwd2(B["jxbp"]))
+    o.guess(False)
+    C = {f: o.get_field(f) for f in PFIELDS}
+    return o, P, B, C
+
+
+def check_guess(make, p=None):
+    p = p or params()
+    h = p.hx / p.nx_split
+    d = p.dens
+    g = Grid(p.ny, p.nz, p.ly, p.lz)
+    o, P, B, C = run_guess(make, p)
+    o.close()
+    exb = 0.5 * (C["Ex"] + P["Ex"])
+    rtot = 0.5 * (C["Rho"] + P["Rho"] + B["rb"] + B["rbp"])
+    close((C["Rho"] - P["Rho"]) / h, d * exb - g.lap(exb, CC), "G1")
+    close(d * (C["Ex"] - P["Ex"]) / h - g.lap((C["Ex"] - P["Ex"]) / h, CC),
+          -d * rtot, "G2")
+    cu, _ = g.curl_x(C["Ey"] + P["Ey"], C["Ez"] + P["Ez"])
+    scale = np.abs(g.dy(C["Ez"], EZK)[0]).max()
+    assert np.abs(cu).max() <= 1e-11 * scale, "G3: %.3e" % (np.abs(cu).max() / scale)
+    dv, _ = g.div(0.5 * (C["Ey"] + P["Ey"]), 0.5 * (C["Ez"] + P["Ez"]))
+    close(d * dv - g.lap(dv, CC), -g.lap(rtot, CC), "G4")
+    for c in ("x", "y", "z"):
+        close(C["J" + c] - P["J" + c], h * d * 0.5 * (C["E" + c] + P["E" + c]), "G5 J" + c)
+    close(-g.lap(C["Bx"], SS), g.curl_x(C["Jy"], C["Jz"])[0], "G6")
+    jxt = C["Jx"] + B["jxb"]
+    close(-g.lap(C["By"], EZK), g.dz(jxt, CC)[0] + d * C["Ez"], "G7")
+    close(-g.lap(C["Bz"], EYK), -g.dy(jxt, CC)[0] - d * C["Ey"], "G8")
+
+
+def check_iterate(make, p=None):
+    p = p or params()
+    h = p.hx / p.nx_split
+    d = p.dens
+    g = Grid(p.ny, p.nz, p.ly, p.lz)
+    o, P, B, C0 = run_guess(make, p, seed=1)
+    rng = np.random.default_rng(11)
+    J = {c: rng.standard_normal((p.nz, p.ny)) for c in ("Jx", "Jy", "Jz")}
+    for c, v in J.items():
+        o.set_field(c, v)
+    res = o.iterate(True)
+    C = {f: o.get_field(f) for f in ("Rho", "Ex", "Ey", "Ez", "Bx", "By", "Bz", "Psi")}
+    o.close()
+    dJ, _ = g.div(J["Jy"] + P["Jy"], J["Jz"] + P["Jz"])
+    close(C["Rho"] - P["Rho"], J["Jx"] - P["Jx"] - 0.5 * h * dJ, "I1")
+    close(g.lap(C["Ex"], CC), g.div(J["Jy"], J["Jz"])[0], "I2")
+    cu, _ = g.curl_x(C["Ey"] + P["Ey"], C["Ez"] + P["Ez"])
+    scale = np.abs(g.dy(C["Ez"], EZK)[0]).max()
+    assert np.abs(cu).max() <= 1e-11 * scale, "I3: %.3e" % (np.abs(cu).max() / scale)
+    dv, _ = g.div(0.5 * (C["Ey"] + P["Ey"]), 0.5 * (C["Ez"] + P["Ez"]))
+    rtot = 0.5 * (C0["Rho"] + P["Rho"] + B["rb"] + B["rbp"])
+    close(d * dv - g.lap(dv, CC), -g.lap(rtot, CC), "I4")
+    close(-g.lap(C["Bx"], SS), g.curl_x(J["Jy"], J["Jz"])[0], "I5")
+    jxt = J["Jx"] + B["jxb"]
+    close(-g.lap(C["By"], EZK), g.dz(jxt, CC)[0] + d * C["Ez"], "I6")
+    close(-g.lap(C["Bz"], EYK), -g.dy(jxt, CC)[0] - d * C["Ey"], "I7")
+    close(-g.lap(C["Psi"], CC), C["Rho"] - J["Jx"], "I8")
+    want = max(np.abs(C[e] - C0[e]).max() / max(1e-5, np.abs(C[e]).max())
+               for e in ("Ex", "Ey", "Ez"))
+    assert abs(res - want) <= 1e-12 * want, "I9: %r vs %r" % (res, want)
+
+
+def test_guess_fields(oracle_mod):
+    check_guess(oracle_mod.Oracle)
+
+
+def test_iterate_fields(oracle_mod):
+    check_iterate(oracle_mod.Oracle)
+
+
+@pytest.mark.parametrize("dens", [1.0, 0.35])
+def test_fields_other_density(oracle_mod, dens):
+    check_guess(oracle_mod.Oracle, params(dens))
+    check_iterate(oracle_mod.Oracle, params(dens))

@@ -11,6 +11,9 @@
   partition of unity, sub-steps split the deposit into fractions summing
   to one);
 * a uniform plasma at rest deposits Dens = -n0 on every node;
+* gather: fields bilinear in (j, k), distinct per field, axis and layer, are
+  interpolated exactly (CIC) or taken at the start node (NGP), as the mean of
+  the P and C layers;
 * sub-cycling: with the quirk fixed the final position is the straight-line
   one; the reference's literal restart (shadowed j_jump/k_jump) only matters
   when |dy| > 1 cell.
@@ -176,3 +179,61 @@ def test_subcycle_fixed_is_straight_line(oracle_mod):
     assert abs(res[0]["y"][0] - np.mod(3.3 + dy, p.ny)) > 1e-3
     for fix in (0, 1):   # the sub-steps split the deposit into fractions summing to one
         assert abs(res[fix]["rho"] - 0.25 * -1.0 / (1 - vx)) < 1e-14
+
+
+def check_gather(make, shape):
+    """Fields bilinear in the node indices (j, k), different in every field
+    and in the P and C layers, are gathered exactly: CIC returns the bilinear
+    value at the particle's position (j + y, k + z) (movepart_layer_CIC.cpp:
+    166-203), NGP the start node's value (movepart_splitlayer.cpp:1101-1217),
+    each the mean of the P and C layers.  Particles at rest see E through the
+    kick dp = q2m hx E / (1 - Vx) (q2m = -1); moving particles see B through
+    the Boris identity p+ - p- = (p+ + p-) x t.  A swapped corner, axis or
+    layer breaks one of them."""
+    p = small(ny=16, nz=12, nslices=2, shape=shape, hx=0.05)
+    rng = np.random.default_rng(7)
+    coef = rng.uniform(-1.0, 1.0, (2, 6, 4))     # layer (C, P), field, bilinear terms
+    K, J = np.meshgrid(np.arange(p.nz), np.arange(p.ny), indexing="ij")
+
+    def field(which, f, y, z):
+        a = coef[which, f]
+        return a[0] + a[1] * y + a[2] * z + a[3] * y * z
+
+    n = 300
+    cj = rng.integers(0, p.ny - 1, n).astype(np.int32)     # keep j+1, k+1 inside
+    ck = rng.integers(0, p.nz - 1, n).astype(np.int32)
+    order = np.lexsort((cj, ck))
+    cj, ck = cj[order], ck[order]
+    fy, fz = rng.uniform(0.0, 1.0, n), rng.uniform(0.0, 1.0, n)
+    if shape == 1:
+        py_, pz_ = cj + fy, ck + fz
+    else:
+        py_, pz_ = cj.astype(float), ck.astype(float)
+    at = {f: 0.5 * (field(0, f, py_, pz_) + field(1, f, py_, pz_)) for f in range(6)}
+    for mode in ("E", "B"):
+        o = make(p)
+        for f in range(6):
+            on = (f < 3) == (mode == "E")
+            for which in (0, 1):
+                o.set_field(EB[f], field(which, f, J, K) if on else 0.0 * J, which=which)
+        mom = np.zeros((3, n)) if mode == "E" else rng.uniform(-1.5, 1.5, (3, n))
+        o.set_particles(cj, ck, fy, fz, mom[0], mom[1], mom[2], np.arange(n, dtype=np.int64))
+        o.push_deposit(1)
+        d = by_id(o.particles())
+        o.close()
+        p1 = np.stack([d["px"], d["py"], d["pz"]], 1)
+        if mode == "E":
+            want = -p.hx * np.stack([at[0], at[1], at[2]], 1)
+            assert np.abs(p1 - want).max() <= 1e-13 * np.abs(want).max(), "E gather"
+        else:
+            p0 = mom.T
+            g = np.sqrt(1 + (p0 ** 2).sum(1))
+            vx = p0[:, 0] / g
+            t = np.stack([at[3], at[4], at[5]], 1) * (-1.0) * p.hx / (2 * g * (1 - vx))[:, None]
+            rhs = np.cross(p1 + p0, t)
+            assert np.abs((p1 - p0) - rhs).max() <= 1e-13 * np.abs(rhs).max(), "B gather"
+
+
+@pytest.mark.parametrize("shape", [0, 1])
+def test_gather_bilinear_fields(oracle_mod, shape):
+    check_gather(oracle_mod.Oracle, shape)
