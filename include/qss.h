@@ -123,6 +123,12 @@ typedef struct qss_params {
    * FP64 or MIXED32; the adjoint works on single-wake contexts.            */
   int32_t nwake;
   const qss_bunch *wake_bunch;
+  /* uniform external magnetic field felt by the plasma electrons, added to
+   * the gathered B before the Boris rotation (bXext..bZext =
+   * BxExternal..BzExternal / sf_DimFields, movepart_splitlayer.cpp:867-869,
+   * 1271-1273; sf_DimFields = 1, domain.cpp:204).  Zero-initialise the
+   * struct and the field is off.                                           */
+  double  b_ext[3];
 } qss_params;
 
 /* ---- context ---------------------------------------------------------- */
@@ -293,17 +299,40 @@ int  qss_device_memory(int device, int64_t *used, int64_t *total);
 /* The reference pipelines a sweep over MPI ranks that own consecutive xi
  * chunks: a rank receives the boundary layer from the rank in front, sweeps
  * its chunk and sends its last layer on (PackLayer / SendLayer /
- * ReceiveLayer, para.cpp:94-412; movepart_splitlayer.cpp:125,545).  These
- * calls are that hand-off's wire format: the state a sweep continues from
- * (particles with their tile starts, the P-layer planes and the spectral
- * state) as one flat buffer of qss_layer_state_bytes bytes, device memory
- * (what a device-to-device send carries) or host memory.  A context that
- * unpacks the state of a context of the same parameters after layer s-1
- * and sweeps layers [s, ...) with qss_sweep_range gives the layers of one
- * unbroken sweep bitwise.  Single-wake contexts.                            */
+ * ReceiveLayer, para.cpp:94-412; movepart_splitlayer.cpp:125,545).
+ *
+ * Two forms of that hand-off:
+ *
+ * 1. The state a sweep continues from, bitwise: a 256-byte header (format
+ *    magic and version, grid, tiling, particle capacity), the particles with
+ *    their tile starts, the P-layer planes and the spectral state, as one
+ *    flat buffer of qss_layer_state_bytes bytes in device memory (what a
+ *    device-to-device send carries) or host memory.  qss_unpack_layer takes
+ *    the buffer's size and returns QSS_E_ARG, leaving the context untouched,
+ *    when the size or header does not match this context or a particle cell
+ *    or tile start is out of place.  A context that unpacks the state of a
+ *    context of the same parameters after layer s-1 and sweeps layers
+ *    [s, ...) with qss_sweep_range gives the layers of one unbroken sweep
+ *    bitwise.  Single-wake contexts.                                         */
 int  qss_layer_state_bytes(const qss_ctx *c, size_t *bytes);
 int  qss_pack_layer(qss_ctx *c, void *dst);
-int  qss_unpack_layer(qss_ctx *c, const void *src);
+int  qss_unpack_layer(qss_ctx *c, const void *src, size_t bytes);
+/* 2. The reference's own record (PackLayer para.cpp:94-160, UnpackLayer
+ *    :167-280), host memory, qss_layer_record_doubles doubles: 12 planes of
+ *    Ny*Nz (y fastest) Bx, By, Bz, Ex, Ey, Ez, Jx, Jy, Jz, Rho,
+ *    fftJxBeamHydro, fftRhoBeamHydro (the beam spectra of the last layer),
+ *    then 12 attribute arrays of Np particles: Px, Py, Pz, X, Y, Z, Q2m,
+ *    Weight, i_X, i_Y, i_Z, isort.  qss_pack_layer_record writes the P layer
+ *    (the last layer swept) as reference layer iLayer (i_X); unpack takes Np
+ *    from the length as ReceiveLayer does (para.cpp:385), requires every
+ *    particle to be a plasma electron of this sweep (Q2m = -1, the plasma
+ *    weight, isort 0, cell inside the grid, fractions in [0, 1]) and Np =
+ *    ppc * Ny * Nz, and rebuilds the P-layer spectra from the planes as the
+ *    reference's Guess does at every layer; the continuation then agrees
+ *    with an unbroken sweep to rounding.  Single-wake contexts.            */
+int  qss_layer_record_doubles(const qss_ctx *c, size_t *n);
+int  qss_pack_layer_record(qss_ctx *c, int iLayer, double *dst);
+int  qss_unpack_layer_record(qss_ctx *c, const double *src, size_t n);
 
 /* Writes history plane `slice` of field f (requires the history_mask bit);
  * the wake an evolving driver is pushed through (qss_driver_push).         */

@@ -22,7 +22,7 @@ check that does not restate the oracle's formula):
 * the whole sweep: the psi/Ex identity (tests/test_oracle_sweep.py) and
   second-order convergence to 3-D linear wake theory
   (tests/test_oracle_linear_wake.py);
-* tests/test_oracle_mutants.py plants 23 plausible mistakes in copies of the
+* tests/test_oracle_mutants.py plants plausible mistakes, one at a time, in copies of the
   source and checks that these pins catch each of them.
 """
 from __future__ import annotations
@@ -88,6 +88,10 @@ def _configure(L):
     L.oracle_driver_set.argtypes = [ctypes.c_void_p, ctypes.c_int64, _i32p, _dp, _dp,
                                     _dp, _i32p]
     L.oracle_driver_density.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+    L.oracle_layer_record_doubles.restype = ctypes.c_long
+    L.oracle_layer_record_doubles.argtypes = [ctypes.c_void_p]
+    L.oracle_pack_layer_record.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    L.oracle_unpack_layer_record.argtypes = [ctypes.c_void_p, _dp, ctypes.c_long]
     L.oracle_transform1d.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int]
     L.oracle_transform2d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, _dp, _dp, ctypes.c_double,
@@ -264,6 +268,17 @@ class Oracle:
         a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.nz, self.ny)
         if self._L.oracle_set_history(self.h, FIELD_ID[name], slice_, _p(a)) != 0:
             raise ValueError("no history for %s" % name)
+
+    # -- the reference's layer record (PackLayer / UnpackLayer) ----------
+    def pack_layer_record(self, iLayer):
+        buf = np.empty(self._L.oracle_layer_record_doubles(self.h), dtype=np.float64)
+        self._L.oracle_pack_layer_record(self.h, int(iLayer), _p(buf))
+        return buf
+
+    def unpack_layer_record(self, rec):
+        r = np.ascontiguousarray(rec, dtype=np.float64)
+        if self._L.oracle_unpack_layer_record(self.h, _p(r), r.size) != 0:
+            raise ValueError("not a layer record of this grid")
 
     # -- evolving driver -----------------------------------------------
     def driver_seed(self, driver: Driver) -> int:

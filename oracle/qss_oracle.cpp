@@ -566,6 +566,9 @@ int push_one(const oracle *o, const std::vector<double> *F, long n,
   pz += ez / (1. - Vx);
   double gamma = std::sqrt(1. + px * px + py * py + pz * pz);
   double gamma_r = 1. / gamma;
+  bx += p.b_ext[0];   // bXext..bZext, :867-869, added at :1271-1273
+  by += p.b_ext[1];
+  bz += p.b_ext[2];
   bx = bx * gamma_r * q2m / (1. - Vx) * hx / 2.;
   by = by * gamma_r * q2m / (1. - Vx) * hx / 2.;
   bz = bz * gamma_r * q2m / (1. - Vx) * hx / 2.;
@@ -1050,6 +1053,57 @@ oracle *oracle_create(const qss_params *p, int use_fft, int nthreads,
 }
 
 void oracle_destroy(oracle *o) { delete o; }
+
+// The reference's layer record of the P layer (PackLayer, para.cpp:94-160):
+// planes Bx, By, Bz, Ex, Ey, Ez, Jx, Jy, Jz, Rho, fftJxBeamHydro,
+// fftRhoBeamHydro, then per particle (attribute-major) Px, Py, Pz, X, Y, Z,
+// Q2m, Weight, i_X, i_Y, i_Z, isort, particles in storage order.
+long oracle_layer_record_doubles(const oracle *o) {
+  return 12 * o->ncell + 12 * (long)o->y.size();
+}
+
+int oracle_pack_layer_record(oracle *o, int iLayer, double *dst) {
+  const long nc = o->ncell, np = (long)o->y.size();
+  const int f[10] = {QSS_BX, QSS_BY, QSS_BZ, QSS_EX, QSS_EY, QSS_EZ,
+                     QSS_JX, QSS_JY, QSS_JZ, QSS_RHO};
+  for (int q = 0; q < 10; ++q) std::copy(o->P[f[q]].begin(), o->P[f[q]].end(), dst + q * nc);
+  std::copy(o->S[SP_JXB].begin(), o->S[SP_JXB].end(), dst + 10 * nc);
+  std::copy(o->S[SP_RHOB].begin(), o->S[SP_RHOB].end(), dst + 11 * nc);
+  double *r = dst + 12 * nc;
+  for (long i = 0; i < np; ++i) {
+    r[i] = o->px[i]; r[i + np] = o->py[i]; r[i + 2 * np] = o->pz[i];
+    r[i + 3 * np] = 0.0; r[i + 4 * np] = o->y[i]; r[i + 5 * np] = o->z[i];
+    r[i + 6 * np] = -1.0; r[i + 7 * np] = o->pweight;
+    r[i + 8 * np] = iLayer; r[i + 9 * np] = o->j[i]; r[i + 10 * np] = o->k[i];
+    r[i + 11 * np] = 0.0;
+  }
+  return QSS_OK;
+}
+
+// UnpackLayer (para.cpp:167-280) into the P layer: Np from the length
+// (ReceiveLayer, para.cpp:385); ids = record positions.  The oracle's Guess
+// re-transforms the P layer at every layer, like the reference, so the
+// planes, the beam spectra and the particles are its whole state.
+int oracle_unpack_layer_record(oracle *o, const double *src, long n) {
+  const long nc = o->ncell;
+  if (n < 12 * nc || (n - 12 * nc) % 12 != 0) return QSS_E_ARG;
+  const long np = (n - 12 * nc) / 12;
+  const int f[10] = {QSS_BX, QSS_BY, QSS_BZ, QSS_EX, QSS_EY, QSS_EZ,
+                     QSS_JX, QSS_JY, QSS_JZ, QSS_RHO};
+  for (int q = 0; q < 10; ++q) std::copy(src + q * nc, src + (q + 1) * nc, o->P[f[q]].begin());
+  std::copy(src + 10 * nc, src + 11 * nc, o->S[SP_JXB].begin());
+  std::copy(src + 11 * nc, src + 12 * nc, o->S[SP_RHOB].begin());
+  const double *r = src + 12 * nc;
+  o->y.resize(np); o->z.resize(np); o->px.resize(np); o->py.resize(np); o->pz.resize(np);
+  o->j.resize(np); o->k.resize(np); o->id.resize(np);
+  for (long i = 0; i < np; ++i) {
+    o->px[i] = r[i]; o->py[i] = r[i + np]; o->pz[i] = r[i + 2 * np];
+    o->y[i] = r[i + 4 * np]; o->z[i] = r[i + 5 * np];
+    o->j[i] = (int)r[i + 9 * np]; o->k[i] = (int)r[i + 10 * np];
+    o->id[i] = i;
+  }
+  return QSS_OK;
+}
 
 int oracle_reset(oracle *o) { reset(o); return QSS_OK; }
 

@@ -48,7 +48,10 @@ SIGNATURES = {
     "qss_set_history": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
     "qss_layer_state_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
     "qss_pack_layer": (ctypes.c_int, [_vp, _vp]),
-    "qss_unpack_layer": (ctypes.c_int, [_vp, _vp]),
+    "qss_unpack_layer": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    "qss_layer_record_doubles": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    "qss_pack_layer_record": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "qss_unpack_layer_record": (ctypes.c_int, [_vp, _dp, ctypes.c_size_t]),
     "qss_driver_seed": (ctypes.c_int, [_vp, ctypes.POINTER(QssDriver), _i64p]),
     "qss_driver_push": (ctypes.c_int, [_vp]),
     "qss_time_step": (ctypes.c_int, [_vp, _dp]),

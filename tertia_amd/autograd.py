@@ -11,7 +11,10 @@ The forward runs the checkpointed sweep on the library's stream and copies
 the Ex history (the longitudinal field) into a torch tensor; the backward
 hands dLoss/dEx (a device tensor) to the hand-written adjoint and returns
 dLoss/dbins.  Both directions synchronize the device around the library
-calls, which run on the library's own stream.
+calls, which run on the library's own stream.  One forward per mesh may be
+awaiting its backward: the reverse sweep recomputes from the mesh's latest
+checkpoints, so a second SweepEx.apply on the same mesh before the first
+backward makes that backward raise instead of returning the wrong gradient.
 """
 from __future__ import annotations
 
@@ -29,11 +32,16 @@ class SweepEx(torch.autograd.Function):
         torch.cuda.synchronize()
         mesh.copy_history("Ex", 0, int(nrun), out.data_ptr())
         ctx.mesh = mesh
+        ctx.generation = mesh.checkpoint_generation
         ctx.bins_device = bins.device
         return out
 
     @staticmethod
     def backward(ctx, grad_ex: torch.Tensor):
+        if ctx.mesh.checkpoint_generation != ctx.generation:
+            raise RuntimeError("SweepEx.backward: the mesh ran another checkpointed sweep or "
+                               "changed its profile bins after this forward; its adjoint "
+                               "would belong to that sweep (use one mesh per graph node)")
         g = grad_ex.detach().to(torch.float64).contiguous()
         torch.cuda.synchronize()
         gb = ctx.mesh.sweep_vjp(device_ptr=g.data_ptr())

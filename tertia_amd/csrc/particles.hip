@@ -81,6 +81,7 @@ struct PushArgs {
   const double *F;
   int ny, nz;
   double hx, hy, hz, weight;
+  double bext[3];   // uniform external B (movepart_splitlayer.cpp:1271-1273)
   int shape, fix_sub, full;
   int32_t *ev_cell, *ev_rank, *ev_sub;
   double *ev_y, *ev_z, *ev_w, *ev_p;
@@ -141,6 +142,9 @@ __global__ __launch_bounds__(NT) void k_push(PushArgs a) {
   pz += ez / (1. - Vx);
   double gamma = sqrt(1. + px * px + py * py + pz * pz);
   const double gamma_r = 1. / gamma;
+  bx += a.bext[0];
+  by += a.bext[1];
+  bz += a.bext[2];
   bx = bx * gamma_r * q2m / (1. - Vx) * hx / 2.;
   by = by * gamma_r * q2m / (1. - Vx) * hx / 2.;
   bz = bz * gamma_r * q2m / (1. - Vx) * hx / 2.;
@@ -480,6 +484,7 @@ hipError_t launch_push(qss_ctx *c, int full_step) {
   a.F = c->F;
   a.ny = c->ny; a.nz = c->nz;
   a.hx = c->hxs; a.hy = c->hy; a.hz = c->hz; a.weight = c->pweight;
+  for (int q = 0; q < 3; ++q) a.bext[q] = c->p.b_ext[q];
   a.shape = c->p.shape; a.fix_sub = c->p.fix_subcycle; a.full = full_step;
   a.ev_cell = c->ev_cell; a.ev_rank = c->ev_rank; a.ev_sub = c->ev_sub;
   a.ev_y = c->ev_y; a.ev_z = c->ev_z; a.ev_w = c->ev_w; a.ev_p = c->ev_p;

@@ -6,7 +6,7 @@
 // movepart_layer_CIC.cpp:166-203,449-496):
 //   f   = sum_c w_c(y0,z0) F_c                 bilinear gather of (P+C)/2
 //   e   = f_E q2m hx/2,  u = 1 - Vp_x,  K = e/u,  H = P0 + K
-//   b   = f_B beta,  beta = q2m hx / (2 g1 u),  g1 = sqrt(1+|H|^2)
+//   b   = (f_B + B_ext) beta,  beta = q2m hx / (2 g1 u),  g1 = sqrt(1+|H|^2)
 //   co  = 2/(1+|b|^2),  p3 = co (H + H x b),  P1 = H + K + p3 x b
 //   Vm  = P1/g2,  Vb = (Vm + Vp)/2,  v = 1 - Vb_x
 //   dj  = w dj0 Vm / v,  drho = w drho0 / v,  d = (Vb_y hx/hy, Vb_z hx/hz)/v
@@ -40,6 +40,7 @@ constexpr double kQ2m = -1.0, kDj0 = -1.0, kDrho0 = -1.0;
 
 struct Consts {
   double hx, hy, hz, weight;
+  double bext[3];   // uniform external B, added to the gathered B
   int ny, nz;
 };
 
@@ -72,14 +73,14 @@ QSS_HD inline void kinematics(const Consts &c, const double P0[3], const double 
   const double px = P0[0], py = P0[1], pz = P0[2];
   k.g0 = sqrt(1. + px * px + py * py + pz * pz);
   k.Vp[0] = px / k.g0; k.Vp[1] = py / k.g0; k.Vp[2] = pz / k.g0;
-  for (int q = 0; q < 3; ++q) { k.fE[q] = f[q]; k.fB[q] = f[3 + q]; }
+  for (int q = 0; q < 3; ++q) { k.fE[q] = f[q]; k.fB[q] = f[3 + q] + c.bext[q]; }
   for (int q = 0; q < 3; ++q) k.e[q] = f[q] * (q2m * hx / 2.);
   k.u = 1. - k.Vp[0];
   for (int q = 0; q < 3; ++q) k.K[q] = k.e[q] / k.u;
   for (int q = 0; q < 3; ++q) k.H[q] = P0[q] + k.K[q];
   k.g1 = sqrt(1. + k.H[0] * k.H[0] + k.H[1] * k.H[1] + k.H[2] * k.H[2]);
   const double gr = 1. / k.g1;
-  for (int q = 0; q < 3; ++q) k.b[q] = f[3 + q] * gr * q2m / k.u * hx / 2.;
+  for (int q = 0; q < 3; ++q) k.b[q] = k.fB[q] * gr * q2m / k.u * hx / 2.;
   k.beta = gr * q2m / k.u * hx / 2.;
   const double *b = k.b, *H = k.H;
   k.co = 2. / (1. + (b[0] * b[0]) + (b[1] * b[1]) + (b[2] * b[2]));

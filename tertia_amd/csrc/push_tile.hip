@@ -57,6 +57,7 @@ struct TileArgs {
   const double *F;
   const float *Ff;        // fp32 push field (mixed modes)
   double hx, hy, hz, weight;
+  double bext[3];         // uniform external B (movepart_splitlayer.cpp:1271-1273)
   int shape, full;
   double *win;            // ntiles x NWIN x 4
   // full step staging (indexed by the old slot)
@@ -209,6 +210,9 @@ void k_push_tile(TileArgs a0) {
       const R hpx = (R)px, hpy = (R)py, hpz = (R)pz;
       R gamma = sqrt(R(1) + hpx * hpx + hpy * hpy + hpz * hpz);
       const R gamma_r = R(1) / gamma;
+      bx += R(a.bext[0]);
+      by += R(a.bext[1]);
+      bz += R(a.bext[2]);
       bx = bx * gamma_r * R(q2m) / (R(1) - Vx) * R(hx) / R(2);
       by = by * gamma_r * R(q2m) / (R(1) - Vx) * R(hx) / R(2);
       bz = bz * gamma_r * R(q2m) / (R(1) - Vx) * R(hx) / R(2);
@@ -645,6 +649,7 @@ TileArgs make_args(qss_ctx *c, int full) {
   a.F = c->F;
   a.Ff = c->Ff;
   a.hx = c->hxs; a.hy = c->hy; a.hz = c->hz; a.weight = c->pweight;
+  for (int q = 0; q < 3; ++q) a.bext[q] = c->p.b_ext[q];
   a.shape = c->p.shape; a.full = full;
   a.win = c->win;
   const long npall = c->np_cap * c->nw;

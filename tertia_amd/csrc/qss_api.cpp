@@ -146,6 +146,11 @@ int validate(const qss_params *p) {
     qss_set_error("nwake must be in [0, 1024]");
     return QSS_E_ARG;
   }
+  for (int q = 0; q < 3; ++q)
+    if (!std::isfinite(p->b_ext[q])) {
+      qss_set_error("b_ext must be finite");
+      return QSS_E_ARG;
+    }
   if (p->nwake > 1 && (p->ny % kTile != 0 || p->nz % kTile != 0 || p->fix_subcycle ||
                        (p->precision != QSS_PREC_FP64 && p->precision != QSS_PREC_MIXED32))) {
     qss_set_error("a batch of wakes needs ny, nz multiples of 16, fix_subcycle = 0 and "
@@ -478,8 +483,36 @@ int do_layer_step(qss_ctx *c, int slice, int isplit, double *res_dev, char *tape
 }
 
 // checkpoint: particles + P planes + spectral state at a layer start
+// header of a packed layer state (qss_pack_layer): magic "QSSLAYR1", format
+// version, grid, tiling and capacity, padded to 256 bytes so the arrays after
+// it keep their 256-byte alignment
+constexpr size_t kStateHeader = 256;
+struct StateHeader {
+  uint64_t magic;
+  int32_t version, ny, nz, ntj, ntk, tiled;
+  int64_t np_cap;
+  uint64_t payload;
+  char pad[kStateHeader - 48];
+};
+static_assert(sizeof(StateHeader) == kStateHeader, "header size");
+
+constexpr int kRecPlanes = 12, kRecAttr = 12;
+// record planes 0..9 (para.cpp:109-118)
+constexpr int kRecField[10] = {QSS_BX, QSS_BY, QSS_BZ, QSS_EX, QSS_EY, QSS_EZ,
+                               QSS_JX, QSS_JY, QSS_JZ, QSS_RHO};
+
 size_t chk_bytes(const qss_ctx *c) {
   return part_bytes(c) + (size_t)(QSS_NFIELD + SP_N) * align256(c->ncell * 8);
+}
+StateHeader state_header(const qss_ctx *c) {
+  StateHeader h;
+  std::memset(&h, 0, sizeof h);
+  h.magic = 0x315259414C535351ull;   // "QSSLAYR1" little-endian
+  h.version = 1;
+  h.ny = c->ny; h.nz = c->nz; h.ntj = c->ntj; h.ntk = c->ntk; h.tiled = c->tiled ? 1 : 0;
+  h.np_cap = c->np_cap;
+  h.payload = chk_bytes(c);
+  return h;
 }
 int checkpoint(qss_ctx *c, char *base, int restore) {
   CHECK(copy_particles(c, part_view(c, base), restore));
@@ -551,6 +584,7 @@ int adj_push_pass(qss_ctx *c, char *slot, int pass, int full, int cur_adj) {
   g.cell = v.cell; g.pid = v.pid; g.tile_start = v.tstart;
   g.ntj = c->ntj; g.ntk = c->ntk; g.ny = c->ny; g.nz = c->nz;
   g.hx = c->hxs; g.hy = c->hy; g.hz = c->hz; g.weight = c->pweight;
+  for (int q = 0; q < 3; ++q) g.bext[q] = c->p.b_ext[q];
   g.F = tape_field(c, slot, pass);
   g.Jb = s.jbar;
   g.full = full;
@@ -1656,13 +1690,15 @@ int qss_timing(qss_ctx *c, double *ms_total, double *ms_kernel, int64_t *launche
 }
 
 // ---- layer hand-off between xi chunks ----------------------------------------
-// The layer state a sweep continues from: the particles (current buffer and
-// tile starts), the P-layer planes and the spectral state; the layout of the
-// adjoint's checkpoints.
+// The layer state a sweep continues from: a 256-byte header, then the
+// particles (current buffer and tile starts), the P-layer planes and the
+// spectral state in the layout of the adjoint's checkpoints.  The header
+// names the grid the state belongs to; unpacking checks it, the size, and the
+// particle cells and tile starts before anything reaches the context.
 int qss_layer_state_bytes(const qss_ctx *c, size_t *bytes) {
   if (!c || !bytes) return QSS_E_ARG;
   if (c->nw != 1) { qss_set_error("layer hand-off: one wake"); return QSS_E_ARG; }
-  *bytes = chk_bytes(c);
+  *bytes = kStateHeader + chk_bytes(c);
   return QSS_OK;
 }
 
@@ -1670,18 +1706,217 @@ int qss_pack_layer(qss_ctx *c, void *dst) {
   if (!c || !dst) return QSS_E_ARG;
   if (c->nw != 1) { qss_set_error("layer hand-off: one wake"); return QSS_E_ARG; }
   HIPCHECK(hipSetDevice(c->device));
-  CHECK(checkpoint(c, static_cast<char *>(dst), 0));
-  HIPCHECK(hipStreamSynchronize(c->stream));
+  const StateHeader h = state_header(c);
+  char *base = static_cast<char *>(dst);
+  HIPCHECK(hipMemcpyAsync(base, &h, sizeof h, hipMemcpyDefault, c->stream));
+  CHECK(checkpoint(c, base + kStateHeader, 0));
+  HIPCHECK(hipStreamSynchronize(c->stream));   // h lives on this stack frame
   return QSS_OK;
 }
 
-int qss_unpack_layer(qss_ctx *c, const void *src) {
+int qss_unpack_layer(qss_ctx *c, const void *src, size_t bytes) {
   if (!c || !src) return QSS_E_ARG;
   if (c->nw != 1) { qss_set_error("layer hand-off: one wake"); return QSS_E_ARG; }
   HIPCHECK(hipSetDevice(c->device));
+  if (bytes != kStateHeader + chk_bytes(c)) {
+    qss_set_error("layer hand-off: the buffer has " + std::to_string(bytes) +
+                  " bytes, this context's state " +
+                  std::to_string(kStateHeader + chk_bytes(c)));
+    return QSS_E_ARG;
+  }
+  const char *base = static_cast<const char *>(src);
+  StateHeader h{}, want = state_header(c);
+  HIPCHECK(hipMemcpy(&h, base, sizeof h, hipMemcpyDefault));
+  if (std::memcmp(&h, &want, sizeof h) != 0) {
+    qss_set_error("layer hand-off: the buffer holds the state of another grid or format "
+                  "(header mismatch)");
+    return QSS_E_ARG;
+  }
+  // the cells and tile starts drive the push kernels' indexing: check them
+  const PartView v = part_view(c, const_cast<char *>(base + kStateHeader));
+  std::vector<int32_t> cell(c->np_cap);
+  HIPCHECK(hipMemcpy(cell.data(), v.cell, cell.size() * sizeof(int32_t), hipMemcpyDefault));
+  for (int32_t q : cell)
+    if (q < 0 || q >= c->ncell) {
+      qss_set_error("layer hand-off: a particle cell is outside the grid");
+      return QSS_E_ARG;
+    }
+  if (c->tiled) {
+    const long nt = (long)c->ntj * c->ntk;
+    std::vector<int32_t> ts(nt + 1);
+    HIPCHECK(hipMemcpy(ts.data(), v.tstart, ts.size() * sizeof(int32_t), hipMemcpyDefault));
+    bool ok = ts[0] == 0 && ts[nt] == c->np_cap;
+    for (long t = 0; ok && t < nt; ++t) {
+      ok = ts[t] <= ts[t + 1] && ts[t + 1] <= c->np_cap;
+      for (long q = ts[t]; ok && q < ts[t + 1]; ++q) {   // each particle in its own tile
+        const int j = cell[q] % c->ny, k = cell[q] / c->ny;
+        ok = (k / kTile) * c->ntj + j / kTile == t;
+      }
+    }
+    if (!ok) {
+      qss_set_error("layer hand-off: the tile starts do not describe the particles");
+      return QSS_E_ARG;
+    }
+  }
   HIPCHECK(hipStreamSynchronize(c->stream));
   forget_splits(c);   // the planes change under any recorded split operands
-  CHECK(checkpoint(c, const_cast<char *>(static_cast<const char *>(src)), 1));
+  CHECK(checkpoint(c, const_cast<char *>(base + kStateHeader), 1));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->np = c->np_cap;
+  c->push_dirty = true;
+  return QSS_OK;
+}
+
+// ---- the reference's layer record (PackLayer / UnpackLayer, para.cpp:94-160, 167-280)
+// Doubles: 12 planes of Ny*Nz (y fastest) in the order Bx, By, Bz, Ex, Ey, Ez,
+// Jx, Jy, Jz, Rho, fftJxBeamHydro, fftRhoBeamHydro (para.cpp:109-120), then
+// 12 attributes of Np particles, attribute-major: Px, Py, Pz, X, Y, Z, Q2m,
+// Weight, i_X, i_Y, i_Z, isort (para.cpp:138-149; integers stored as doubles).
+// The planes are the P layer (the last layer swept); the two beam planes are
+// the beam spectra the next layer's Guess takes as its previous beam
+// spectrum (CUDA_WRAP_getBeamFFT, paraCPUlayers.cu:56).  Np comes from the
+// record's length as in ReceiveLayer (para.cpp:385).
+int qss_layer_record_doubles(const qss_ctx *c, size_t *n) {
+  if (!c || !n) return QSS_E_ARG;
+  if (c->nw != 1) { qss_set_error("layer record: one wake"); return QSS_E_ARG; }
+  *n = (size_t)kRecPlanes * c->ncell + (size_t)kRecAttr * c->np;
+  return QSS_OK;
+}
+
+int qss_pack_layer_record(qss_ctx *c, int iLayer, double *dst) {
+  if (!c || !dst) return QSS_E_ARG;
+  if (c->nw != 1) { qss_set_error("layer record: one wake"); return QSS_E_ARG; }
+  HIPCHECK(hipSetDevice(c->device));
+  const long nc = c->ncell, np = c->np;
+  for (int q = 0; q < 10; ++q)
+    HIPCHECK(hipMemcpyAsync(dst + q * nc, c->P[kRecField[q]], nc * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipMemcpyAsync(dst + 10 * nc, c->S[SP_JXB], nc * sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipMemcpyAsync(dst + 11 * nc, c->S[SP_RHOB], nc * sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
+  const int b = c->cur;
+  std::vector<double> a(5 * np);
+  std::vector<int32_t> cell(np);
+  double *const from[5] = {c->ppx[b], c->ppy[b], c->ppz[b], c->py_[b], c->pz_[b]};
+  for (int q = 0; q < 5; ++q)
+    HIPCHECK(hipMemcpyAsync(a.data() + q * np, from[q], np * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipMemcpyAsync(cell.data(), c->pcell[b], np * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  double *r = dst + kRecPlanes * nc;
+  for (long i = 0; i < np; ++i) {   // storage (slot) order
+    r[i] = a[i];                            // Px
+    r[i + np] = a[i + np];                  // Py
+    r[i + 2 * np] = a[i + 2 * np];          // Pz
+    r[i + 3 * np] = 0.0;                    // X: layer particles sit at x = 0
+    r[i + 4 * np] = a[i + 3 * np];          // Y (in-cell fraction)
+    r[i + 5 * np] = a[i + 4 * np];          // Z
+    r[i + 6 * np] = -1.0;                   // Q2m of the electrons
+    r[i + 7 * np] = c->pweight;             // Weight
+    r[i + 8 * np] = (double)iLayer;         // i_X
+    r[i + 9 * np] = (double)(cell[i] % c->ny);   // i_Y
+    r[i + 10 * np] = (double)(cell[i] / c->ny);  // i_Z
+    r[i + 11 * np] = 0.0;                   // isort: species 0
+  }
+  return QSS_OK;
+}
+
+int qss_unpack_layer_record(qss_ctx *c, const double *src, size_t n) {
+  if (!c || !src) return QSS_E_ARG;
+  if (c->nw != 1) { qss_set_error("layer record: one wake"); return QSS_E_ARG; }
+  const long nc = c->ncell;
+  if (n < (size_t)kRecPlanes * nc || (n - (size_t)kRecPlanes * nc) % kRecAttr != 0) {
+    qss_set_error("layer record: the length is not 12 Ny*Nz + 12 Np doubles");
+    return QSS_E_ARG;
+  }
+  const long np = (long)((n - (size_t)kRecPlanes * nc) / kRecAttr);
+  if (np != c->np_cap) {
+    qss_set_error("layer record: " + std::to_string(np) + " particles, the sweep carries " +
+                  std::to_string(c->np_cap) + " (ppc per cell)");
+    return QSS_E_ARG;
+  }
+  const double *r = src + kRecPlanes * nc;
+  std::vector<int32_t> cell(np);
+  for (long i = 0; i < np; ++i) {
+    const double jy = r[i + 9 * np], kz = r[i + 10 * np];
+    const double y = r[i + 4 * np], z = r[i + 5 * np];
+    const bool ok = jy >= 0 && jy < c->ny && jy == std::floor(jy) && kz >= 0 && kz < c->nz &&
+                    kz == std::floor(kz) && y >= 0 && y <= 1 && z >= 0 && z <= 1 &&
+                    r[i + 6 * np] == -1.0 && r[i + 7 * np] == c->pweight &&
+                    r[i + 11 * np] == 0.0 && std::isfinite(r[i]) &&
+                    std::isfinite(r[i + np]) && std::isfinite(r[i + 2 * np]);
+    if (!ok) {
+      qss_set_error("layer record: particle " + std::to_string(i) +
+                    " is not a plasma electron of this sweep (cell, in-cell position, Q2m, "
+                    "Weight, isort or a momentum)");
+      return QSS_E_ARG;
+    }
+    cell[i] = (int32_t)jy + c->ny * (int32_t)kz;
+  }
+  // storage order: tile-major (stable, so a record written by
+  // qss_pack_layer_record comes back in the order it was written)
+  std::vector<long> order(np);
+  std::iota(order.begin(), order.end(), 0L);
+  std::vector<int32_t> ts;
+  if (c->tiled) {
+    const long nt = (long)c->ntj * c->ntk;
+    auto tile = [&](long i) {
+      return (cell[i] / c->ny / kTile) * c->ntj + (cell[i] % c->ny) / kTile;
+    };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](long x, long y) { return tile(x) < tile(y); });
+    ts.assign(nt + 1, 0);
+    for (long i = 0; i < np; ++i) ++ts[tile(i) + 1];
+    for (long t = 0; t < nt; ++t) ts[t + 1] += ts[t];
+  }
+  std::vector<double> a(5 * np);
+  std::vector<int32_t> sc(np);
+  std::vector<int64_t> id(np);
+  const int src_attr[5] = {4, 5, 0, 1, 2};   // y, z, px, py, pz
+  for (long q = 0; q < np; ++q) {
+    const long i = order[q];
+    for (int m = 0; m < 5; ++m) a[m * np + q] = r[i + src_attr[m] * np];
+    sc[q] = cell[i];
+    id[q] = i;   // the record's position labels the particle
+  }
+  HIPCHECK(hipSetDevice(c->device));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  forget_splits(c);
+  const int b = c->cur;
+  double *const to[5] = {c->py_[b], c->pz_[b], c->ppx[b], c->ppy[b], c->ppz[b]};
+  for (int m = 0; m < 5; ++m)
+    HIPCHECK(hipMemcpyAsync(to[m], a.data() + m * np, np * sizeof(double),
+                            hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->pcell[b], sc.data(), np * sizeof(int32_t), hipMemcpyHostToDevice,
+                          c->stream));
+  HIPCHECK(hipMemcpyAsync(c->pid[b], id.data(), np * sizeof(int64_t), hipMemcpyHostToDevice,
+                          c->stream));
+  if (c->tiled)
+    HIPCHECK(hipMemcpyAsync(c->tile_start[b], ts.data(), ts.size() * sizeof(int32_t),
+                            hipMemcpyHostToDevice, c->stream));
+  for (int q = 0; q < 10; ++q)
+    HIPCHECK(hipMemcpyAsync(c->P[kRecField[q]], src + q * nc, nc * sizeof(double),
+                            hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->S[SP_JXB], src + 10 * nc, nc * sizeof(double),
+                          hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(hipMemcpyAsync(c->S[SP_RHOB], src + 11 * nc, nc * sizeof(double),
+                          hipMemcpyHostToDevice, c->stream));
+  for (int f : {QSS_RHOBEAM, QSS_JXBEAM, QSS_PSI})
+    HIPCHECK(launch_fill_f64(c->P[f], nc, 0.0, c->stream));
+  // the spectra the next Guess reads as its P-layer spectra: the reference
+  // re-transforms the P layer at every layer (movefieldshydrolinlayersplit.cpp:
+  // 569-623); this context keeps them, so a record supplies them this way
+  const int kz[7] = {QSS_KIND_C, QSS_KIND_C, QSS_KIND_C, QSS_KIND_S,
+                     QSS_KIND_C, QSS_KIND_C, QSS_KIND_S};
+  const int ky[7] = {QSS_KIND_C, QSS_KIND_C, QSS_KIND_S, QSS_KIND_C,
+                     QSS_KIND_C, QSS_KIND_S, QSS_KIND_C};
+  const double *in[7] = {c->P[QSS_RHO], c->P[QSS_EX], c->P[QSS_EY], c->P[QSS_EZ],
+                         c->P[QSS_JX], c->P[QSS_JY], c->P[QSS_JZ]};
+  double *out[7] = {c->S[SP_RHO], c->S[SP_EX], c->S[SP_EY], c->S[SP_EZ],
+                    c->S[SP_JX], c->S[SP_JY], c->S[SP_JZ]};
+  CHECK(transform_batch(c, 7, kz, ky, in, out, 1.0, nullptr, nullptr, nullptr));
   HIPCHECK(hipStreamSynchronize(c->stream));
   c->np = c->np_cap;
   c->push_dirty = true;

@@ -87,6 +87,7 @@ struct AdjTileArgs {
   const int32_t *tile_start;
   int ntj, ntk, ny, nz;
   double hx, hy, hz, weight;
+  double bext[3];       // uniform external B
   const double *F;      // the pass's push field (6 planes, taped)
   const double *Jb;     // deposit adjoint planes of Jx, Jy, Jz
   int full;

@@ -15,7 +15,11 @@ thickness the reference actually runs with.
 The lists used by the xi sweep:
 
 * ``&Domain`` (domain.cpp:42-79): ``Hx, Hy, Hz, Xlength, Ylength, Zlength,
-  NxSplit`` (default 1), ``nIter`` (default 1), ``Nspecies`` (default 1).
+  NxSplit`` (default 1), ``nIter`` (default 1), ``Nspecies`` (default 1),
+  ``BxExternal, ByExternal, BzExternal`` (default 0): the uniform external B
+  that the plasma push adds to the gathered field (bXext..bZext,
+  movepart_splitlayer.cpp:867-869, 1271-1273; Cell::sf_DimFields = 1,
+  domain.cpp:204).
   Grid sizing as the reference (domain.cpp:141-145, one rank):
   ``l_Xsize = long(Xlength/Hx) + 1`` layers, ``l_Ysize = long(Ylength/Hy) + 1``
   cells over ``l_Ysize * Hy`` (same for z).
@@ -227,9 +231,10 @@ def load_deck(deck: Union[str, os.PathLike], shape: int = QSS_SHAPE_NGP_REF,
                              polarity=sp.d("Polarity", 1.0), vx=vx))
     if len(bunches) > 4:
         raise DeckError("at most 4 beam species")
+    b_ext = (dom.d("BxExternal", 0.0), dom.d("ByExternal", 0.0), dom.d("BzExternal", 0.0))
     return Params(ny=ny, nz=nz, ly=ny * hy, lz=nz * hz, hx=hx, nx_split=nx_split,
                   nslices=nslices, n_iter=niter, dens=dens, ppc_y=2, ppc_z=2, jitter=0.0,
-                  shape=shape, fix_subcycle=0, bunches=bunches)
+                  shape=shape, fix_subcycle=0, bunches=bunches, b_ext=b_ext)
 
 
 def load_driver(deck: Union[str, os.PathLike]) -> Driver:

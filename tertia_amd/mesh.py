@@ -59,6 +59,10 @@ class Mesh:
         self.h = h
         self.ny, self.nz = params.ny, params.nz
         self.l_Mx = params.nslices
+        # bumped by every checkpointed sweep and every change of the profile
+        # bins: an adjoint taken for an older forward would be that of the
+        # newer one (the reverse sweep recomputes from the latest checkpoints)
+        self.checkpoint_generation = 0
 
     # -- lifetime --------------------------------------------------------
     def close(self):
@@ -210,8 +214,25 @@ class Mesh:
 
     def unpack_layer(self, buf: np.ndarray):
         buf = np.ascontiguousarray(buf, dtype=np.uint8)
-        check(self.L.qss_unpack_layer(self.h, buf.ctypes.data_as(ctypes.c_void_p)),
+        check(self.L.qss_unpack_layer(self.h, buf.ctypes.data_as(ctypes.c_void_p), buf.size),
               "qss_unpack_layer")
+
+    def PackLayer(self, iLayer: int) -> np.ndarray:
+        """The reference's layer record (PackLayer, para.cpp:94-160) of the
+        last swept layer, written as reference layer iLayer: 12 planes of
+        Ny*Nz doubles, then 12 attribute arrays of Np particles."""
+        n = ctypes.c_size_t()
+        check(self.L.qss_layer_record_doubles(self.h, ctypes.byref(n)),
+              "qss_layer_record_doubles")
+        buf = np.empty(n.value, dtype=np.float64)
+        check(self.L.qss_pack_layer_record(self.h, int(iLayer), _p(buf)), "qss_pack_layer_record")
+        return buf
+
+    def UnpackLayer(self, record: np.ndarray):
+        """Continue from a reference layer record (UnpackLayer, para.cpp:167-280):
+        Np follows from its length as in ReceiveLayer (para.cpp:385)."""
+        r = np.ascontiguousarray(record, dtype=np.float64)
+        check(self.L.qss_unpack_layer_record(self.h, _p(r), r.size), "qss_unpack_layer_record")
 
     # -- evolving driver: Domain::Step (step.cpp:63,105) ---------------------
     def SeedBeamParticles(self, driver: Driver) -> int:
@@ -377,6 +398,7 @@ class Mesh:
         """Replace the driver's piecewise-constant longitudinal profile."""
         b = np.ascontiguousarray(bins, dtype=np.float64)
         check(self.L.qss_set_bins(self.h, _p(b), int(b.size)), "qss_set_bins")
+        self.checkpoint_generation += 1
 
     def sweep_checkpointed(self, nrun: int, every: int = 32) -> np.ndarray:
         """Sweep layers [0, nrun) from a fresh front, keeping a checkpoint every
@@ -384,6 +406,7 @@ class Mesh:
         res = np.zeros(nrun)
         check(self.L.qss_sweep_checkpointed(self.h, int(nrun), int(every), _p(res)),
               "qss_sweep_checkpointed")
+        self.checkpoint_generation += 1
         return res
 
     def sweep_vjp(self, ex_bar=None, device_ptr: Optional[int] = None) -> np.ndarray:

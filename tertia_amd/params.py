@@ -90,6 +90,7 @@ class QssParams(ctypes.Structure):
         ("bins", ctypes.POINTER(ctypes.c_double)),
         ("nwake", ctypes.c_int32),
         ("wake_bunch", ctypes.POINTER(QssBunch)),
+        ("b_ext", ctypes.c_double * 3),
     ]
 
 
@@ -131,6 +132,8 @@ class Params:
     # batch of independent wakes on this grid: per-wake bunch lists (all the
     # same length as `bunches`); None = one wake
     wakes: Optional[List[List[Bunch]]] = None
+    # uniform external B on the plasma (&Domain BxExternal..BzExternal)
+    b_ext: Sequence[float] = (0.0, 0.0, 0.0)
 
     @property
     def n_particles(self) -> int:
@@ -163,6 +166,8 @@ class Params:
         else:
             c.nbins = 0
             c.bins = None
+        for a in range(3):
+            c.b_ext[a] = float(self.b_ext[a])
         c.nwake = self.nwake
         if self.wakes:
             nbunch = len(self.bunches)

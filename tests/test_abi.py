@@ -124,7 +124,10 @@ def test_driver_entry_points_refuse_before_seeding_without_a_gpu():
     n = ctypes.c_size_t()
     assert L.qss_layer_state_bytes(None, ctypes.byref(n)) == -1
     assert L.qss_pack_layer(None, None) == -1
-    assert L.qss_unpack_layer(None, None) == -1
+    assert L.qss_unpack_layer(None, None, 0) == -1
+    assert L.qss_layer_record_doubles(None, ctypes.byref(n)) == -1
+    assert L.qss_pack_layer_record(None, 0, None) == -1
+    assert L.qss_unpack_layer_record(None, None, 0) == -1
 
 
 def test_bench_refuses_more_than_one_gpu():

@@ -36,3 +36,19 @@ def test_spectral_transposes():
     r = _run("spectral_adj_check.cpp")
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_autograd_refuses_a_stale_backward():
+    """SweepEx.backward refuses when the mesh has run another checkpointed
+    sweep (or changed its bins) since the forward it belongs to; the check
+    comes before any device work."""
+    import types
+
+    import pytest
+    import torch
+
+    from tertia_amd.autograd import SweepEx
+    mesh = types.SimpleNamespace(checkpoint_generation=3)
+    ctx = types.SimpleNamespace(mesh=mesh, generation=2, bins_device=torch.device("cpu"))
+    with pytest.raises(RuntimeError, match="another checkpointed sweep"):
+        SweepEx.backward(ctx, torch.zeros(1, dtype=torch.float64))

@@ -145,3 +145,16 @@ def test_deck_to_driver():
     assert list(d.b_ext) == [0.0, f32(0.1), 0.0]
     with pytest.raises(DeckError):
         load_driver(DECK.replace(" Beam = 1\n", " Beam = 1\n Type = 0\n"))
+
+
+def test_deck_external_b_reaches_the_plasma():
+    """&Domain BxExternal..BzExternal (read as floats) are the plasma push's
+    bXext..bZext (movepart_splitlayer.cpp:867-869; sf_DimFields = 1)."""
+    assert list(load_deck(DECK).b_ext) == [0.0, 0.0, 0.0]
+    deck = DECK.replace(" Nspecies = 2\n", " Nspecies = 2\n BxExternal = 0.3\n"
+                        " BzExternal = -0.05\n")
+    p = load_deck(deck)
+    assert list(p.b_ext) == [f32(0.3), 0.0, f32(-0.05)]
+    c = p.to_c()
+    assert list(c.b_ext) == [f32(0.3), 0.0, f32(-0.05)]
+

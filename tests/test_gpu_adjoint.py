@@ -95,10 +95,12 @@ def test_zero_loss_adjoint_is_zero():
     assert np.all(g == 0.0)
 
 
-@pytest.mark.parametrize("every", [8, 56])
-def test_adjoint_vs_finite_differences_small(every):
+@pytest.mark.parametrize("every,b_ext", [(8, (0.0, 0.0, 0.0)), (56, (0.0, 0.0, 0.0)),
+                                          (8, (0.5, -0.3, 0.8))])
+def test_adjoint_vs_finite_differences_small(every, b_ext):
+    """The last case puts the plasma in a uniform external B."""
     from tertia_amd.mesh import Mesh
-    p = small_bins()
+    p = small_bins(b_ext=b_ext)
     nrun = 56
     m = Mesh(p, history=("Ex",))
     try:
@@ -156,10 +158,22 @@ loss.backward()
 l2, gex = witness_loss_np(ex.detach().cpu().numpy(), p, **WIN_SMALL)
 m.sweep_checkpointed(56, 8)
 g2 = m.sweep_vjp(gex)
+# a second forward on the same mesh before the first backward: that backward
+# must refuse (it would return the second forward's gradient)
+ba = torch.tensor(p.bins, dtype=torch.float64, requires_grad=True)
+la = witness_loss(SweepEx.apply(ba, m, 56, 8), p, **WIN_SMALL)
+lb = witness_loss(SweepEx.apply(ba * 1.01, m, 56, 8), p, **WIN_SMALL)
+try:
+    la.backward()
+    stale_refused = False
+except RuntimeError:
+    stale_refused = True
 m.close()
 d = float(np.abs(bins.grad.numpy() - g2).max() / np.abs(g2).max())
-print("torch loss %%.6e numpy loss %%.6e grad diff %%.2e" %% (loss.item(), l2, d))
-sys.exit(0 if abs(loss.item() - l2) <= 1e-12 * abs(l2) and d < 1e-12 else 1)
+print("torch loss %%.6e numpy loss %%.6e grad diff %%.2e stale refused %%s" %%
+      (loss.item(), l2, d, stale_refused))
+sys.exit(0 if abs(loss.item() - l2) <= 1e-12 * abs(l2) and d < 1e-12 and stale_refused
+         else 1)
 """ % ROOT
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
                        timeout=300, cwd=ROOT)

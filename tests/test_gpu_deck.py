@@ -13,9 +13,12 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("Ex", "Ez", "By", "Psi")
 
 
-def test_deck_sweep_vs_oracle(oracle_mod):
+@pytest.mark.parametrize("bext", ["", " BxExternal = 0.3\n ByExternal = -0.2\n BzExternal = 0.15\n"])
+def test_deck_sweep_vs_oracle(oracle_mod, bext):
+    """Without and with the deck's uniform external B on the plasma."""
     from tertia_amd.mesh import Mesh
-    p = load_deck(DECK)
+    p = load_deck(DECK.replace(" Nspecies = 2\n", " Nspecies = 2\n" + bext))
+    assert any(p.b_ext) == bool(bext)
     m = Mesh(p, history=FIELDS)
     o = oracle_mod.Oracle(p, use_fft=True, nthreads=8, history=FIELDS)
     try:

@@ -121,9 +121,21 @@ def test_guess_and_iterate(oracle_mod, nz, ny):
 def test_push_deposit(oracle_mod, shape, fix_sub, ny, nz):
     """(48,32) and (16,16) with the kept sub-cycle quirk run the tiled LDS
     deposit; (32,24) and fix_sub=1 run the global counting-sort deposit."""
+    _push_deposit(oracle_mod, shape, fix_sub, ny, nz)
+
+
+@pytest.mark.parametrize("ny,nz", [(32, 24), (48, 32)])
+def test_push_deposit_external_b(oracle_mod, ny, nz):
+    """The deck's uniform external B on the plasma (movepart_splitlayer.cpp:
+    1271-1273), global (32x24) and tiled (48x32) deposit paths."""
+    _push_deposit(oracle_mod, 1, 0, ny, nz, b_ext=(0.6, -0.4, 0.9))
+
+
+def _push_deposit(oracle_mod, shape, fix_sub, ny, nz, b_ext=(0.0, 0.0, 0.0)):
     rng = np.random.default_rng(100 + 10 * shape + fix_sub + ny)
     # a thick layer makes fast particles sub-cycle (|dy| > 1 cell)
-    p = small(ny=ny, nz=nz, nslices=3, hx=0.4, shape=shape, fix_subcycle=fix_sub)
+    p = small(ny=ny, nz=nz, nslices=3, hx=0.4, shape=shape, fix_subcycle=fix_sub,
+              b_ext=b_ext)
     m, o = make_pair(oracle_mod, p)
     for which, lay in ((0, "C"), (1, "P")):
         for f in EB:

@@ -17,7 +17,7 @@ import pytest
 
 from tests.test_oracle_beam import check_beam, check_bins
 from tests.test_oracle_fields import check_guess, check_iterate
-from tests.test_oracle_push import check_gather
+from tests.test_oracle_push import check_external_b, check_gather
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_DIR = os.path.join(os.path.dirname(HERE), "oracle")
@@ -81,6 +81,7 @@ MUTANTS = {
     "NGP gather from the next node": ("for (int c = 0; c < 6; ++c) f[c] = F[c][n00];",
                                       "for (int c = 0; c < 6; ++c) f[c] = F[c][n00 + 1];",
                                       [lambda mk: check_gather(mk, 0)]),
+    "external B dropped": ("  by += p.b_ext[1];\n", "\n", [check_external_b]),
     # frozen driver
     "beam R = sigma": ("double Ry = std::sqrt(2.0) * B.sigma_r;", "double Ry = B.sigma_r;",
                        [check_beam, check_bins]),

@@ -14,6 +14,7 @@
 * gather: fields bilinear in (j, k), distinct per field, axis and layer, are
   interpolated exactly (CIC) or taken at the start node (NGP), as the mean of
   the P and C layers;
+* the deck's uniform external B adds to the gathered B in the rotation;
 * sub-cycling: with the quirk fixed the final position is the straight-line
   one; the reference's literal restart (shadowed j_jump/k_jump) only matters
   when |dy| > 1 cell.
@@ -237,3 +238,34 @@ def check_gather(make, shape):
 @pytest.mark.parametrize("shape", [0, 1])
 def test_gather_bilinear_fields(oracle_mod, shape):
     check_gather(oracle_mod.Oracle, shape)
+
+
+def check_external_b(make):
+    """The uniform external field of the deck (BxExternal..BzExternal) is added
+    to the gathered B before the Boris rotation (movepart_splitlayer.cpp:
+    1271-1273): with a uniform gathered B the rotation obeys the
+    implicit-midpoint identity with t built from B + B_ext."""
+    bext = np.array([0.4, -0.9, 1.3])
+    p = small(ny=16, nz=12, nslices=2, shape=1, hx=0.05, b_ext=tuple(bext))
+    o = make(p)
+    B = np.array([0.7, -1.3, 2.1])
+    for which in (0, 1):
+        for f, v in zip(EB, [0, 0, 0, *B]):
+            o.set_field(f, np.full((p.nz, p.ny), float(v)), which=which)
+    rng = np.random.default_rng(9)
+    parts = particles(rng, p, 300, 2.0)
+    o.set_particles(*parts)
+    o.push_deposit(1)
+    d = by_id(o.particles())
+    o.close()
+    p0 = np.stack(parts[4:7], 1)
+    p1 = np.stack([d["px"], d["py"], d["pz"]], 1)
+    g = np.sqrt(1 + (p0 ** 2).sum(1))
+    vx = p0[:, 0] / g
+    t = (B + bext)[None, :] * (-1.0) * p.hx / (2 * g * (1 - vx))[:, None]
+    rhs = np.cross(p1 + p0, t)
+    assert np.abs((p1 - p0) - rhs).max() <= 1e-13 * np.abs(rhs).max(), "external B"
+
+
+def test_external_b(oracle_mod):
+    check_external_b(lambda p: oracle_mod.Oracle(p, history=()))
