@@ -334,6 +334,21 @@ int  qss_layer_record_doubles(const qss_ctx *c, size_t *n);
 int  qss_pack_layer_record(qss_ctx *c, int iLayer, double *dst);
 int  qss_unpack_layer_record(qss_ctx *c, const double *src, size_t n);
 
+/* ---- allocation-free transform ------------------------------------------ */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): qss_transform2d without a context.  out = scale *
+ * T_kind_z . in . T_kind_y for one Ny x Nz plane (y fastest) in device
+ * memory, FP64 MFMA, enqueued on `stream` (a hipStream_t of the current
+ * device; NULL = its null stream): no allocation and no synchronisation.
+ * The transform tables and the intermediate plane live in a fixed per-device
+ * scratch array of 3 x 512 x 512 doubles (6 MiB, held by each device the
+ * library runs on until the process ends), so ny, nz <= 512 (QSS_E_ARG
+ * beyond).  Every call on a device shares that scratch: calls that overlap
+ * on one device race (issue them on one stream).                           */
+int  qss_transform2d_noalloc(int ny, int nz, int kind_z, int kind_y, const double *in_dev,
+                             double *out_dev, double scale, void *stream);
+
 /* Writes history plane `slice` of field f (requires the history_mask bit);
  * the wake an evolving driver is pushed through (qss_driver_push).         */
 int  qss_set_history(qss_ctx *c, int field, int slice, const double *host);

@@ -134,3 +134,15 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run(["python", os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_noalloc_transform_checks_sizes_before_touching_a_device():
+    """qss_transform2d_noalloc refuses planes larger than its fixed scratch
+    (512 x 512) and unknown kinds with QSS_E_ARG before any HIP call."""
+    from tertia_amd import _lib
+    L = _lib.load()
+    assert L.qss_transform2d_noalloc(600, 32, 0, 0, None, None, 1.0, None) == -1
+    assert b"512" in L.qss_last_error()
+    assert L.qss_transform2d_noalloc(32, 32, 2, 0, None, None, 1.0, None) == -1
+    assert L.qss_transform2d_noalloc(32, 32, 0, 0, None, None, 1.0, None) == -1
+
