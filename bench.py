@@ -66,9 +66,17 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2028)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--lead-in", type=int, default=-1,
+                    help="untimed slices swept before --warmup, so that a short timed window "
+                         "sits inside the wake (default: up to the driver centre, slice 205, "
+                         "as far as --warmup + --steps leave room)")
     ap.add_argument("--cpu-slices", type=int, default=96,
                     help="slices of the CPU baseline sample (0: skip)")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads (default: the cores this process may run on, at "
+                         "most OMP_NUM_THREADS)")
+    ap.add_argument("--no-small-configs", action="store_true",
+                    help="skip the full 64^2 and 256^2 sweeps on the GPU and the CPU")
     ap.add_argument("--mixed", default="mixed32,bf16x6,bf16x3",
                     help="comma list of mixed modes to time and compare ('' = none)")
     ap.add_argument("--no-accuracy", action="store_true",
@@ -132,7 +140,38 @@ def profiled_traffic(name):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(params, nslices, nthreads):
+def host_cpu():
+    """CPU model, logical CPUs of the host, and the cores this process may run
+    on (its affinity mask: the share of the host a job is given)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "cores_available": avail}
+
+
+def cpu_threads(a):
+    """All the cores this process may use, capped by OMP_NUM_THREADS when the
+    job sets it (the share of a shared host that the job is given)."""
+    if a.cpu_threads:
+        return a.cpu_threads
+    n = host_cpu()["cores_available"]
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_baseline(params, nslices, nthreads, what=None):
     from oracle import oracle as O
     O.build()
     o = O.Oracle(params, use_fft=True, nthreads=nthreads, history=())
@@ -141,11 +180,45 @@ def cpu_baseline(params, nslices, nthreads):
     dt = time.perf_counter() - t0
     n = o.n_particles
     o.close()
+    cpu = host_cpu()
     return {"value": n * nslices / dt, "unit": "particle-slice pushes/s", "cores": nthreads,
-            "kind": "port",
-            "sample": "first %d slices behind the front of the same 512x512x4ppc workload "
-                      "(FP64 oracle, FFT-form DCT-IV, %d OpenMP threads, %.1f s)"
-                      % (nslices, nthreads, dt)}
+            "kind": "port", "slices_per_s": nslices / dt, "seconds": dt,
+            "cpu_model": cpu["model"], "nproc": cpu["nproc"],
+            "cores_available": cpu["cores_available"],
+            "sample": (what or "first %d slices behind the front of the same 512x512x4ppc "
+                       "workload" % nslices) +
+                      " (FP64 oracle, FFT-form DCT-IV, %d OpenMP threads, %.1f s)"
+                      % (nthreads, dt)}
+
+
+def small_configs(a):
+    """The other two single-wake FP64 workloads end to end: the 64^2 linear
+    wake (200 slices) and the 256^2 blowout (1024 slices), each swept whole on
+    the GPU (graph replay, after one untimed sweep) and by the oracle on the
+    host cores."""
+    from tertia_amd.mesh import Mesh
+    from tertia_amd.params import blowout_256, linear_wake_64
+    out = {}
+    nthr = cpu_threads(a)
+    for name, p in (("linear_wake_64", linear_wake_64()), ("blowout_256", blowout_256())):
+        m = Mesh(p, history=())
+        try:
+            m.autotune(2)
+            m.enable_graph(True)
+            m.MoveAllSplitLayers()
+            t0 = time.perf_counter()
+            m.MoveAllSplitLayers()
+            dt = time.perf_counter() - t0
+            npart = m.n_particles
+        finally:
+            m.close()
+        cpu = cpu_baseline(p, p.nslices, nthr, what="the whole %d-slice sweep" % p.nslices)
+        out[name] = {"workload": "%dx%d x 4 ppc (%d electrons/slice), %d slices, FP64"
+                                 % (p.ny, p.nz, npart, p.nslices),
+                     "gpu_pushes_per_s": npart * p.nslices / dt,
+                     "gpu_slices_per_s": p.nslices / dt, "gpu_s": dt,
+                     "cpu": cpu, "gpu_over_cpu": cpu["seconds"] / dt}
+    return out
 
 
 def timed_run(mode, a):
@@ -176,11 +249,12 @@ def timed_run(mode, a):
         def window(events, graph):
             m.enable_graph(graph)
             m.reset()
-            if a.warmup:
-                m.sweep_range(0, a.warmup)
+            first = a.lead_in + a.warmup
+            if first:
+                m.sweep_range(0, first)
             m.enable_timing(events)
             t0 = time.perf_counter()
-            res = m.sweep_range(a.warmup, a.steps)   # synchronises at the end
+            res = m.sweep_range(first, a.steps)   # synchronises at the end
             dt = time.perf_counter() - t0
             tm = m.timing() if events else None
             m.enable_timing(False)
@@ -221,16 +295,19 @@ def timed_run(mode, a):
     return out
 
 
+ACC_FIELDS = ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "Psi")
+
+
 def sampled_history(mode, every):
-    """Whole sweep (from the front) with Ex and psi stored; every `every`-th layer."""
+    """Whole sweep (from the front) storing Ex..Bz and psi; every `every`-th layer."""
     from tertia_amd.params import PRECISIONS, two_bunch_512
     from tertia_amd.mesh import Mesh
     p = two_bunch_512(precision=PRECISIONS[mode])
-    m = Mesh(p, history=("Ex", "Psi"))
+    m = Mesh(p, history=ACC_FIELDS)
     try:
         m.MoveAllSplitLayers()
-        idx = range(0, p.nslices, every)
-        return {f: np.stack([m.history(f, s) for s in idx]) for f in ("Ex", "Psi")}
+        idx = range(every - 1, p.nslices, every)
+        return {f: np.stack([m.history(f, s) for s in idx]) for f in ACC_FIELDS}
     finally:
         m.close()
 
@@ -498,8 +575,11 @@ def main():
         print(json.dumps({"driver": driver_run(a.driver_pushes or 10, sweep=False)}))
         return 0
     from tertia_amd.params import two_bunch_512
-    if a.warmup + a.steps > two_bunch_512().nslices:
-        print("warmup + steps must be <= %d slices" % two_bunch_512().nslices, file=sys.stderr)
+    ntot = two_bunch_512().nslices
+    if a.lead_in < 0:
+        a.lead_in = max(0, min(205, ntot - a.warmup - a.steps))
+    if a.lead_in + a.warmup + a.steps > ntot:
+        print("lead-in + warmup + steps must be <= %d slices" % ntot, file=sys.stderr)
         return 2
     modes = [s for s in a.mixed.split(",") if s]
 
@@ -516,7 +596,15 @@ def main():
             h = sampled_history(mode, 8)
             mixed[mode]["rel_l2_vs_fp64"] = {
                 f: float(np.linalg.norm(h[f] - ref[f]) / np.linalg.norm(ref[f]))
-                for f in ("Ex", "Psi")}
+                for f in ACC_FIELDS}
+            mixed[mode]["max_err_vs_fp64"] = {
+                f: float(np.abs(h[f] - ref[f]).max() / np.abs(ref[f]).max())
+                for f in ACC_FIELDS}
+            mixed[mode]["accuracy_note"] = (
+                "whole 2048-slice sweep, every 8th layer, against the FP64 GPU sweep (which "
+                "matches the FP64 oracle to rel-L2 <= 2e-12 through slice 704, "
+                "tests/test_gpu_two_bunch_512.py); max error relative to the field's max")
+            del h
             print("%s accuracy %s" % (mode, mixed[mode]["rel_l2_vs_fp64"]), file=sys.stderr,
                   flush=True)
     for mode, r in mixed.items():
@@ -544,6 +632,12 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded plasma lattice + frozen bi-Gaussian driver/witness)",
         "config": {"workload": WORKLOAD, "slices_per_s": base["slices_per_s"],
+                   "timed_slices": [a.lead_in + a.warmup, a.lead_in + a.warmup + a.steps],
+                   "timed_xi": [round((a.lead_in + a.warmup + 1) * 0.01, 2),
+                                round((a.lead_in + a.warmup + a.steps) * 0.01, 2)],
+                   "window_note": "%d untimed lead-in + %d warm-up slices from the front, "
+                                  "then the timed slices (the driver centre is at xi = 2.1)"
+                                  % (a.lead_in, a.warmup),
                    "gemm_tiles": base["gemm_tiles"], "max_residual": base["max_residual"]},
     }
     if "gemm" in base:
@@ -578,8 +672,13 @@ def main():
         line["adjoint"] = adjoint_run(a.fd_bins)
         print("adjoint/forward %.2f" % line["adjoint"]["adjoint_over_forward"], file=sys.stderr,
               flush=True)
+    if not a.no_small_configs:
+        line["small_configs"] = small_configs(a)
+        print("small configs %s" % {k: round(v["gpu_over_cpu"], 1)
+                                    for k, v in line["small_configs"].items()},
+              file=sys.stderr, flush=True)
     if a.cpu_slices > 0:
-        nthr = a.cpu_threads or min(16, os.cpu_count() or 1)
+        nthr = cpu_threads(a)
         line["cpu_baseline"] = cpu_baseline(two_bunch_512(), a.cpu_slices, nthr)
         # one core, as the reference runs one rank (FFTW with at most 2 threads)
         line["cpu_baseline_1core"] = cpu_baseline(two_bunch_512(), max(1, a.cpu_slices // 12),
