@@ -76,6 +76,8 @@ __global__ __launch_bounds__(NT) void k_push_tile_adj(AdjTileArgs a) {
 
   qss_adj::Consts cs;
   cs.hx = a.hx; cs.hy = a.hy; cs.hz = a.hz; cs.weight = a.weight; cs.ny = ny; cs.nz = nz;
+  cs.hxy = a.hx / a.hy;   // the forward push's hx / hy, hx / hz
+  cs.hxz = a.hx / a.hz;
   for (int q = 0; q < 3; ++q) cs.bext[q] = a.bext[q];
   const long np = a.np;
   const int base = a.tile_start[tile];

@@ -81,6 +81,7 @@ struct PushArgs {
   const double *F;
   int ny, nz;
   double hx, hy, hz, weight;
+  double hxy, hxz;  // hx / hy, hx / hz
   double bext[3];   // uniform external B (movepart_splitlayer.cpp:1271-1273)
   int shape, fix_sub, full;
   int32_t *ev_cell, *ev_rank, *ev_sub;
@@ -112,10 +113,12 @@ __global__ __launch_bounds__(NT) void k_push(PushArgs a) {
   if (!(yp >= 0 && yp <= 1 && zp >= 0 && zp <= 1)) atomicOr(&a.counters[1], ST_PARTICLE);
   double px = a.px[n], py = a.py[n], pz = a.pz[n];
   const double gammap = sqrt(1. + px * px + py * py + pz * pz);
-  double Vx = px / gammap;
+  // one reciprocal per denominator, then products (the tiled push's arithmetic)
+  const double igp = 1. / gammap;
+  double Vx = px * igp;
   const double Vxp = Vx;
-  const double Vyp = py / gammap;
-  const double Vzp = pz / gammap;
+  const double Vyp = py * igp;
+  const double Vzp = pz * igp;
   double f[6];
   if (a.shape == QSS_SHAPE_NGP_REF) {
     gather6(a.F, (long)ny * nz, c0, f);
@@ -137,17 +140,20 @@ __global__ __launch_bounds__(NT) void k_push(PushArgs a) {
   ex *= q2m * hx / 2.;
   ey *= q2m * hx / 2.;
   ez *= q2m * hx / 2.;
-  px += ex / (1. - Vx);
-  py += ey / (1. - Vx);
-  pz += ez / (1. - Vx);
+  const double iu = 1. / (1. - Vx);
+  const double kx = ex * iu, ky = ey * iu, kz = ez * iu;
+  px += kx;
+  py += ky;
+  pz += kz;
   double gamma = sqrt(1. + px * px + py * py + pz * pz);
   const double gamma_r = 1. / gamma;
   bx += a.bext[0];
   by += a.bext[1];
   bz += a.bext[2];
-  bx = bx * gamma_r * q2m / (1. - Vx) * hx / 2.;
-  by = by * gamma_r * q2m / (1. - Vx) * hx / 2.;
-  bz = bz * gamma_r * q2m / (1. - Vx) * hx / 2.;
+  const double beta = gamma_r * q2m * iu * (0.5 * hx);
+  bx = bx * beta;
+  by = by * beta;
+  bz = bz * beta;
   const double co = 2. / (1. + (bx * bx) + (by * by) + (bz * bz));
   double p3x = py * bz - pz * by + px;
   double p3y = pz * bx - px * bz + py;
@@ -158,28 +164,24 @@ __global__ __launch_bounds__(NT) void k_push(PushArgs a) {
   const double px_new = p3y * bz - p3z * by;
   const double py_new = p3z * bx - p3x * bz;
   const double pz_new = p3x * by - p3y * bx;
-  px += ex / (1. - Vx) + px_new;
-  py += ey / (1. - Vx) + py_new;
-  pz += ez / (1. - Vx) + pz_new;
+  px += kx + px_new;
+  py += ky + py_new;
+  pz += kz + pz_new;
   gamma = sqrt(1. + px * px + py * py + pz * pz);
-  const double Vxm = px / gamma;
-  const double Vym = py / gamma;
-  const double Vzm = pz / gamma;
+  const double ig = 1. / gamma;
+  const double Vxm = px * ig;
+  const double Vym = py * ig;
+  const double Vzm = pz * ig;
   Vx = 0.5 * (Vxm + Vxp);
   const double Vy = 0.5 * (Vym + Vyp);
   const double Vz = 0.5 * (Vzm + Vzp);
-  double djx = weight * djx0 * Vxm;
-  double djy = weight * djy0 * Vym;
-  double djz = weight * djz0 * Vzm;
-  double drho = weight * drho0;
-  djx *= 1. / (1. - Vx);
-  djy *= 1. / (1. - Vx);
-  djz *= 1. / (1. - Vx);
-  drho *= 1. / (1. - Vx);
-  double dy = Vy * hx / hy;
-  double dz = Vz * hx / hz;
-  dy = dy / (1. - Vx);
-  dz = dz / (1. - Vx);
+  const double iv = 1. / (1. - Vx);
+  double djx = weight * djx0 * Vxm * iv;
+  double djy = weight * djy0 * Vym * iv;
+  double djz = weight * djz0 * Vzm * iv;
+  double drho = weight * drho0 * iv;
+  double dy = Vy * a.hxy * iv;
+  double dz = Vz * a.hxz * iv;
   // sub-cycling, movepart_splitlayer.cpp:1380-1466
   double partdx = 0., step = 1.;
   if (fabs(dy) > 1. || fabs(dz) > 1.) {
@@ -484,6 +486,7 @@ hipError_t launch_push(qss_ctx *c, int full_step) {
   a.F = c->F;
   a.ny = c->ny; a.nz = c->nz;
   a.hx = c->hxs; a.hy = c->hy; a.hz = c->hz; a.weight = c->pweight;
+  a.hxy = c->hxs / c->hy; a.hxz = c->hxs / c->hz;
   for (int q = 0; q < 3; ++q) a.bext[q] = c->p.b_ext[q];
   a.shape = c->p.shape; a.fix_sub = c->p.fix_subcycle; a.full = full_step;
   a.ev_cell = c->ev_cell; a.ev_rank = c->ev_rank; a.ev_sub = c->ev_sub;

@@ -40,6 +40,7 @@ constexpr double kQ2m = -1.0, kDj0 = -1.0, kDrho0 = -1.0;
 
 struct Consts {
   double hx, hy, hz, weight;
+  double hxy, hxz;  // hx / hy, hx / hz
   double bext[3];   // uniform external B, added to the gathered B
   int ny, nz;
 };
@@ -66,22 +67,25 @@ struct Kin {
 };
 
 // Momentum update and sub-step control; the expressions and their order
-// follow the push kernel, so a replay reproduces its sub-steps.
+// follow the push kernel (one reciprocal per denominator, then products), so
+// a replay reproduces its sub-steps bit for bit.
 QSS_HD inline void kinematics(const Consts &c, const double P0[3], const double f[6], Kin &k) {
   const double q2m = kQ2m, hx = c.hx;
   for (int q = 0; q < 3; ++q) k.P0[q] = P0[q];
   const double px = P0[0], py = P0[1], pz = P0[2];
   k.g0 = sqrt(1. + px * px + py * py + pz * pz);
-  k.Vp[0] = px / k.g0; k.Vp[1] = py / k.g0; k.Vp[2] = pz / k.g0;
+  const double ig0 = 1. / k.g0;
+  k.Vp[0] = px * ig0; k.Vp[1] = py * ig0; k.Vp[2] = pz * ig0;
   for (int q = 0; q < 3; ++q) { k.fE[q] = f[q]; k.fB[q] = f[3 + q] + c.bext[q]; }
   for (int q = 0; q < 3; ++q) k.e[q] = f[q] * (q2m * hx / 2.);
   k.u = 1. - k.Vp[0];
-  for (int q = 0; q < 3; ++q) k.K[q] = k.e[q] / k.u;
+  const double iu = 1. / k.u;
+  for (int q = 0; q < 3; ++q) k.K[q] = k.e[q] * iu;
   for (int q = 0; q < 3; ++q) k.H[q] = P0[q] + k.K[q];
   k.g1 = sqrt(1. + k.H[0] * k.H[0] + k.H[1] * k.H[1] + k.H[2] * k.H[2]);
   const double gr = 1. / k.g1;
-  for (int q = 0; q < 3; ++q) k.b[q] = k.fB[q] * gr * q2m / k.u * hx / 2.;
-  k.beta = gr * q2m / k.u * hx / 2.;
+  k.beta = gr * q2m * iu * (0.5 * hx);
+  for (int q = 0; q < 3; ++q) k.b[q] = k.fB[q] * k.beta;
   const double *b = k.b, *H = k.H;
   k.co = 2. / (1. + (b[0] * b[0]) + (b[1] * b[1]) + (b[2] * b[2]));
   k.Q[0] = H[1] * b[2] - H[2] * b[1] + H[0];
@@ -93,19 +97,15 @@ QSS_HD inline void kinematics(const Consts &c, const double P0[3], const double 
   k.R[2] = k.p3[0] * b[1] - k.p3[1] * b[0];
   for (int q = 0; q < 3; ++q) k.P1[q] = H[q] + (k.K[q] + k.R[q]);
   k.g2 = sqrt(1. + k.P1[0] * k.P1[0] + k.P1[1] * k.P1[1] + k.P1[2] * k.P1[2]);
-  for (int q = 0; q < 3; ++q) k.Vm[q] = k.P1[q] / k.g2;
+  const double ig2 = 1. / k.g2;
+  for (int q = 0; q < 3; ++q) k.Vm[q] = k.P1[q] * ig2;
   for (int q = 0; q < 3; ++q) k.Vb[q] = 0.5 * (k.Vm[q] + k.Vp[q]);
   k.v = 1. - k.Vb[0];
+  const double iv = 1. / k.v;
   const double wdj = c.weight * kDj0;
-  for (int q = 0; q < 3; ++q) {
-    k.dj[q] = wdj * k.Vm[q];
-    k.dj[q] *= 1. / k.v;
-  }
-  k.dj[3] = c.weight * kDrho0;
-  k.dj[3] *= 1. / k.v;
-  double dy = k.Vb[1] * hx / c.hy, dz = k.Vb[2] * hx / c.hz;
-  dy = dy / k.v;
-  dz = dz / k.v;
+  for (int q = 0; q < 3; ++q) k.dj[q] = wdj * k.Vm[q] * iv;
+  k.dj[3] = c.weight * kDrho0 * iv;
+  double dy = k.Vb[1] * c.hxy * iv, dz = k.Vb[2] * c.hxz * iv;
   double step = 1., partdx = 0.;
   if (fabs(dy) > 1. || fabs(dz) > 1.) {
     if (fabs(dy) > fabs(dz)) step = partdx = fabs(dy);

@@ -57,6 +57,7 @@ struct TileArgs {
   const double *F;
   const float *Ff;        // fp32 push field (mixed modes)
   double hx, hy, hz, weight;
+  double hxy, hxz;        // hx / hy, hx / hz
   double bext[3];         // uniform external B (movepart_splitlayer.cpp:1271-1273)
   int shape, full;
   double *win;            // ntiles x NWIN x 4
@@ -176,8 +177,12 @@ void k_push_tile(TileArgs a0) {
       px = a0.px[gs]; py = a0.py[gs]; pz = a0.pz[gs];
       const R rpx = (R)px, rpy = (R)py, rpz = (R)pz;
       const R gammap = sqrt(R(1) + rpx * rpx + rpy * rpy + rpz * rpz);
-      R Vx = rpx / gammap;
-      const R Vxp = Vx, Vyp = rpy / gammap, Vzp = rpz / gammap;
+      // one correctly rounded reciprocal per denominator, then products: the
+      // reference divides by the same gamma and 1 - Vx many times over
+      // (movepart_splitlayer.cpp:1239-1330); results agree to an ulp per step
+      const R igp = R(1) / gammap;
+      R Vx = rpx * igp;
+      const R Vxp = Vx, Vyp = rpy * igp, Vzp = rpz * igp;
       // gather from the staged window: local node (j - j0, k - k0)
       int lj = j - j0, lk = k - k0;
       if (lj < 0 || lj >= T || lk < 0 || lk >= T) {   // not in this tile: never expected
@@ -203,7 +208,8 @@ void k_push_tile(TileArgs a0) {
       ey *= R(q2m * hx / 2.);
       ez *= R(q2m * hx / 2.);
       // half kick, carried in the double state
-      const R kx = ex / (R(1) - Vx), ky = ey / (R(1) - Vx), kz = ez / (R(1) - Vx);
+      const R iu = R(1) / (R(1) - Vx);
+      const R kx = ex * iu, ky = ey * iu, kz = ez * iu;
       px += (double)kx;
       py += (double)ky;
       pz += (double)kz;
@@ -213,9 +219,10 @@ void k_push_tile(TileArgs a0) {
       bx += R(a.bext[0]);
       by += R(a.bext[1]);
       bz += R(a.bext[2]);
-      bx = bx * gamma_r * R(q2m) / (R(1) - Vx) * R(hx) / R(2);
-      by = by * gamma_r * R(q2m) / (R(1) - Vx) * R(hx) / R(2);
-      bz = bz * gamma_r * R(q2m) / (R(1) - Vx) * R(hx) / R(2);
+      const R beta = gamma_r * R(q2m) * iu * R(0.5 * hx);
+      bx = bx * beta;
+      by = by * beta;
+      bz = bz * beta;
       const R co = R(2) / (R(1) + (bx * bx) + (by * by) + (bz * bz));
       R p3x = hpy * bz - hpz * by + hpx;
       R p3y = hpz * bx - hpx * bz + hpy;
@@ -231,21 +238,17 @@ void k_push_tile(TileArgs a0) {
       pz += (double)(kz + pz_new);
       const R npx = (R)px, npy = (R)py, npz = (R)pz;
       gamma = sqrt(R(1) + npx * npx + npy * npy + npz * npz);
-      const R Vxm = npx / gamma, Vym = npy / gamma, Vzm = npz / gamma;
+      const R ig = R(1) / gamma;
+      const R Vxm = npx * ig, Vym = npy * ig, Vzm = npz * ig;
       Vx = R(0.5) * (Vxm + Vxp);
       const R Vy = R(0.5) * (Vym + Vyp), Vz = R(0.5) * (Vzm + Vzp);
-      djx = R(weight * djx0) * Vxm;
-      djy = R(weight * djy0) * Vym;
-      djz = R(weight * djz0) * Vzm;
-      drho = R(weight * drho0);
-      djx *= R(1) / (R(1) - Vx);
-      djy *= R(1) / (R(1) - Vx);
-      djz *= R(1) / (R(1) - Vx);
-      drho *= R(1) / (R(1) - Vx);
-      dy = Vy * R(hx) / R(hy);
-      dz = Vz * R(hx) / R(hz);
-      dy = dy / (R(1) - Vx);
-      dz = dz / (R(1) - Vx);
+      const R iv = R(1) / (R(1) - Vx);
+      djx = R(weight * djx0) * Vxm * iv;
+      djy = R(weight * djy0) * Vym * iv;
+      djz = R(weight * djz0) * Vzm * iv;
+      drho = R(weight * drho0) * iv;
+      dy = Vy * R(a.hxy) * iv;
+      dz = Vz * R(a.hxz) * iv;
       if (fabs(dy) > R(1) || fabs(dz) > R(1)) {
         if (fabs(dy) > fabs(dz)) step = partdx = fabs(dy);
         else step = partdx = fabs(dz);
@@ -649,6 +652,7 @@ TileArgs make_args(qss_ctx *c, int full) {
   a.F = c->F;
   a.Ff = c->Ff;
   a.hx = c->hxs; a.hy = c->hy; a.hz = c->hz; a.weight = c->pweight;
+  a.hxy = c->hxs / c->hy; a.hxz = c->hxs / c->hz;
   for (int q = 0; q < 3; ++q) a.bext[q] = c->p.b_ext[q];
   a.shape = c->p.shape; a.full = full;
   a.win = c->win;

@@ -79,6 +79,8 @@ int main(int argc, char **argv) {
     cs.c.hx = 0.05;
     cs.c.hy = cs.c.hz = (t % 3 == 0) ? 0.01 : 0.1;   // small cells: sub-cycling
     cs.c.weight = 0.25;
+    cs.c.hxy = cs.c.hx / cs.c.hy;
+    cs.c.hxz = cs.c.hx / cs.c.hz;
     for (int q = 0; q < 3; ++q) cs.c.bext[q] = (t % 5 == 2) ? 1.5 * U(rng) : 0.0;
     cs.c.ny = NY;
     cs.c.nz = NZ;

@@ -5,10 +5,12 @@ Tolerances (FP64 both sides):
   * beam planes: max-rel 1e-14 (same sums, same order; exp may differ by 1 ulp)
   * Guess/Iterate: max-rel 1e-12 per spectral/real plane (inputs differ only
     by transform rounding, ~1e-15)
-  * push + deposit: max-rel 1e-13 on J, rho (same arithmetic with FMA
-    contraction off on both sides; the sums over a node's contributions are
-    associated differently: the GPU adds per tile window, then windows) and
-    1e-14 on the pushed particles
+  * push + deposit: max-rel 1e-13 on J, rho (the reference's operation
+    order with FMA contraction off on both sides, except that the GPU takes
+    one reciprocal per denominator where the reference divides repeatedly;
+    the sums over a node's contributions are associated differently: the GPU
+    adds per tile window, then windows) and, on the pushed particles, 1e-14
+    of the momenta's max and 1e-14 x N on the in-cell fractions
 """
 import numpy as np
 import pytest
@@ -160,8 +162,19 @@ def _push_deposit(oracle_mod, shape, fix_sub, ny, nz, b_ext=(0.0, 0.0, 0.0)):
     assert np.array_equal(po["ck"][oo], pg["ck"][og])
     for k in ("fy", "fz", "px", "py", "pz"):
         d = np.abs(po[k][oo] - pg[k][og]).max()
-        assert d <= 1e-14 * max(1.0, np.abs(po[k]).max()), k
+        assert d <= _ptol(k, po, p), (k, d)
     m.close()
+
+
+def _ptol(k, po, p):
+    """Particle bar: momenta 1e-14 of their max; in-cell fractions 1e-14 of
+    the grid size, since a fraction is what remains of a global position of up
+    to N cells (y + dy + j, wrapped) and inherits that coordinate's rounding.
+    The push takes one reciprocal per denominator where the reference divides
+    repeatedly, so positions and momenta agree to a few ulps, not bitwise."""
+    if k in ("fy", "fz"):
+        return 1e-14 * max(p.ny, p.nz)
+    return 1e-14 * max(1.0, float(np.abs(po[k]).max()))
 
 
 def test_push_error_paths(oracle_mod):
@@ -224,7 +237,7 @@ def test_push_deposit_particle_split(oracle_mod, nsplit):
         assert np.array_equal(po["id"][oo], pg["id"][og])
         for k in ("fy", "fz", "px", "py", "pz"):
             d = np.abs(po[k][oo] - pg[k][og]).max()
-            assert d <= 1e-14 * max(1.0, np.abs(po[k]).max()), k
+            assert d <= _ptol(k, po, p), (k, d)
         runs.append((got, pg))
         m.close()
     for a, b in zip(runs[0][0], runs[1][0]):
