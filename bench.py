@@ -97,6 +97,9 @@ def parse():
                     help="FP64 transform tile variants per batch size 1..8 for the y and z "
                          "passes, e.g. 11,12,11,13,11,14,0,0/11,12,12,13,11,15,0,2 "
                          "(default: the autotuner's choice)")
+    ap.add_argument("--transform", default="auto", choices=("auto", "dense", "factored"),
+                    help="2-D transform method of the FP64 sweeps (auto: the context's "
+                         "default, factored on grids of at least 256^2)")
     ap.add_argument("--particle-split", type=int, default=0, choices=(0, 1, 2, 4),
                     help="work-groups per tile of the push (0: the autotuner's choice)")
     ap.add_argument("--library-gemm", action="store_true",
@@ -232,6 +235,8 @@ def timed_run(mode, a):
     m = Mesh(p, history=())
     out = {}
     try:
+        if a.transform != "auto" and mode == "fp64":
+            m.transform_method = 1 if a.transform == "factored" else 0
         choice = m.autotune(3)
         if a.gemm_tiles and mode == "fp64":
             # a fixed tile table (y-pass and z-pass variant per batch size 1..8)

@@ -334,6 +334,24 @@ int  qss_layer_record_doubles(const qss_ctx *c, size_t *n);
 int  qss_pack_layer_record(qss_ctx *c, int iLayer, double *dst);
 int  qss_unpack_layer_record(qss_ctx *c, const double *src, size_t n);
 
+/* ---- transform method ------------------------------------------------------ */
+/* No counterpart in the reference (whose FFTW plans pick their own
+ * algorithm).  DENSE: each 1-D pass is one GEMM with the N x N half-integer
+ * table (the reference's HalfInteger matrix form, fft_matrix.cu:10-24).
+ * FACTORED: one radix-2 step of the DCT-IV's half-length complex DFT turns
+ * each pass into an (N/2 x N/2) GEMM applied to both halves of every row,
+ * half the multiply-adds, between twiddle kernels (FP64 contexts with ny,
+ * nz multiples of 8 and at least 256).  Contexts start DENSE.  Results
+ * agree to rounding.  Setting a method drops a captured graph.              */
+#define QSS_TRANSFORM_DENSE    0
+#define QSS_TRANSFORM_FACTORED 1
+int  qss_set_transform_method(qss_ctx *c, int method);
+int  qss_transform_method(const qss_ctx *c);
+/* The FACTORED method's GEMM tile shapes (qss_autotune tunes them; layout of
+ * qss_gemm_choice, FP64 variants).                                         */
+int  qss_fact_choice(const qss_ctx *c, int *chosen);
+int  qss_set_fact_choice(qss_ctx *c, const int *chosen);
+
 /* ---- allocation-free transform ------------------------------------------ */
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

@@ -49,6 +49,10 @@ SIGNATURES = {
     "qss_layer_state_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
     "qss_pack_layer": (ctypes.c_int, [_vp, _vp]),
     "qss_unpack_layer": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    "qss_set_transform_method": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "qss_transform_method": (ctypes.c_int, [_vp]),
+    "qss_fact_choice": (ctypes.c_int, [_vp, _ip]),
+    "qss_set_fact_choice": (ctypes.c_int, [_vp, _ip]),
     "qss_transform2d_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int, _vp, _vp, ctypes.c_double, _vp]),
     "qss_layer_record_doubles": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),

@@ -175,6 +175,69 @@ inline int gemm_pick(const qss_ctx *c, int pass, int n) {
   return c->gemm_force >= 0 ? c->gemm_force : c->gemm_var[pass][n];
 }
 
+// The factorized form of transform_batch (fact.hip): pre-twiddle, GEMM with
+// G_y, butterflies + z pre-twiddle, GEMM with G_z, butterflies + epilogue.
+int transform_fact(qss_ctx *c, int n, const int *kz, const int *ky, const double *const *in,
+                   double *const *out, double alpha, const int *resid, const int *push,
+                   const double *const *pplane, double beta, bool one_wake) {
+  const long nc = c->ncell;
+  FactArgs f{};
+  f.ny = c->ny; f.nz = c->nz; f.nbatch = n;
+  f.nw = one_wake ? 1 : c->nw;
+  f.w_in = nc; f.w_out = nc; f.w_p = nc; f.w_f = nc * kPushComp; f.w_r = kResidPerWake;
+  f.twy = c->d_tw[0]; f.twz = c->d_tw[1];
+  f.alpha = alpha; f.beta = beta;
+  f.F = c->F; f.Ff = c->Ff; f.rslots = c->d_resid;
+  for (int b = 0; b < n; ++b) {
+    f.in[b] = in[b]; f.out[b] = out[b];
+    f.P[b] = pplane ? pplane[b] : nullptr;
+    f.kz[b] = kz[b]; f.ky[b] = ky[b];
+    f.resid_c[b] = resid ? resid[b] : -1;
+    f.push[b] = push ? push[b] : -1;
+    f.t0[b] = c->tmp[b]; f.t1[b] = c->tmp2[b];
+  }
+  {
+    Timed t(c, KC_SPEC);
+    HIPCHECK(launch_fact(f, 0, c->stream));
+  }
+  GemmArgs g{};
+  g.nbatch = n;
+  g.nw = f.nw;
+  g.alpha = 1.0;
+  g.wA = nc; g.wB = 0; g.wC = nc;
+  g.M = 2 * c->nz; g.N = c->ny / 2; g.K = c->ny / 2;
+  for (int b = 0; b < n; ++b) {
+    g.b[b].A = c->tmp[b]; g.b[b].B = c->d_G[0]; g.b[b].C = c->tmp2[b];
+    g.b[b].resid = -1; g.b[b].push = -1;
+  }
+  {
+    Timed t(c, KC_GEMM, true);
+    HIPCHECK(launch_gemm(g, c->stream, QSS_PREC_FP64,
+                           c->gemm_force >= 0 ? c->gemm_force : c->fact_var[0][n], t.e0, t.e1));
+  }
+  if (c->timing) c->gemm_flops += 2.0 * g.M * g.N * g.K * n * g.nw;
+  {
+    Timed t(c, KC_SPEC);
+    HIPCHECK(launch_fact(f, 1, c->stream));
+  }
+  g.wA = 0; g.wB = nc; g.wC = nc;
+  g.M = c->nz / 2; g.N = 2 * c->ny; g.K = c->nz / 2;
+  for (int b = 0; b < n; ++b) {
+    g.b[b].A = c->d_G[1]; g.b[b].B = c->tmp[b]; g.b[b].C = c->tmp2[b];
+  }
+  {
+    Timed t(c, KC_GEMM, true);
+    HIPCHECK(launch_gemm(g, c->stream, QSS_PREC_FP64,
+                           c->gemm_force >= 0 ? c->gemm_force : c->fact_var[1][n], t.e0, t.e1));
+  }
+  if (c->timing) c->gemm_flops += 2.0 * g.M * g.N * g.K * n * g.nw;
+  {
+    Timed t(c, KC_SPEC);
+    HIPCHECK(launch_fact(f, 2, c->stream));
+  }
+  return QSS_OK;
+}
+
 // out_b = alpha * T_kz[b] . in_b . T_ky[b]: y-pass into tmp, then z-pass
 // reuse: the split-bf16 operands may come from the producer of the inputs
 // (c->xs_src records which planes xs holds); false for caller-owned planes
@@ -186,6 +249,8 @@ int transform_batch(qss_ctx *c, int n, const int *kz, const int *ky,
     qss_set_error("transform batch size");
     return QSS_E_ARG;
   }
+  if (c->fact_use && !c->npc && !c->mixed)
+    return transform_fact(c, n, kz, ky, in, out, alpha, resid, push, pplane, beta, one_wake);
   GemmArgs g{};
   g.M = c->nz; g.N = c->ny; g.K = c->ny;
   g.alpha = 1.0;
@@ -927,6 +992,32 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
     if ((r = dalloc(&c->S[s], ncw)) != QSS_OK) return fail(r);
   for (int b = 0; b < kGemmMaxBatch; ++b)
     if ((r = dalloc(&c->tmp[b], ncw)) != QSS_OK) return fail(r);
+  // factorized transforms (fact.hip) for large FP64 grids
+  c->fact_ok = p->precision == QSS_PREC_FP64 && c->ny % 8 == 0 && c->nz % 8 == 0 &&
+               c->ny >= kFactMinN && c->nz >= kFactMinN;
+  // dense by default: at 512^2 the factored form's GEMMs take 17.1 against
+  // 27.4 us per launch, but its three twiddle kernels per transform cost more
+  // than that saves (measured 0.718 against 0.705 ms per layer step)
+  c->fact_use = false;
+  if (c->fact_ok) {
+    for (int b = 0; b < kGemmMaxBatch; ++b)
+      if ((r = dalloc(&c->tmp2[b], ncw)) != QSS_OK) return fail(r);
+    for (int i = 0; i < 2; ++i) {
+      const int n = i == 0 ? c->ny : c->nz;
+      std::vector<double> g, tw;
+      fact_matrix(n, i == 1, g);
+      fact_twiddles(n, tw);
+      double *twd = nullptr;
+      if ((r = dalloc(&c->d_G[i], g.size())) != QSS_OK || (r = dalloc(&twd, tw.size())) != QSS_OK)
+        return fail(r);
+      c->d_tw[i] = reinterpret_cast<double2 *>(twd);
+      if (hipMemcpy(c->d_G[i], g.data(), g.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(twd, tw.data(), tw.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+        qss_set_error("factorized transform table upload failed");
+        return fail(QSS_E_HIP);
+      }
+    }
+  }
   if ((r = dalloc(&c->F, ncw * kPushComp)) != QSS_OK) return fail(r);
   if (c->mixed && (r = dalloc(&c->Ff, ncw * kPushComp)) != QSS_OK) return fail(r);
   if (nw > 1) {
@@ -1029,6 +1120,8 @@ int qss_destroy(qss_ctx *c) {
   for (int f = 0; f < QSS_NFIELD; ++f) { dfree(c->C[f]); dfree(c->P[f]); dfree(c->hist[f]); }
   for (auto &p : c->S) dfree(p);
   for (auto &p : c->tmp) dfree(p);
+  for (auto &p : c->tmp2) dfree(p);
+  dfree(c->d_G[0]); dfree(c->d_G[1]); dfree(c->d_tw[0]); dfree(c->d_tw[1]);
   dfree(c->F); dfree(c->Ff); dfree(c->d_bins); dfree(c->d_wbunch);
   for (auto &p : c->Ts) dfree(p);
   for (auto &p : c->xs) dfree(p);
@@ -1520,6 +1613,26 @@ int qss_set_particles(qss_ctx *c, int64_t n, const int32_t *cell_j, const int32_
   return QSS_OK;
 }
 
+int qss_set_transform_method(qss_ctx *c, int method) {
+  if (!c || (method != QSS_TRANSFORM_DENSE && method != QSS_TRANSFORM_FACTORED)) {
+    qss_set_error("unknown transform method");
+    return QSS_E_ARG;
+  }
+  if (method == QSS_TRANSFORM_FACTORED && !c->fact_ok) {
+    qss_set_error("the factored transform needs FP64 and ny, nz multiples of 8, at least " +
+                  std::to_string(kFactMinN));
+    return QSS_E_ARG;
+  }
+  graph_free(c);   // a captured graph holds the other method's launches
+  c->fact_use = method == QSS_TRANSFORM_FACTORED;
+  return QSS_OK;
+}
+
+int qss_transform_method(const qss_ctx *c) {
+  if (!c) return QSS_E_ARG;
+  return c->fact_use ? QSS_TRANSFORM_FACTORED : QSS_TRANSFORM_DENSE;
+}
+
 int qss_gemm_variants(int precision) {
   if (precision < QSS_PREC_FP64 || precision > QSS_PREC_MIXED_BF16X6) return QSS_E_ARG;
   return gemm_num_variants(precision);
@@ -1537,6 +1650,29 @@ int qss_gemm_choice(const qss_ctx *c, int *chosen) {
   for (int p = 0; p < 2; ++p)
     for (int b = 0; b <= kGemmMaxBatch; ++b)
       chosen[p * (kGemmMaxBatch + 1) + b] = c->gemm_var[p][b];
+  return QSS_OK;
+}
+
+int qss_fact_choice(const qss_ctx *c, int *chosen) {
+  if (!c || !chosen) return QSS_E_ARG;
+  for (int p = 0; p < 2; ++p)
+    for (int b = 0; b <= kGemmMaxBatch; ++b)
+      chosen[p * (kGemmMaxBatch + 1) + b] = c->fact_var[p][b];
+  return QSS_OK;
+}
+
+int qss_set_fact_choice(qss_ctx *c, const int *chosen) {
+  if (!c || !chosen) return QSS_E_ARG;
+  graph_free(c);
+  const int nv = gemm_num_variants(QSS_PREC_FP64);
+  for (int i = 0; i < 2 * (kGemmMaxBatch + 1); ++i)
+    if (chosen[i] < 0 || chosen[i] >= nv) {
+      qss_set_error("factored gemm choice out of range");
+      return QSS_E_ARG;
+    }
+  for (int p = 0; p < 2; ++p)
+    for (int b = 0; b <= kGemmMaxBatch; ++b)
+      c->fact_var[p][b] = chosen[p * (kGemmMaxBatch + 1) + b];
   return QSS_OK;
 }
 
@@ -1617,6 +1753,42 @@ int qss_autotune(qss_ctx *c, int reps, float *table_ms) {
         if (ms < best) { best = ms; bestv = v; }
       }
       c->gemm_var[pass][b] = bestv;
+    }
+  // the factorized transform's two GEMM shapes (no epilogue: it runs in the
+  // post kernel), per pass and batch size
+  for (int pass = 0; pass < 2 && rc == QSS_OK && c->fact_ok; ++pass)
+    for (int b = 1; b <= kGemmMaxBatch && rc == QSS_OK; ++b) {
+      GemmArgs g{};
+      g.alpha = 1.0; g.nbatch = b; g.nw = c->nw;
+      if (pass == 0) {
+        g.M = 2 * c->nz; g.N = c->ny / 2; g.K = c->ny / 2;
+        g.wA = c->ncell; g.wB = 0;
+      } else {
+        g.M = c->nz / 2; g.N = 2 * c->ny; g.K = c->nz / 2;
+        g.wA = 0; g.wB = c->ncell;
+      }
+      g.wC = c->ncell;
+      for (int q = 0; q < b; ++q) {
+        g.b[q].A = pass == 0 ? c->S[q] : c->d_G[1];
+        g.b[q].B = pass == 0 ? c->d_G[0] : c->S[q];
+        g.b[q].C = c->tmp2[q];
+        g.b[q].resid = -1;
+        g.b[q].push = -1;
+      }
+      float best = 1e30f;
+      int bestv = 0;
+      const int nv64 = gemm_num_variants(QSS_PREC_FP64);
+      for (int v = 0; v < nv64; ++v) {
+        if (launch_gemm(g, c->stream, QSS_PREC_FP64, v) != hipSuccess) { rc = QSS_E_HIP; break; }
+        (void)hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps; ++r) (void)launch_gemm(g, c->stream, QSS_PREC_FP64, v);
+        (void)hipEventRecord(e1, c->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) { rc = QSS_E_HIP; break; }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) { best = ms; bestv = v; }
+      }
+      c->fact_var[pass][b] = bestv;
     }
   // work-groups per tile of the push: one predictor pass (state unchanged)
   // plus the window combine into scratch planes, per candidate

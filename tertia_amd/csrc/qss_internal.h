@@ -78,6 +78,36 @@ hipError_t launch_split_bf16(const double *const *in, uint16_t *const *out, int 
                              hipEvent_t ev1 = nullptr);
 
 // ---------------------------------------------------------------------------
+// factorized transform (tertia_amd/csrc/fact.hip): one radix-2 step of the
+// DCT-IV's half-length DFT, two half-size GEMMs per 2-D transform
+// ---------------------------------------------------------------------------
+struct FactArgs {
+  int ny, nz, nbatch, nw;
+  const double *in[kGemmMaxBatch];
+  double *out[kGemmMaxBatch];
+  const double *P[kGemmMaxBatch];    // P-layer planes of the push-field epilogue
+  long w_in, w_out, w_p;             // wake strides (0: shared)
+  int kz[kGemmMaxBatch], ky[kGemmMaxBatch];
+  int resid_c[kGemmMaxBatch], push[kGemmMaxBatch];   // -1: none
+  double *t0[kGemmMaxBatch], *t1[kGemmMaxBatch];    // intermediates (per wake: ny*nz)
+  const double2 *twy, *twz;          // twiddles of the y and z axes
+  double alpha, beta;
+  double *F;
+  float *Ff;
+  long w_f;
+  unsigned long long *rslots;
+  long w_r;
+};
+// stage 0: pre-twiddle along y; 1: butterflies along y + pre-twiddle along z;
+// 2: butterflies along z + epilogue
+hipError_t launch_fact(const FactArgs &a, int stage, hipStream_t s, hipEvent_t ev0 = nullptr,
+                       hipEvent_t ev1 = nullptr);
+void fact_twiddles(int n, std::vector<double> &tw);
+void fact_matrix(int n, bool transpose, std::vector<double> &g);
+// grids with both sizes multiples of 8 and at least this large use it in FP64
+constexpr int kFactMinN = 256;
+
+// ---------------------------------------------------------------------------
 // adjoint sweep (tertia_amd/csrc/adjoint.hip, qss_sweep_vjp)
 // ---------------------------------------------------------------------------
 struct AdjTileArgs {
@@ -162,6 +192,13 @@ struct qss_ctx {
   double *P[QSS_NFIELD] = {};
   double *S[SP_N] = {};
   double *tmp[kGemmMaxBatch] = {};   // transform intermediates
+  // factorized transform: eligible, in use, G matrices (y, z), twiddles,
+  // second intermediate per batch slot, GEMM variants per pass and batch size
+  bool fact_ok = false, fact_use = false;
+  double *d_G[2] = {};
+  double2 *d_tw[2] = {};
+  double *tmp2[kGemmMaxBatch] = {};
+  int fact_var[2][kGemmMaxBatch + 1] = {};
   double *F = nullptr;               // push field
   float *Ff = nullptr;               // push field, fp32 (mixed modes)
   bool mixed = false;                // precision != QSS_PREC_FP64

@@ -358,8 +358,14 @@ class Mesh:
     def gemm_choice(self) -> dict:
         arr = (ctypes.c_int * 18)()
         check(self.L.qss_gemm_choice(self.h, arr), "qss_gemm_choice")
-        return {"y_pass": list(arr[1:9]), "z_pass": list(arr[10:18]),
-                "particle_split": self.particle_split}
+        out = {"y_pass": list(arr[1:9]), "z_pass": list(arr[10:18]),
+               "particle_split": self.particle_split,
+               "transform_method": self.transform_method}
+        fa = (ctypes.c_int * 18)()
+        check(self.L.qss_fact_choice(self.h, fa), "qss_fact_choice")
+        out["fact_y_pass"] = list(fa[1:9])
+        out["fact_z_pass"] = list(fa[10:18])
+        return out
 
     def set_gemm_choice(self, choice: dict) -> None:
         """Install a tuning choice (as returned by gemm_choice/autotune):
@@ -370,6 +376,13 @@ class Mesh:
         for i, v in enumerate(choice["z_pass"]):
             arr[10 + i] = v
         check(self.L.qss_set_gemm_choice(self.h, arr), "qss_set_gemm_choice")
+        if "fact_y_pass" in choice:
+            fa = (ctypes.c_int * 18)()
+            for i, v in enumerate(choice["fact_y_pass"]):
+                fa[1 + i] = v
+            for i, v in enumerate(choice["fact_z_pass"]):
+                fa[10 + i] = v
+            check(self.L.qss_set_fact_choice(self.h, fa), "qss_set_fact_choice")
         if "particle_split" in choice:
             self.particle_split = choice["particle_split"]
 
@@ -384,6 +397,15 @@ class Mesh:
 
     def set_gemm_variant(self, v: int):
         check(self.L.qss_set_gemm_variant(self.h, int(v)), "qss_set_gemm_variant")
+
+    @property
+    def transform_method(self) -> int:
+        """QSS_TRANSFORM_DENSE (0) or QSS_TRANSFORM_FACTORED (1)."""
+        return self.L.qss_transform_method(self.h)
+
+    @transform_method.setter
+    def transform_method(self, method: int) -> None:
+        check(self.L.qss_set_transform_method(self.h, int(method)), "qss_set_transform_method")
 
     @staticmethod
     def gemm_variants(precision: int = 0) -> int:

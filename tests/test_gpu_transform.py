@@ -88,3 +88,58 @@ def test_every_tile_shape(mesh_factory, oracle_mod):
         choice = m.autotune(2)
         assert all(0 <= c < Mesh.gemm_variants() for c in choice["y_pass"] + choice["z_pass"])
         assert max_rel(m.transform2d(x, 1, 0), ref) < 1e-13
+
+
+@pytest.mark.parametrize("nz,ny", [(256, 264), (320, 256), (264, 512)])
+def test_factored_transform_vs_oracle(mesh_factory, oracle_mod, nz, ny):
+    """The factored form (one radix-2 step of the DCT-IV's half-length DFT, two
+    half-size GEMMs between twiddle kernels) and the dense form both match the
+    oracle's O(N^2)-per-row sums to 1e-13 on every kind, and each other."""
+    from tertia_amd.params import QSS_KIND_C
+    m = mesh_factory(nz, ny)
+    assert m.transform_method == 0          # contexts start dense
+    m.transform_method = 1                  # FP64, multiples of 8, >= 256: allowed
+    rng = np.random.default_rng(nz + ny)
+    x = rng.standard_normal((nz, ny))
+    for kz in (0, 1):
+        for ky in (0, 1):
+            ref = oracle_mod.transform2d(x, kz, ky, use_fft=True, nthreads=4)
+            m.transform_method = 1
+            fact = m.transform2d(x, kz, ky)
+            m.transform_method = 0
+            dense = m.transform2d(x, kz, ky)
+            e = (max_rel(fact, ref), max_rel(dense, ref), max_rel(fact, dense))
+            print("factored %dx%d kinds (%d,%d): vs oracle %.1e, dense %.1e, fact-dense %.1e"
+                  % (nz, ny, kz, ky, *e), flush=True)
+            assert max(e) < 1e-13, (kz, ky, e)
+    m.transform_method = 0
+
+
+def test_factored_method_refused_where_not_eligible(mesh_factory):
+    from tertia_amd._lib import QssError
+    m = mesh_factory(40, 72)
+    assert m.transform_method == 0
+    with pytest.raises(QssError):
+        m.transform_method = 1
+
+
+def test_factored_sweep_matches_dense_sweep():
+    """A 256^2 sweep with either transform method: the fields agree to
+    rounding (rel-L2 <= 1e-11 over 40 layers of the blowout workload)."""
+    from tertia_amd.mesh import Mesh
+    from tertia_amd.params import blowout_256
+    p = blowout_256(nslices=40)
+    out = {}
+    for method in (0, 1):
+        m = Mesh(p, history=("Ex", "Ey", "Bx", "Psi"))
+        try:
+            m.transform_method = method
+            m.MoveAllSplitLayers()
+            out[method] = {f: np.stack([m.history(f, s) for s in range(p.nslices)])
+                           for f in ("Ex", "Ey", "Bx", "Psi")}
+        finally:
+            m.close()
+    for f in out[0]:
+        e = float(np.linalg.norm(out[1][f] - out[0][f]) / np.linalg.norm(out[0][f]))
+        print("sweep factored vs dense", f, "%.1e" % e, flush=True)
+        assert e <= 1e-11, (f, e)
