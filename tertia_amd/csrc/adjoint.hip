@@ -200,8 +200,11 @@ __global__ __launch_bounds__(NT) void k_push_tile_adj(AdjTileArgs a) {
 }
 
 // out[q][node] = sum of the 17x17 windows holding the node, fixed order
+// pshare (optional): the P-layer share of the field adjoint, 1/2 of each
+// gathered field's adjoint, assigned (pmode 1) or accumulated (pmode 2)
 __global__ __launch_bounds__(NT) void k_fbar_combine(const double *fwin, int ntj, int ntk, int ny,
-                                                     int nz, int nsplit, double *out) {
+                                                     int nz, int nsplit, double *out,
+                                                     double *pshare, int pmode) {
   const long n = (long)blockIdx.x * NT + threadIdx.x;
   const long plane = (long)ny * nz;
   if (n >= plane) return;
@@ -224,6 +227,13 @@ __global__ __launch_bounds__(NT) void k_fbar_combine(const double *fwin, int ntj
   }
 #pragma unroll
   for (int q = 0; q < 6; ++q) out[q * plane + n] = s[q];
+  if (pmode == 1) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) pshare[q * plane + n] = 0.5 * s[q];
+  } else if (pmode == 2) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) pshare[q * plane + n] += 0.5 * s[q];
+  }
 }
 
 // ---- beam profile adjoint ----------------------------------------------------
@@ -313,9 +323,10 @@ hipError_t launch_push_tile_adj(qss_ctx *c, const AdjTileArgs &a) {
   return hipGetLastError();
 }
 
-hipError_t launch_fbar_combine(qss_ctx *c, const double *fwin, double *out) {
+hipError_t launch_fbar_combine(qss_ctx *c, const double *fwin, double *out, double *pshare,
+                               int pmode) {
   hipLaunchKernelGGL(k_fbar_combine, grid1(c->ncell), dim3(NT), 0, c->stream, fwin, c->ntj,
-                     c->ntk, c->ny, c->nz, c->nsplit, out);
+                     c->ntk, c->ny, c->nz, c->nsplit, out, pshare, pmode);
   return hipGetLastError();
 }
 

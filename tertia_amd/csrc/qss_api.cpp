@@ -526,22 +526,27 @@ double *tape_field(const qss_ctx *c, char *slot, int pass) {
 // field (the reverse sweep replays the pushes from them)
 int do_layer_step(qss_ctx *c, int slice, int isplit, double *res_dev, char *tape = nullptr) {
   if (tape) CHECK(tape_particles(c, part_view(c, tape)));
-  auto rec = [&](int pass) -> int {
-    if (tape)
-      HIPCHECK(hipMemcpyAsync(tape_field(c, tape, pass), c->F,
-                              kPushComp * c->ncell * sizeof(double), hipMemcpyDeviceToDevice,
-                              c->stream));
-    return QSS_OK;
+  // recording: the inverse transforms' epilogue writes each pass's push field
+  // straight into its tape slot, which that pass's push then reads
+  double *const F0 = c->F;
+  auto field_of = [&](int pass) {
+    if (tape) c->F = tape_field(c, tape, pass);
   };
+  struct Restore {
+    qss_ctx *c;
+    double *F;
+    ~Restore() { c->F = F; }
+  } restore{c, F0};
   CHECK(do_beam(c, slice, isplit));
+  field_of(0);
   CHECK(do_guess(c, slice == 0));
   for (int it = 0; it < c->p.n_iter; ++it) {
-    CHECK(rec(it));
     CHECK(do_push_deposit(c, 0));
+    field_of(it + 1);
     CHECK(do_iterate(c, 0, nullptr));
   }
-  CHECK(rec(c->p.n_iter));
   CHECK(do_push_deposit(c, 1));
+  c->F = F0;   // the final corrector's push field is not replayed
   CHECK(do_iterate(c, 1, res_dev));
   swap_c_p(c);
   return QSS_OK;
@@ -665,8 +670,8 @@ int adj_push_pass(qss_ctx *c, char *slot, int pass, int full, int cur_adj) {
   }
   {
     Timed t(c, KC_REDUCE);
-    HIPCHECK(launch_fbar_combine(c, s.fwin, s.fbar));
-    HIPCHECK(launch_axpy(s.pbarEB, s.fbar, 0.5, c->ncell * kPushComp, c->stream));
+    // the layer's first reverse pass (the full step) starts the P-layer share
+    HIPCHECK(launch_fbar_combine(c, s.fwin, s.fbar, s.pbarEB, full ? 1 : 2));
   }
   const int kz[6] = {QSS_KIND_C, QSS_KIND_C, QSS_KIND_S, QSS_KIND_S, QSS_KIND_S, QSS_KIND_C};
   const int ky[6] = {QSS_KIND_C, QSS_KIND_S, QSS_KIND_C, QSS_KIND_S, QSS_KIND_C, QSS_KIND_S};
@@ -713,8 +718,8 @@ int adj_layer_step(qss_ctx *c, int slice, char *slot, const double *exbar, int &
       in[q] = s.pbarEB + q * nc;
       out[q] = s.A[SP_EX + q];
     }
+    // pbarEB is reassigned by the full step's field-adjoint combine below
     CHECK(transform_batch(c, 6, kz, ky, in, out, sinv, nullptr, nullptr, nullptr, 1.0));
-    HIPCHECK(launch_fill_f64(s.pbarEB, nc * kPushComp, 0.0, c->stream));
   }
   // final Iterate, full-step push
   {

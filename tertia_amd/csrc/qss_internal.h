@@ -328,7 +328,9 @@ hipError_t launch_add_i32(int *p, int v, hipStream_t s);
 hipError_t launch_guess_adj(qss_ctx *c, double *const *A, int first_layer);
 hipError_t launch_iterate_adj(qss_ctx *c, double *const *A, int final_pass);
 hipError_t launch_push_tile_adj(qss_ctx *c, const AdjTileArgs &a);
-hipError_t launch_fbar_combine(qss_ctx *c, const double *fwin, double *out);
+// out = the six field-adjoint planes; pshare (pmode 1: =, 2: +=) their halves
+hipError_t launch_fbar_combine(qss_ctx *c, const double *fwin, double *out, double *pshare,
+                               int pmode);
 // q: per layer, side and work-group partial sums (kBeamAdjBlocks per pair)
 constexpr int kBeamAdjBlocks = 64;
 hipError_t launch_beam_adj(qss_ctx *c, int slice, const double *rbar, const double *jbar,
