@@ -443,10 +443,17 @@ struct RowBF16 {
 template <int NPC>
 __device__ inline void split_bf16(double x, uint16_t *p) { qss_split::split<NPC>(x, p); }
 
+// KS > 1: KS waves share each wave tile, wave ks multiplying the ks-th part
+// of every K slab; their fp64 partial sums are added through LDS in ks order
+// before the store (launches of one to three fields otherwise leave most
+// SIMDs without a wave)
 template <int BM, int BN, int BK, int WGM, int WGN, int NPC, bool GUARD, bool TOUT,
-          bool EPI = true>
-__global__ __launch_bounds__(WGM * WGN * 64) void k_gemm_bf16(GemmArgs a) {
-  constexpr int NT = WGM * WGN * 64;
+          bool EPI = true, int KS = 1>
+__global__ __launch_bounds__(WGM * WGN * KS * 64) void k_gemm_bf16(GemmArgs a) {
+  constexpr int NWT = WGM * WGN;
+  constexpr int NT = NWT * KS * 64;
+  constexpr int KSTEP = BK / 32 / KS;          // MFMA k-steps per wave per slab
+  static_assert(KSTEP * KS * 32 == BK, "K split");
   constexpr int WTM = BM / WGM, WTN = BN / WGN;
   constexpr int TI = WTM / 16, TJ = WTN / 16;
   constexpr int KP = BK + 8;                   // LDS row pitch (bf16): 16-byte pad
@@ -461,11 +468,17 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_gemm_bf16(GemmArgs a) {
   const int m0 = it.m0, n0 = it.n0;
   const long pa = (long)M * K, pb = (long)N * K;   // piece strides
 
-  __shared__ __attribute__((aligned(16))) uint16_t As[NPC][2][BM][KP];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[NPC][2][BN][KP];
+  constexpr size_t kSlabA = sizeof(uint16_t) * NPC * 2 * BM * KP;
+  constexpr size_t kSlabB = sizeof(uint16_t) * NPC * 2 * BN * KP;
+  constexpr size_t kRed = KS > 1 ? sizeof(d4) * (KS - 1) * NWT * TI * TJ * 64 : 0;
+  constexpr size_t kSmem = kSlabA + kSlabB > kRed ? kSlabA + kSlabB : kRed;
+  __shared__ __attribute__((aligned(16))) char smem[kSmem];
+  uint16_t(*As)[2][BM][KP] = reinterpret_cast<uint16_t(*)[2][BM][KP]>(smem);
+  uint16_t(*Bs)[2][BN][KP] = reinterpret_cast<uint16_t(*)[2][BN][KP]>(smem + kSlabA);
 
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wm = w / WGN, wn = w - wm * WGN;
+  const int ks = w / NWT, wl = w - ks * NWT;
+  const int wm = wl / WGN, wn = wl - wm * WGN;
   const int li = lane & 15, lk = lane >> 4;
 
   // group g of an operand tile: row g / KG, k offset 8 (g % KG); consecutive
@@ -539,8 +552,8 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_gemm_bf16(GemmArgs a) {
     // the previous slab's hi sums (their MFMAs finished behind the barrier)
     flush_hi();
 #pragma unroll
-    for (int kk = 0; kk < BK / 32; ++kk) {
-      const int kc = kk * 32 + 8 * lk;
+    for (int kq = 0; kq < KSTEP; ++kq) {
+      const int kc = (ks * KSTEP + kq) * 32 + 8 * lk;
       bf8 av[NPC][TI], bv[NPC][TJ];
 #pragma unroll
       for (int p = 0; p < NPC; ++p) {
@@ -573,6 +586,36 @@ __global__ __launch_bounds__(WGM * WGN * 64) void k_gemm_bf16(GemmArgs a) {
       [&](int buf) { store(buf, ra0, rb0); }, [&](int buf) { store(buf, ra1, rb1); },
       compute);
   flush_hi();
+
+  if constexpr (KS > 1) {
+    // each wave's fp64 sum hid + lo; waves ks > 0 hand theirs to wave 0 of
+    // their wave tile through LDS (the slab storage, once every wave has
+    // finished its last slab), added in ks order
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) hid[i][j][r] += (double)lo[i][j][r];
+        lo[i][j] = f4{0, 0, 0, 0};
+      }
+    d4(*red)[NWT][TI * TJ][64] = reinterpret_cast<d4(*)[NWT][TI * TJ][64]>(smem);
+    __syncthreads();
+    if (ks > 0) {
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) red[ks - 1][wl][i * TJ + j][lane] = hid[i][j];
+    }
+    __syncthreads();
+    if (ks > 0) return;
+#pragma unroll
+    for (int q = 0; q < KS - 1; ++q)
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) hid[i][j] += red[q][wl][i * TJ + j][lane];
+  }
 
   if constexpr (!TOUT) {
     auto val = [&](int i, int j, int r) { return hid[i][j][r] + (double)lo[i][j][r]; };
@@ -714,6 +757,16 @@ const Variant kF32[] = {
     QSS_GEMM_DENSE(MathF32, 32, 64, 32, 1, 2, 2),
 };
 
+#define QSS_GEMM_BF16K(NPC, BM, BN, BK, WGM, WGN, KS)                                  \
+  Variant{BM, BN, BK, WGM * WGN * KS * 64,                                             \
+          {{k_gemm_bf16<BM, BN, BK, WGM, WGN, NPC, false, false, false, KS>,           \
+            k_gemm_bf16<BM, BN, BK, WGM, WGN, NPC, false, false, true, KS>,            \
+            k_gemm_bf16<BM, BN, BK, WGM, WGN, NPC, false, true, true, KS>},            \
+           {k_gemm_bf16<BM, BN, BK, WGM, WGN, NPC, true, false, false, KS>,            \
+            k_gemm_bf16<BM, BN, BK, WGM, WGN, NPC, true, false, true, KS>,             \
+            k_gemm_bf16<BM, BN, BK, WGM, WGN, NPC, true, true, true, KS>}},            \
+          #BM "x" #BN "x" #BK "_w" #WGM "x" #WGN "_k" #KS}
+
 #define QSS_BF16_TABLE(NPC)                     \
   {                                             \
     QSS_GEMM_BF16(NPC, 64, 64, 32, 2, 2),       \
@@ -725,6 +778,12 @@ const Variant kF32[] = {
     QSS_GEMM_BF16(NPC, 128, 128, 32, 2, 2),     \
     QSS_GEMM_BF16(NPC, 32, 64, 32, 1, 2),       \
     QSS_GEMM_BF16(NPC, 64, 32, 32, 2, 1),       \
+    QSS_GEMM_BF16K(NPC, 32, 32, 64, 1, 1, 2),   \
+    QSS_GEMM_BF16K(NPC, 32, 32, 128, 1, 1, 4),  \
+    QSS_GEMM_BF16K(NPC, 32, 64, 64, 1, 2, 2),   \
+    QSS_GEMM_BF16K(NPC, 64, 32, 64, 2, 1, 2),   \
+    QSS_GEMM_BF16K(NPC, 64, 64, 64, 2, 2, 2),   \
+    QSS_GEMM_BF16K(NPC, 32, 32, 128, 1, 1, 2),  \
   }
 
 const Variant kBF16x3[] = QSS_BF16_TABLE(2);
