@@ -444,6 +444,69 @@ int  qss_enable_timing(qss_ctx *c, int on);
 int  qss_timing(qss_ctx *c, double *ms_total, double *ms_kernel,
                 int64_t *launches, double *gemm_flops);
 
+/* ---- HDF5 output and restart files: Mesh::Save / Mesh::Load ------------ */
+/* Self-contained (no libhdf5): files of the HDF5 format the reference's
+ * library defaults produce (superblock 0, symbol-table root group, contiguous
+ * datasets), readable by any HDF5 reader; the reader takes such files from
+ * any writer.  Host-only: these calls need no GPU except qss_save/qss_load.
+ *
+ * Particle records as stored (SavedParticle / SavedIon, particles.h:111-153;
+ * compound members registered at save.cpp:225-251): */
+typedef struct qss_saved_particle {
+  int32_t cell;                 /* ghosted cell number GetN(i,j,k)          */
+  int32_t pad_;
+  double  x, y, z;              /* position (cell + in-cell fraction) * h   */
+  double  px, py, pz, weight, q2m;
+} qss_saved_particle;           /* 72 bytes                                 */
+typedef struct qss_saved_ion {
+  int32_t cell;
+  int32_t pad_;
+  double  x, y, z, px, py, pz, weight, q2m;
+  int32_t i_z, sort;            /* charge state, species                     */
+} qss_saved_ion;                /* 80 bytes                                 */
+
+#define QSS_H5_F64       0      /* H5T_IEEE_F64LE                            */
+#define QSS_H5_I32       1      /* H5T_STD_I32LE                             */
+#define QSS_H5_I64       2      /* H5T_STD_I64LE                             */
+#define QSS_H5_PARTICLE  3      /* qss_saved_particle records               */
+#define QSS_H5_ION       4      /* qss_saved_ion records                    */
+typedef struct qss_h5_set {
+  const char *name;             /* dataset name in the root group           */
+  int32_t     kind;             /* QSS_H5_*                                  */
+  int32_t     rank;             /* 1..3                                      */
+  int64_t     dims[3];          /* slowest first                             */
+  const void *data;             /* host, product(dims) elements              */
+} qss_h5_set;
+/* WriteHDF5Recorddouble / RecordInt / RecordLong (myhdfshell.cpp) for a
+ * whole file: creates `path` (truncating) with n datasets.                  */
+int  qss_h5_write(const char *path, int n, const qss_h5_set *sets);
+/* Shape and kind of a dataset (kind: QSS_H5_* of the stored type, -1 for
+ * other types); dims[3] filled slowest first, unused entries 0.            */
+int  qss_h5_info(const char *path, const char *name, int32_t *kind, int32_t *rank,
+                 int64_t *dims);
+/* ReadHDF5Record*: elements [first, first+count) of a dataset, converted to
+ * `kind` (integer and float widths; compound records member by member,
+ * matched by name).                                                        */
+int  qss_h5_read(const char *path, const char *name, int32_t kind, int64_t first,
+                 int64_t count, void *dst);
+/* Dataset names of the file, one per line, NUL-terminated.                  */
+int  qss_h5_list(const char *path, char *names, size_t cap);
+
+/* Mesh::Save (save.cpp:178-835) of PE 0: writes <dir>/vs<isave>_3d_fields.h5
+ * (meshinfo, phase, X Y Z axes with the window shift, ex ey ez bx by bz jx jy
+ * jz dens Dens<sort> as [Mz][My][Mx] from the history of the selected wake;
+ * needs history of QSS_EX..QSS_JZ, and QSS_RHOBEAM without an evolving
+ * driver) and <dir>/vs<isave>_3d_particles.h5 (the evolving driver's
+ * particles as Ions_of_sort_<1+bunch>_at_PE_0 records, Nparticles_at_PE_0).
+ * Sort 0 is the plasma, which has no particles in the 3-D mesh.            */
+int  qss_save(qss_ctx *c, const char *dir, int isave, double phase, int shift);
+/* Mesh::Load (load.cpp:285-1080): checks meshinfo against the context,
+ * loads every field of ex..jz that the context keeps history of into that
+ * history (selected wake) and, with a seeded evolving driver, replaces its
+ * particles by the file's records of sorts 1..nbunch (qss_driver_set; the
+ * in-cell fraction is x/hx - i, clamped to [0, 1]).  phase may be NULL.     */
+int  qss_load(qss_ctx *c, const char *dir, int isave, double *phase);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1123,10 +1123,12 @@ int oracle_sweep_range(oracle *o, int first, int count, double *residuals) {
       st = std::min(st, layer_step(o, s, is, &r));
       if (residuals) residuals[(s - first) * o->p.nx_split + is] = r;
     }
+    // the layer's fields are in P after copy_c_to_p; its beam planes in C
     for (int f = 0; f < QSS_NFIELD; ++f)
-      if (!o->hist[f].empty())
-        std::copy(o->P[f].begin(), o->P[f].end(),
-                  o->hist[f].begin() + (size_t)s * o->ncell);
+      if (!o->hist[f].empty()) {
+        const std::vector<double> &src = f == QSS_RHOBEAM || f == QSS_JXBEAM ? o->C[f] : o->P[f];
+        std::copy(src.begin(), src.end(), o->hist[f].begin() + (size_t)s * o->ncell);
+      }
   }
   return st;
 }

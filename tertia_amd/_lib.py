@@ -21,6 +21,12 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
 
+
+class QssH5Set(ctypes.Structure):
+    """qss_h5_set: one dataset of qss_h5_write."""
+    _fields_ = [("name", ctypes.c_char_p), ("kind", ctypes.c_int32), ("rank", ctypes.c_int32),
+                ("dims", ctypes.c_int64 * 3), ("data", ctypes.c_void_p)]
+
 # every symbol declared in include/qss.h: (restype, argtypes)
 SIGNATURES = {
     "qss_create": (ctypes.c_int, [ctypes.POINTER(QssParams), ctypes.c_int, ctypes.c_uint32,
@@ -89,6 +95,14 @@ SIGNATURES = {
     "qss_particle_split": (ctypes.c_int, [_vp]),
     "qss_enable_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "qss_timing": (ctypes.c_int, [_vp, _dp, _dp, _i64p, _dp]),
+    "qss_h5_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(QssH5Set)]),
+    "qss_h5_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _i32p, _i32p, _i64p]),
+    "qss_h5_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32,
+                                   ctypes.c_int64, ctypes.c_int64, _vp]),
+    "qss_h5_list": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t]),
+    "qss_save": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
+                                ctypes.c_int]),
+    "qss_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int, _dp]),
 }
 
 _lib = None

@@ -33,7 +33,8 @@ NO_CONTRACT = {"particles.hip", "push_tile.hip", "beam.hip", "spectral.hip", "ad
 # draining the MFMA pipeline once per iteration
 MFMA_VGPR = {"gemm.hip"}
 SOURCES = ["gemm.hip", "spectral.hip", "beam.hip", "particles.hip", "push_tile.hip",
-           "adjoint.hip", "driver.hip", "noalloc.hip", "fact.hip", "qss_api.cpp"]
+           "adjoint.hip", "driver.hip", "noalloc.hip", "fact.hip", "qss_api.cpp",
+           "h5file.cpp", "save.cpp"]
 
 
 def _compile(src: str) -> str:

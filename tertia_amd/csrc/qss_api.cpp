@@ -25,21 +25,6 @@ namespace {
 
 thread_local std::string g_err;
 
-#define HIPCHECK(x)                                                              \
-  do {                                                                           \
-    hipError_t e_ = (x);                                                         \
-    if (e_ != hipSuccess) {                                                      \
-      qss_set_error(std::string(#x) + ": " + hipGetErrorString(e_));             \
-      return QSS_E_HIP;                                                          \
-    }                                                                            \
-  } while (0)
-
-#define CHECK(x)                                                                 \
-  do {                                                                           \
-    int r_ = (x);                                                                \
-    if (r_ != QSS_OK) return r_;                                                 \
-  } while (0)
-
 constexpr int kNClass = QSS_NKCLASS;
 
 // ---- timing helpers --------------------------------------------------------
@@ -760,14 +745,20 @@ int adj_layer_step(qss_ctx *c, int slice, char *slot, const double *exbar, int &
 }
 
 // the finished layer into the history (movepart_splitlayer.cpp:467-473);
-// wake w of a batch at hist + (w * nslices + s) * ncell
+// wake w of a batch at hist + (w * nslices + s) * ncell.  The layer's fields
+// are in P after the closing swap; its beam planes stay in C (the swap
+// leaves them, the next layer overwrites them).
+const double *history_source(const qss_ctx *c, int f) {
+  return f == QSS_RHOBEAM || f == QSS_JXBEAM ? c->C[f] : c->P[f];
+}
+
 int store_history(qss_ctx *c, int s) {
   const size_t plane = c->ncell * sizeof(double);
   for (int f = 0; f < QSS_NFIELD; ++f)
     if (c->hist[f])
       HIPCHECK(hipMemcpy2DAsync(c->hist[f] + (size_t)s * c->ncell, plane * c->p.nslices,
-                                c->P[f], plane, plane, c->nw, hipMemcpyDeviceToDevice,
-                                c->stream));
+                                history_source(c, f), plane, plane, c->nw,
+                                hipMemcpyDeviceToDevice, c->stream));
   return QSS_OK;
 }
 
@@ -803,7 +794,7 @@ int sweep_graph_pair(qss_ctx *c, int s, int *done) {
       // not the first layer; residual slot of layer 0 (+ d_slice at replay)
       r = do_layer_step(c, 1, 0, c->d_res_hist);
       for (int f = 0; f < QSS_NFIELD && r == QSS_OK; ++f)
-        if (c->hist[f] && launch_store_hist(c, c->P[f], c->hist[f]) != hipSuccess)
+        if (c->hist[f] && launch_store_hist(c, history_source(c, f), c->hist[f]) != hipSuccess)
           r = QSS_E_HIP;
       if (r == QSS_OK && launch_add_i32(c->d_slice, 1, c->stream) != hipSuccess) r = QSS_E_HIP;
     }

@@ -338,3 +338,21 @@ hipError_t launch_beam_adj(qss_ctx *c, int slice, const double *rbar, const doub
 
 // host helpers
 void qss_set_error(const std::string &m);
+
+// early returns of the C-ABI functions: a HIP error (message recorded) or a
+// non-OK status of a nested call
+#define HIPCHECK(x)                                                              \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess) {                                                      \
+      qss_set_error(std::string(#x) + ": " + hipGetErrorString(e_));             \
+      return QSS_E_HIP;                                                          \
+    }                                                                            \
+  } while (0)
+
+#define CHECK(x)                                                                 \
+  do {                                                                           \
+    int r_ = (x);                                                                \
+    if (r_ != QSS_OK) return r_;                                                 \
+  } while (0)
+

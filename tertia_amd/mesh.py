@@ -25,6 +25,7 @@ layer behind the front) down to 0; ``slice = l_Mx-1-iLayer``.  Errors raise
 from __future__ import annotations
 
 import ctypes
+import os
 from typing import Iterable, Optional
 
 import numpy as np
@@ -233,6 +234,32 @@ class Mesh:
         Np follows from its length as in ReceiveLayer (para.cpp:385)."""
         r = np.ascontiguousarray(record, dtype=np.float64)
         check(self.L.qss_unpack_layer_record(self.h, _p(r), r.size), "qss_unpack_layer_record")
+
+    # -- HDF5 output and restart: Mesh::Save / Mesh::Load --------------------
+    def Save(self, isave: int, directory: str = ".", phase: float = 0.0, shift: int = 0) -> None:
+        """Mesh::Save (save.cpp:178-835): writes vs<isave>_3d_fields.h5 (the
+        swept wake as [Mz][My][Mx] fields ex..jz, dens, Dens<sort>, with the
+        axes, meshinfo and phase) and vs<isave>_3d_particles.h5 (the evolving
+        driver's particles) into `directory`.  Needs the history of ex..jz
+        (and of rhobeam with the frozen driver)."""
+        check(self.L.qss_save(self.h, directory.encode(), int(isave), float(phase), int(shift)),
+              "qss_save")
+
+    def Load(self, isave: int, directory: str = ".") -> float:
+        """Mesh::Load (load.cpp:285-1080): the history of ex..jz that this
+        mesh keeps, and a seeded evolving driver's particles, from the files
+        Save wrote; returns the phase."""
+        ph = ctypes.c_double()
+        check(self.L.qss_load(self.h, directory.encode(), int(isave), ctypes.byref(ph)),
+              "qss_load")
+        if getattr(self, "_driver_n", None) is not None:   # particles were replaced
+            from . import h5
+            path = os.path.join(directory, "vs%03d_3d_particles.h5" % int(isave))
+            have = set(h5.names(path))
+            self._driver_n = sum(h5.info(path, "Ions_of_sort_%02d_at_PE_0" % s)[1][0]
+                                 for s in range(1, len(self.params.bunches) + 1)
+                                 if "Ions_of_sort_%02d_at_PE_0" % s in have)
+        return ph.value
 
     # -- evolving driver: Domain::Step (step.cpp:63,105) ---------------------
     def SeedBeamParticles(self, driver: Driver) -> int:
