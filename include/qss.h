@@ -340,15 +340,23 @@ int  qss_unpack_layer_record(qss_ctx *c, const double *src, size_t n);
  * table (the reference's HalfInteger matrix form, fft_matrix.cu:10-24).
  * FACTORED: one radix-2 step of the DCT-IV's half-length complex DFT turns
  * each pass into an (N/2 x N/2) GEMM applied to both halves of every row,
- * half the multiply-adds, between twiddle kernels (FP64 contexts with ny,
- * nz multiples of 8 and at least 256).  Contexts start DENSE.  Results
+ * half the multiply-adds (FP64 contexts with ny, nz multiples of 8 and at
+ * least 256).  Where ny and nz are multiples of 128 (at most 1024) the
+ * twiddles run inside the two GEMMs; elsewhere in twiddle kernels between
+ * them.  Contexts start FACTORED where the twiddles run inside the GEMMs
+ * (measured faster at 256^2, 384^2 and 512^2) and DENSE otherwise.  Results
  * agree to rounding.  Setting a method drops a captured graph.              */
 #define QSS_TRANSFORM_DENSE    0
 #define QSS_TRANSFORM_FACTORED 1
 int  qss_set_transform_method(qss_ctx *c, int method);
 int  qss_transform_method(const qss_ctx *c);
-/* The FACTORED method's GEMM tile shapes (qss_autotune tunes them; layout of
- * qss_gemm_choice, FP64 variants).                                         */
+/* The FACTORED method's tile shapes (qss_autotune tunes them), 4 blocks of
+ * 9 ints, each indexed by batch size 0..8 like qss_gemm_choice: the GEMMs of
+ * the twiddle-kernel form (y pass, z pass; FP64 variants), then the fused
+ * kernels (y pass, z pass; their own variant numbers, -1 where the context
+ * has no fused form).  Copying a choice between contexts of the same grid
+ * makes their transforms bitwise alike.                                    */
+#define QSS_FACT_CHOICE_INTS (4 * 9)
 int  qss_fact_choice(const qss_ctx *c, int *chosen);
 int  qss_set_fact_choice(qss_ctx *c, const int *chosen);
 

@@ -31,9 +31,9 @@ NO_CONTRACT = {"particles.hip", "push_tile.hip", "beam.hip", "spectral.hip", "ad
 # MFMA accumulators in VGPRs: with AGPR accumulators the compiler copies
 # every loop-carried accumulator AGPR->VGPR->AGPR at the K-loop back-edge,
 # draining the MFMA pipeline once per iteration
-MFMA_VGPR = {"gemm.hip"}
+MFMA_VGPR = {"gemm.hip", "fact_fused.hip"}
 SOURCES = ["gemm.hip", "spectral.hip", "beam.hip", "particles.hip", "push_tile.hip",
-           "adjoint.hip", "driver.hip", "noalloc.hip", "fact.hip", "qss_api.cpp",
+           "adjoint.hip", "driver.hip", "noalloc.hip", "fact.hip", "fact_fused.hip", "qss_api.cpp",
            "h5file.cpp", "save.cpp"]
 
 

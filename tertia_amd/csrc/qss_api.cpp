@@ -166,6 +166,37 @@ int transform_fact(qss_ctx *c, int n, const int *kz, const int *ky, const double
                    double *const *out, double alpha, const int *resid, const int *push,
                    const double *const *pplane, double beta, bool one_wake) {
   const long nc = c->ncell;
+  if (c->ffused_ok) {
+    // twiddles inside the GEMMs: y pass in -> tmp, z pass tmp -> out with the
+    // epilogue
+    FusedArgs f{};
+    f.ny = c->ny; f.nz = c->nz; f.nbatch = n;
+    f.nw = one_wake ? 1 : c->nw;
+    f.w_in = nc; f.w_out = nc; f.w_p = nc; f.w_f = nc * kPushComp; f.w_r = kResidPerWake;
+    f.alpha = alpha; f.beta = beta;
+    f.F = c->F; f.Ff = c->Ff; f.rslots = c->d_resid;
+    for (int b = 0; b < n; ++b) {
+      f.in[b] = in[b]; f.out[b] = c->tmp[b];
+      f.P[b] = pplane ? pplane[b] : nullptr;
+      f.kz[b] = kz[b]; f.ky[b] = ky[b];
+      f.resid_c[b] = resid ? resid[b] : -1;
+      f.push[b] = push ? push[b] : -1;
+    }
+    f.G = c->d_Gf[0]; f.tw = c->d_tw[0];
+    {
+      Timed t(c, KC_GEMM, true);
+      HIPCHECK(launch_fact_fused(f, 0, c->ffv[0][n], c->stream, t.e0, t.e1));
+    }
+    if (c->timing) c->gemm_flops += 2.0 * (2.0 * c->nz) * (c->ny / 2) * (c->ny / 2) * n * f.nw;
+    for (int b = 0; b < n; ++b) { f.in[b] = c->tmp[b]; f.out[b] = out[b]; }
+    f.G = c->d_Gf[1]; f.tw = c->d_tw[1];
+    {
+      Timed t(c, KC_GEMM, true);
+      HIPCHECK(launch_fact_fused(f, 1, c->ffv[1][n], c->stream, t.e0, t.e1));
+    }
+    if (c->timing) c->gemm_flops += 2.0 * (c->nz / 2) * (2.0 * c->ny) * (c->nz / 2) * n * f.nw;
+    return QSS_OK;
+  }
   FactArgs f{};
   f.ny = c->ny; f.nz = c->nz; f.nbatch = n;
   f.nw = one_wake ? 1 : c->nw;
@@ -991,9 +1022,10 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
   // factorized transforms (fact.hip) for large FP64 grids
   c->fact_ok = p->precision == QSS_PREC_FP64 && c->ny % 8 == 0 && c->nz % 8 == 0 &&
                c->ny >= kFactMinN && c->nz >= kFactMinN;
-  // dense by default: at 512^2 the factored form's GEMMs take 17.1 against
-  // 27.4 us per launch, but its three twiddle kernels per transform cost more
-  // than that saves (measured 0.718 against 0.705 ms per layer step)
+  // dense by default where the twiddles need kernels of their own: at 512^2
+  // those three kernels per transform cost more than the halved GEMMs save
+  // (0.718 against 0.705 ms per layer step); the fused form (below) is the
+  // default where it applies
   c->fact_use = false;
   if (c->fact_ok) {
     for (int b = 0; b < kGemmMaxBatch; ++b)
@@ -1012,6 +1044,28 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
         qss_set_error("factorized transform table upload failed");
         return fail(QSS_E_HIP);
       }
+    }
+    // the fused form where its tile shapes fit (sizes multiples of 128)
+    int v0 = -1, v1 = -1;
+    for (int v = fact_fused_variants(0) - 1; v >= 0; --v)
+      if (fact_fused_shape_ok(0, v, c->ny, c->nz)) v0 = v;
+    for (int v = fact_fused_variants(1) - 1; v >= 0; --v)
+      if (fact_fused_shape_ok(1, v, c->ny, c->nz)) v1 = v;
+    if (v0 >= 0 && v1 >= 0) {
+      for (int i = 0; i < 2; ++i) {
+        std::vector<double> g;
+        fact_fused_matrix(i == 0 ? c->ny : c->nz, i, g);
+        if ((r = dalloc(&c->d_Gf[i], g.size())) != QSS_OK) return fail(r);
+        if (hipMemcpy(c->d_Gf[i], g.data(), g.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+          qss_set_error("fused transform table upload failed");
+          return fail(QSS_E_HIP);
+        }
+      }
+      for (int b = 0; b <= kGemmMaxBatch; ++b) { c->ffv[0][b] = v0; c->ffv[1][b] = v1; }
+      c->ffused_ok = true;
+      // faster than the dense tables at 256^2 (0.293 against 0.300 ms per
+      // layer step), 384^2 (0.416 / 0.457) and 512^2 (0.591 / 0.695)
+      c->fact_use = true;
     }
   }
   if ((r = dalloc(&c->F, ncw * kPushComp)) != QSS_OK) return fail(r);
@@ -1118,6 +1172,7 @@ int qss_destroy(qss_ctx *c) {
   for (auto &p : c->tmp) dfree(p);
   for (auto &p : c->tmp2) dfree(p);
   dfree(c->d_G[0]); dfree(c->d_G[1]); dfree(c->d_tw[0]); dfree(c->d_tw[1]);
+  dfree(c->d_Gf[0]); dfree(c->d_Gf[1]);
   dfree(c->F); dfree(c->Ff); dfree(c->d_bins); dfree(c->d_wbunch);
   for (auto &p : c->Ts) dfree(p);
   for (auto &p : c->xs) dfree(p);
@@ -1651,24 +1706,36 @@ int qss_gemm_choice(const qss_ctx *c, int *chosen) {
 
 int qss_fact_choice(const qss_ctx *c, int *chosen) {
   if (!c || !chosen) return QSS_E_ARG;
+  constexpr int S = kGemmMaxBatch + 1;
   for (int p = 0; p < 2; ++p)
-    for (int b = 0; b <= kGemmMaxBatch; ++b)
-      chosen[p * (kGemmMaxBatch + 1) + b] = c->fact_var[p][b];
+    for (int b = 0; b < S; ++b) {
+      chosen[p * S + b] = c->fact_var[p][b];
+      chosen[(2 + p) * S + b] = c->ffused_ok ? c->ffv[p][b] : -1;
+    }
   return QSS_OK;
 }
 
 int qss_set_fact_choice(qss_ctx *c, const int *chosen) {
   if (!c || !chosen) return QSS_E_ARG;
-  graph_free(c);
+  constexpr int S = kGemmMaxBatch + 1;
   const int nv = gemm_num_variants(QSS_PREC_FP64);
-  for (int i = 0; i < 2 * (kGemmMaxBatch + 1); ++i)
+  for (int i = 0; i < 2 * S; ++i)
     if (chosen[i] < 0 || chosen[i] >= nv) {
       qss_set_error("factored gemm choice out of range");
       return QSS_E_ARG;
     }
+  for (int p = 0; p < 2 && c->ffused_ok; ++p)
+    for (int b = 0; b < S; ++b)
+      if (!fact_fused_shape_ok(p, chosen[(2 + p) * S + b], c->ny, c->nz)) {
+        qss_set_error("fused transform choice out of range for this grid");
+        return QSS_E_ARG;
+      }
+  graph_free(c);
   for (int p = 0; p < 2; ++p)
-    for (int b = 0; b <= kGemmMaxBatch; ++b)
-      c->fact_var[p][b] = chosen[p * (kGemmMaxBatch + 1) + b];
+    for (int b = 0; b < S; ++b) {
+      c->fact_var[p][b] = chosen[p * S + b];
+      if (c->ffused_ok) c->ffv[p][b] = chosen[(2 + p) * S + b];
+    }
   return QSS_OK;
 }
 
@@ -1785,6 +1852,40 @@ int qss_autotune(qss_ctx *c, int reps, float *table_ms) {
         if (ms < best) { best = ms; bestv = v; }
       }
       c->fact_var[pass][b] = bestv;
+    }
+  // the fused form's tile shapes, per pass and batch size; the z pass timed
+  // with the corrector's epilogue where the sweep runs it (6 and 8 fields)
+  for (int pass = 0; pass < 2 && rc == QSS_OK && c->ffused_ok; ++pass)
+    for (int b = 1; b <= kGemmMaxBatch && rc == QSS_OK; ++b) {
+      FusedArgs f{};
+      f.ny = c->ny; f.nz = c->nz; f.nbatch = b; f.nw = c->nw;
+      f.w_in = c->ncell; f.w_out = c->ncell; f.w_p = c->ncell;
+      f.w_f = c->ncell * kPushComp; f.w_r = kResidPerWake;
+      f.alpha = 1.0; f.F = c->F; f.Ff = c->Ff; f.rslots = c->d_resid;
+      f.G = c->d_Gf[pass]; f.tw = c->d_tw[pass];
+      const bool epi = pass == 1 && (b == 6 || b == 8);
+      for (int q = 0; q < b; ++q) {
+        f.in[q] = pass == 0 ? c->S[q] : c->tmp2[q];
+        f.out[q] = pass == 0 ? c->tmp2[q] : c->tmp[q];
+        f.P[q] = epi && q < 6 ? c->P[q] : nullptr;
+        f.kz[q] = f.ky[q] = QSS_KIND_C;
+        f.resid_c[q] = epi && q < 3 ? q : -1;
+        f.push[q] = epi && q < 6 ? q : -1;
+      }
+      float best = 1e30f;
+      int bestv = c->ffv[pass][b];
+      for (int v = 0; v < fact_fused_variants(pass); ++v) {
+        if (!fact_fused_shape_ok(pass, v, c->ny, c->nz)) continue;
+        if (launch_fact_fused(f, pass, v, c->stream) != hipSuccess) { rc = QSS_E_HIP; break; }
+        (void)hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps; ++r) (void)launch_fact_fused(f, pass, v, c->stream);
+        (void)hipEventRecord(e1, c->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) { rc = QSS_E_HIP; break; }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) { best = ms; bestv = v; }
+      }
+      c->ffv[pass][b] = bestv;
     }
   // work-groups per tile of the push: one predictor pass (state unchanged)
   // plus the window combine into scratch planes, per candidate

@@ -107,6 +107,32 @@ void fact_matrix(int n, bool transpose, std::vector<double> &g);
 // grids with both sizes multiples of 8 and at least this large use it in FP64
 constexpr int kFactMinN = 256;
 
+// the same with the twiddles inside the GEMMs (fact_fused.hip): pass 0 reads
+// X and writes the y-transformed plane, pass 1 reads that and writes the
+// result through the dense path's epilogue.  Sizes multiples of 128, <= 1024.
+struct FusedArgs {
+  int ny, nz, nbatch, nw;
+  const double *in[kGemmMaxBatch];
+  double *out[kGemmMaxBatch];
+  const double *P[kGemmMaxBatch];
+  long w_in, w_out, w_p;
+  int kz[kGemmMaxBatch], ky[kGemmMaxBatch];
+  int resid_c[kGemmMaxBatch], push[kGemmMaxBatch];
+  const double *G;                   // the pass's permuted table (h x h)
+  const double2 *tw;                 // the pass's axis twiddles (pre, mid, post)
+  double alpha, beta;
+  double *F;
+  float *Ff;
+  long w_f;
+  unsigned long long *rslots;
+  long w_r;
+};
+int fact_fused_variants(int pass);
+bool fact_fused_shape_ok(int pass, int variant, int ny, int nz);
+hipError_t launch_fact_fused(const FusedArgs &a, int pass, int variant, hipStream_t s,
+                             hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+void fact_fused_matrix(int n, int pass, std::vector<double> &g);
+
 // ---------------------------------------------------------------------------
 // adjoint sweep (tertia_amd/csrc/adjoint.hip, qss_sweep_vjp)
 // ---------------------------------------------------------------------------
@@ -195,6 +221,10 @@ struct qss_ctx {
   // factorized transform: eligible, in use, G matrices (y, z), twiddles,
   // second intermediate per batch slot, GEMM variants per pass and batch size
   bool fact_ok = false, fact_use = false;
+  // fused form (fact_fused.hip): eligible, permuted tables (y, z), variants
+  bool ffused_ok = false;
+  double *d_Gf[2] = {};
+  int ffv[2][kGemmMaxBatch + 1] = {};
   double *d_G[2] = {};
   double2 *d_tw[2] = {};
   double *tmp2[kGemmMaxBatch] = {};

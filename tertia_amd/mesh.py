@@ -388,10 +388,12 @@ class Mesh:
         out = {"y_pass": list(arr[1:9]), "z_pass": list(arr[10:18]),
                "particle_split": self.particle_split,
                "transform_method": self.transform_method}
-        fa = (ctypes.c_int * 18)()
+        fa = (ctypes.c_int * 36)()
         check(self.L.qss_fact_choice(self.h, fa), "qss_fact_choice")
         out["fact_y_pass"] = list(fa[1:9])
         out["fact_z_pass"] = list(fa[10:18])
+        out["fused_y_pass"] = list(fa[19:27])
+        out["fused_z_pass"] = list(fa[28:36])
         return out
 
     def set_gemm_choice(self, choice: dict) -> None:
@@ -404,11 +406,15 @@ class Mesh:
             arr[10 + i] = v
         check(self.L.qss_set_gemm_choice(self.h, arr), "qss_set_gemm_choice")
         if "fact_y_pass" in choice:
-            fa = (ctypes.c_int * 18)()
-            for i, v in enumerate(choice["fact_y_pass"]):
-                fa[1 + i] = v
-            for i, v in enumerate(choice["fact_z_pass"]):
-                fa[10 + i] = v
+            fa = (ctypes.c_int * 36)()
+            cur = (ctypes.c_int * 36)()
+            check(self.L.qss_fact_choice(self.h, cur), "qss_fact_choice")
+            for i in range(36):
+                fa[i] = cur[i]
+            for blk, key in enumerate(("fact_y_pass", "fact_z_pass", "fused_y_pass",
+                                       "fused_z_pass")):
+                for i, v in enumerate(choice.get(key, ())):
+                    fa[9 * blk + 1 + i] = v
             check(self.L.qss_set_fact_choice(self.h, fa), "qss_set_fact_choice")
         if "particle_split" in choice:
             self.particle_split = choice["particle_split"]

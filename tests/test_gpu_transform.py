@@ -81,6 +81,8 @@ def test_every_tile_shape(mesh_factory, oracle_mod):
         m = mesh_factory(nz, ny)
         x = np.random.default_rng(11).standard_normal((nz, ny))
         ref = oracle_mod.transform2d(x, 1, 0, use_fft=False, nthreads=4)
+        method = m.transform_method
+        m.transform_method = 0                      # the dense tables' kernels
         for v in range(Mesh.gemm_variants()):
             m.set_gemm_variant(v)
             assert max_rel(m.transform2d(x, 1, 0), ref) < 1e-13, (nz, ny, v)
@@ -88,16 +90,51 @@ def test_every_tile_shape(mesh_factory, oracle_mod):
         choice = m.autotune(2)
         assert all(0 <= c < Mesh.gemm_variants() for c in choice["y_pass"] + choice["z_pass"])
         assert max_rel(m.transform2d(x, 1, 0), ref) < 1e-13
+        m.transform_method = method
 
 
-@pytest.mark.parametrize("nz,ny", [(256, 264), (320, 256), (264, 512)])
+@pytest.mark.parametrize("nz,ny", [(512, 512), (256, 384)])
+def test_every_fused_tile_shape(mesh_factory, oracle_mod, nz, ny):
+    """Every tile shape of the fused factored kernels that fits the grid, for
+    both passes and every kind pair, against the oracle (1e-13); shapes that
+    do not fit are refused."""
+    from tertia_amd._lib import QssError
+    m = mesh_factory(nz, ny)
+    m.transform_method = 1                  # both sizes multiples of 128: the fused form
+    base = m.gemm_choice()
+    x = np.random.default_rng(nz + 7 * ny).standard_normal((nz, ny))
+    refs = {(kz, ky): oracle_mod.transform2d(x, kz, ky, use_fft=True, nthreads=4)
+            for kz in (0, 1) for ky in (0, 1)}
+    tried = 0
+    for v in range(16):
+        for key in ("fused_y_pass", "fused_z_pass"):
+            c = dict(base)
+            c[key] = [v] * 8
+            try:
+                m.set_gemm_choice(c)
+            except QssError:
+                continue
+            assert m.gemm_choice()[key] == [v] * 8
+            tried += 1
+            for (kz, ky), ref in refs.items():
+                assert max_rel(m.transform2d(x, kz, ky), ref) < 1e-13, (key, v, kz, ky)
+    assert tried >= 12
+    m.set_gemm_choice(base)
+
+
+@pytest.mark.parametrize("nz,ny", [(256, 264), (320, 256), (264, 512), (256, 256), (384, 256),
+                                   (512, 384)])
 def test_factored_transform_vs_oracle(mesh_factory, oracle_mod, nz, ny):
     """The factored form (one radix-2 step of the DCT-IV's half-length DFT, two
-    half-size GEMMs between twiddle kernels) and the dense form both match the
-    oracle's O(N^2)-per-row sums to 1e-13 on every kind, and each other."""
+    half-size GEMMs: with the twiddles inside the GEMMs where both sizes are
+    multiples of 128, between twiddle kernels otherwise) and the dense form
+    both match the oracle's O(N^2)-per-row sums to 1e-13 on every kind, and
+    each other."""
     from tertia_amd.params import QSS_KIND_C
     m = mesh_factory(nz, ny)
-    assert m.transform_method == 0          # contexts start dense
+    # contexts start factored where the twiddles run inside the GEMMs (both
+    # sizes multiples of 128), dense elsewhere
+    assert m.transform_method == (1 if nz % 128 == 0 and ny % 128 == 0 else 0)
     m.transform_method = 1                  # FP64, multiples of 8, >= 256: allowed
     rng = np.random.default_rng(nz + ny)
     x = rng.standard_normal((nz, ny))
