@@ -21,13 +21,16 @@ split-BF16-MFMA transforms) and each mode's whole sweep is compared with the
 FP64 sweep: relative L2 over every 8th layer of Ex (the longitudinal field,
 E_z in the usual wakefield notation) and psi.
 
-roofline: the dominant kernel is the transform GEMM; achieved = its
-algorithmic flops (2*M*N*K per GEMM pass and field: 37 2-D transforms of 4N^3
-per slice) / its summed duration over the instrumented window, each launch
-timed by a HIP event pair that the dispatch itself stamps with its start and
-end (hipExtLaunchKernel, on the context's stream), against MI355X's 78.6
-TFLOP/s FP64 MFMA peak.  traffic = HBM bytes per launch from the committed
-rocprofv3 FETCH_SIZE/WRITE_SIZE summary.  roofline_push: the push+deposit
+roofline: the dominant kernel is the transform GEMM; achieved = the flops of
+the transform method in use (2*M*N*K per GEMM pass and field: 4N^3 per 2-D
+transform with the dense tables, 2N^3 with the factored form, whose radix-2
+step halves the multiply-adds; 35 2-D transforms per slice) / its summed
+duration over the instrumented window, each launch timed by a HIP event pair
+that the dispatch itself stamps with its start and end (hipExtLaunchKernel,
+on the context's stream), against MI355X's 78.6 TFLOP/s FP64 MFMA peak;
+dense_equivalent_tflops restates the rate in the dense tables' flops.
+traffic = HBM bytes per launch from the committed rocprofv3
+FETCH_SIZE/WRITE_SIZE summary of that kernel.  roofline_push: the push+deposit
 kernel against the 8 TB/s HBM peak, its algorithmic bytes per launch being
 Np*52 B of particle state + 16 N^2 planes of field reads / deposit writes.
 
@@ -93,9 +96,10 @@ def parse():
     ap.add_argument("--batch-slices", type=int, default=512)
     ap.add_argument("--driver-pushes", type=int, default=10,
                     help="evolving-driver pushes timed after one time step (0: skip)")
-    ap.add_argument("--gemm-tiles", default="", metavar="Y/Z",
+    ap.add_argument("--gemm-tiles", default="", metavar="Y/Z[/FY/FZ]",
                     help="FP64 transform tile variants per batch size 1..8 for the y and z "
-                         "passes, e.g. 11,12,11,13,11,14,0,0/11,12,12,13,11,15,0,2 "
+                         "passes, e.g. 11,12,11,13,11,14,0,0/11,12,12,13,11,15,0,2, then "
+                         "optionally those of the fused factored kernels "
                          "(default: the autotuner's choice)")
     ap.add_argument("--transform", default="auto", choices=("auto", "dense", "factored"),
                     help="2-D transform method of the FP64 sweeps (auto: the context's "
@@ -239,10 +243,15 @@ def timed_run(mode, a):
             m.transform_method = 1 if a.transform == "factored" else 0
         choice = m.autotune(3)
         if a.gemm_tiles and mode == "fp64":
-            # a fixed tile table (y-pass and z-pass variant per batch size 1..8)
-            # instead of the tuned one, e.g. to repeat a run's kernels exactly
-            y, z = ([int(v) for v in part.split(",")] for part in a.gemm_tiles.split("/"))
-            m.set_gemm_choice({"y_pass": y, "z_pass": z})
+            # a fixed tile table (y-pass and z-pass variant per batch size 1..8;
+            # then optionally those of the fused factored kernels) instead of
+            # the tuned one, e.g. to repeat a run's kernels exactly
+            parts = [[int(v) for v in part.split(",")] for part in a.gemm_tiles.split("/")]
+            c = m.gemm_choice()
+            c.update({"y_pass": parts[0], "z_pass": parts[1]})
+            if len(parts) == 4:
+                c.update({"fused_y_pass": parts[2], "fused_z_pass": parts[3]})
+            m.set_gemm_choice(c)
             choice.update(m.gemm_choice())
         if a.particle_split:
             # work-groups per push tile fixed instead of tuned (the tuned
@@ -647,11 +656,18 @@ def main():
     }
     if "gemm" in base:
         g = base["gemm"]
+        factored = base["gemm_tiles"].get("transform_method") == 1
         line["roofline"] = {"bound": "mfma", "achieved": g["achieved_tflops"],
                             "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
                             "frac": g["achieved_tflops"] / PEAK_TFLOPS["fp64"],
-                            "traffic": profiled_traffic("k_gemm_fp64_512x512"),
-                            "kernel": "k_gemm<MathF64,...> (transform GEMM)",
+                            "traffic": profiled_traffic("k_fact_fp64_512x512" if factored
+                                                        else "k_gemm_fp64_512x512"),
+                            "kernel": ("k_fact_y / k_fact_z (fused factored transform GEMMs, "
+                                       "2N^3 flop per 2-D transform)" if factored else
+                                       "k_gemm<MathF64,...> (transform GEMM, 4N^3 flop per "
+                                       "2-D transform)"),
+                            "dense_equivalent_tflops": g["achieved_tflops"] * (2 if factored
+                                                                               else 1),
                             "avg_launch_us": g["avg_launch_us"], "launches": g["launches"]}
         pu = base["push"]
         line["roofline_push"] = {"bound": "hbm", "achieved": pu["achieved_gbs"], "peak": 8000.0,
