@@ -70,8 +70,10 @@ __global__ __launch_bounds__(NT) void k_push_tile_adj(AdjTileArgs a) {
 #pragma unroll
     for (int q = 0; q < 3; ++q) Jw[i * 3 + q] = a.Jb[q * plane + g];
   }
-  double acc0[6] = {0, 0, 0, 0, 0, 0}, acc1[6] = {0, 0, 0, 0, 0, 0};
-  const int n0 = t, n1 = t + NT;
+  // this work-group's window: chunk sums written by the first chunk, added
+  // by later ones (each thread owns nodes t and t + NT)
+  double *wout = a.fwin + (long)blockIdx.x * NF * 6;
+  bool first = true;
   __syncthreads();
 
   qss_adj::Consts cs;
@@ -97,14 +99,8 @@ __global__ __launch_bounds__(NT) void k_push_tile_adj(AdjTileArgs a) {
       const double y0 = a.y[slot], z0 = a.z[slot];
       const double P0[3] = {a.px[slot], a.py[slot], a.pz[slot]};
       const long id = a.pid[slot];
-      double Ybar = 0, Zbar = 0, Pbar[3] = {0, 0, 0};
-      if (a.full) {
-        Ybar = a.pin[id];
-        Zbar = a.pin[np + id];
-        Pbar[0] = a.pin[2 * np + id];
-        Pbar[1] = a.pin[3 * np + id];
-        Pbar[2] = a.pin[4 * np + id];
-      }
+      // adjoint of the new position (0, 1) and momentum (2..4), full step only
+      auto NB = [&](int i) { return a.pin[i * np + id]; };
       auto Fn = [&](int q, int dj, int dk) { return Fw[((lj + dj) + FW * (lk + dk)) * 6 + q]; };
       bool far = false;
       auto JBn = [&](int q, int nj, int nk) -> double {
@@ -121,7 +117,7 @@ __global__ __launch_bounds__(NT) void k_push_tile_adj(AdjTileArgs a) {
         return Jw[((dk + H) * W + dj + H) * 3 + q];
       };
       qss_adj::AdjOut o;
-      qss_adj::push_adjoint(cs, j, k, y0, z0, P0, Fn, JBn, a.full != 0, Ybar, Zbar, Pbar, o);
+      qss_adj::push_adjoint_cb(cs, j, k, y0, z0, P0, Fn, JBn, a.full != 0, NB, o);
       if (far) atomicOr(&a.counters[1], ST_FAR);
       if (a.full) {
         a.pout[id] = o.y0;
@@ -171,9 +167,9 @@ __global__ __launch_bounds__(NT) void k_push_tile_adj(AdjTileArgs a) {
     __syncthreads();
 #pragma unroll
     for (int oi = 0; oi < 2; ++oi) {
-      const int n = oi == 0 ? n0 : n1;
+      const int n = t + oi * NT;
       if (n >= NF) continue;
-      double *accn = oi == 0 ? acc0 : acc1;
+      double accn[6] = {0, 0, 0, 0, 0, 0};
       const int nl_j = n % FW, nl_k = n / FW;
 #pragma unroll
       for (int cn = 0; cn < 4; ++cn) {
@@ -188,15 +184,16 @@ __global__ __launch_bounds__(NT) void k_push_tile_adj(AdjTileArgs a) {
           for (int q = 0; q < 6; ++q) accn[q] += wv * ev_f[e * 6 + q];
         }
       }
+#pragma unroll
+      for (int q = 0; q < 6; ++q) wout[n * 6 + q] = first ? accn[q] : wout[n * 6 + q] + accn[q];
     }
+    first = false;
     __syncthreads();
   }
-  double *wout = a.fwin + (long)blockIdx.x * NF * 6;   // window of this part
+  if (first)   // no particles for this work-group
+    for (int n = t; n < NF; n += NT)
 #pragma unroll
-  for (int q = 0; q < 6; ++q) wout[n0 * 6 + q] = acc0[q];
-  if (n1 < NF)
-#pragma unroll
-    for (int q = 0; q < 6; ++q) wout[n1 * 6 + q] = acc1[q];
+      for (int q = 0; q < 6; ++q) wout[n * 6 + q] = 0.0;
 }
 
 // out[q][node] = sum of the 17x17 windows holding the node, fixed order

@@ -164,6 +164,23 @@ struct AdjOut {
   double w[4];     // CIC weights at the start position (for the scatter)
 };
 
+// x unchanged, but opaque to the device compiler: values derived from it are
+// recomputed rather than kept in registers from an earlier identical
+// computation (the adjoint recomputes the kinematics after the sub-step
+// replay instead of holding ~35 doubles across it)
+QSS_HD inline double fresh(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  __asm__ volatile("" : "+v"(x));
+#endif
+  return x;
+}
+QSS_HD inline int fresh(int x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  __asm__ volatile("" : "+v"(x));
+#endif
+  return x;
+}
+
 // a x b
 QSS_HD inline void cross(const double a[3], const double b[3], double r[3]) {
   r[0] = a[1] * b[2] - a[2] * b[1];
@@ -171,35 +188,65 @@ QSS_HD inline void cross(const double a[3], const double b[3], double r[3]) {
   r[2] = a[0] * b[1] - a[1] * b[0];
 }
 
+// Gathered fields (and, with gy/gz, their derivatives along y and z) at the
+// start position; the corner values are not kept.
+template <class FN>
+QSS_HD inline void gather6(FN F, int o0, const double w[4], const double *wy, const double *wz,
+                           double f[6], double *gy, double *gz) {
+  for (int q = 0; q < 6; ++q) {
+    double Fc[4];
+    for (int cn = 0; cn < 4; ++cn) Fc[cn] = F(q, (cn & 1) + o0, (cn >> 1) + o0);
+    f[q] = w[0] * Fc[0] + w[1] * Fc[1] + w[2] * Fc[2] + w[3] * Fc[3];
+    if (gy) {
+      gy[q] = 0;
+      gz[q] = 0;
+      for (int cn = 0; cn < 4; ++cn) {
+        gy[q] += Fc[cn] * wy[cn];
+        gz[q] += Fc[cn] * wz[cn];
+      }
+    }
+  }
+}
+
 // Reverse of one particle's push.  F(q, dj, dk) returns the gathered field q
 // at corner (j+dj, k+dk) of the start cell; JB(q, nj, nk) the deposit adjoint
 // of component q (0..3 = Jx, Jy, Jz, rho) at node (nj, nk), indices taken
-// periodically by the callee.  full: the new position/momentum adjoints
-// (Ybar, Zbar, Pbar) are given.  Returns false if the particle's displacement
-// was frozen by the sub-step guard (then only the frozen path's derivative
-// is returned).
-template <class FN, class JBN>
-QSS_HD inline bool push_adjoint(const Consts &c, int j, int k, double y0, double z0,
-                                const double P0[3], FN F, JBN JB, bool full, double Ybar,
-                                double Zbar, const double Pbar[3], AdjOut &o) {
-  double w[4], wy[4], wz[4];
-  cic_weights(y0, z0, w, wy, wz);
-  double f[6], Fc[4][6];
-  for (int q = 0; q < 6; ++q) {
-    for (int cn = 0; cn < 4; ++cn) Fc[cn][q] = F(q, cn & 1, cn >> 1);
-    f[q] = w[0] * Fc[0][q] + w[1] * Fc[1][q] + w[2] * Fc[2][q] + w[3] * Fc[3][q];
+// periodically by the callee.  full: the new position/momentum adjoints are
+// given by NB(i) (i = 0, 1: Ybar, Zbar; 2..4: Pbar), read where they are
+// used.  Returns false if the particle's displacement was frozen by the
+// sub-step guard (then only the frozen path's derivative is returned).
+//
+// Registers: the gathers and the kinematics are computed again where they
+// are needed (from the same inputs, so bitwise the same values) rather than
+// held across the sub-step replay.
+template <class FN, class JBN, class NBN>
+QSS_HD inline bool push_adjoint_cb(const Consts &c, int j, int k, double y0, double z0,
+                                   const double P0[3], FN F, JBN JB, bool full, NBN NB,
+                                   AdjOut &o) {
+  // what the sub-step replay needs of the kinematics
+  int n;
+  double dy, dz, s_div, djk[4];
+  bool frozen;
+  {
+    double w[4], wy[4], wz[4], f[6];
+    cic_weights(y0, z0, w, wy, wz);
+    gather6(F, 0, w, nullptr, nullptr, f, nullptr, nullptr);
+    Kin k1;
+    kinematics(c, P0, f, k1);
+    n = k1.nsub;
+    dy = k1.d[0];
+    dz = k1.d[1];
+    s_div = k1.s;
+    frozen = k1.frozen;
+    for (int q = 0; q < 4; ++q) djk[q] = k1.dj[q];
   }
-  Kin kn;
-  kinematics(c, P0, f, kn);
-  const int n = kn.nsub;
-  const double dy = kn.d[0], dz = kn.d[1];
 
   // --- sub-step replay: deposit adjoint and the position chain ---------
   double a_dj[4] = {0, 0, 0, 0};
   double A_y = 0, A_z = 0, PA_y = 0, PA_z = 0, Wy = 0, Wz = 0;
   double pdep_lt = 0, pdep_n = 0, afy_n = 0, afz_n = 0;
   {
-    Parts pts(kn.s);
+    Parts pts(s_div);
     double yt = y0, zt = z0, Pcum = 0;
     int jj = j, kk = k;
     for (int i = 1; i <= n; ++i) {
@@ -218,13 +265,13 @@ QSS_HD inline bool push_adjoint(const Consts &c, int j, int k, double y0, double
           Gz += jb * swz[cn];
         }
         a_dj[q] += part * G;
-        a_p += kn.dj[q] * G;
-        afy += kn.dj[q] * part * Gy;
-        afz += kn.dj[q] * part * Gz;
+        a_p += djk[q] * G;
+        afy += djk[q] * part * Gy;
+        afz += djk[q] * part * Gz;
       }
       if (full && i == n) {
-        afy += Ybar;
-        afz += Zbar;
+        afy += NB(0);
+        afz += NB(1);
       }
       A_y += afy;
       A_z += afz;
@@ -238,16 +285,27 @@ QSS_HD inline bool push_adjoint(const Consts &c, int j, int k, double y0, double
     }
   }
   double a_d[2] = {PA_y, PA_z};
-  if (kn.s > 1.) {
-    const double s2 = kn.s * kn.s;
+  if (s_div > 1.) {
+    const double s2 = s_div * s_div;
     const double a_s = -(pdep_lt + dy * Wy + dz * Wz) / s2 +
                        (double)(n - 1) / s2 * (pdep_n + dy * afy_n + dz * afz_n);
     if (fabs(dy) > fabs(dz)) a_d[0] += (dy > 0 ? 1. : -1.) * a_s;
     else a_d[1] += (dz > 0 ? 1. : -1.) * a_s;
   }
-  if (kn.frozen) { a_d[0] = 0.; a_d[1] = 0.; }
+  if (frozen) { a_d[0] = 0.; a_d[1] = 0.; }
 
   // --- kinematics reverse -------------------------------------------------
+  // the same kinematics again, from inputs the compiler cannot match with the
+  // first call's (bitwise the same values)
+  Kin kn;
+  {
+    const double yr = fresh(y0), zr = fresh(z0);
+    double Pr[3], w[4], wy[4], wz[4], f[6];
+    for (int q = 0; q < 3; ++q) Pr[q] = fresh(P0[q]);
+    cic_weights(yr, zr, w, wy, wz);
+    gather6(F, fresh(0), w, nullptr, nullptr, f, nullptr, nullptr);
+    kinematics(c, Pr, f, kn);
+  }
   const double cy = c.hx / c.hy, cz = c.hx / c.hz, v = kn.v;
   const double wdj = c.weight * kDj0;
   double a_v = 0;
@@ -267,7 +325,7 @@ QSS_HD inline bool push_adjoint(const Consts &c, int j, int k, double y0, double
     const double *P1 = kn.P1, g2 = kn.g2;
     const double dot = P1[0] * a_Vm[0] + P1[1] * a_Vm[1] + P1[2] * a_Vm[2];
     for (int q = 0; q < 3; ++q)
-      a_P1[q] = a_Vm[q] / g2 - P1[q] * dot / (g2 * g2 * g2) + (full ? Pbar[q] : 0.);
+      a_P1[q] = a_Vm[q] / g2 - P1[q] * dot / (g2 * g2 * g2) + (full ? NB(2 + q) : 0.);
   }
   double a_H[3], a_K[3], a_R[3], a_p3[3], a_b[3], a_Q[3];
   for (int q = 0; q < 3; ++q) { a_H[q] = a_P1[q]; a_K[q] = a_P1[q]; a_R[q] = a_P1[q]; }
@@ -312,20 +370,30 @@ QSS_HD inline bool push_adjoint(const Consts &c, int j, int k, double y0, double
   }
   // gather weights
   double ay = A_y, az = A_z;
-  for (int q = 0; q < 6; ++q) {
-    double gy = 0, gz = 0;
-    for (int cn = 0; cn < 4; ++cn) {
-      gy += Fc[cn][q] * wy[cn];
-      gz += Fc[cn][q] * wz[cn];
+  {
+    const double yr = fresh(y0), zr = fresh(z0);
+    double w[4], wy[4], wz[4], f[6], gy[6], gz[6];
+    cic_weights(yr, zr, w, wy, wz);
+    gather6(F, fresh(0), w, wy, wz, f, gy, gz);
+    for (int q = 0; q < 6; ++q) {
+      ay += o.f[q] * gy[q];
+      az += o.f[q] * gz[q];
     }
-    ay += o.f[q] * gy;
-    az += o.f[q] * gz;
+    for (int cn = 0; cn < 4; ++cn) o.w[cn] = w[cn];
   }
   o.y0 = ay;
   o.z0 = az;
   for (int q = 0; q < 3; ++q) o.P0[q] = a_P0[q];
-  for (int cn = 0; cn < 4; ++cn) o.w[cn] = w[cn];
   return !kn.frozen;
+}
+
+// The same with the final-state adjoints given as values.
+template <class FN, class JBN>
+QSS_HD inline bool push_adjoint(const Consts &c, int j, int k, double y0, double z0,
+                                const double P0[3], FN F, JBN JB, bool full, double Ybar,
+                                double Zbar, const double Pbar[3], AdjOut &o) {
+  auto nb = [&](int i) { return i == 0 ? Ybar : i == 1 ? Zbar : Pbar[i - 2]; };
+  return push_adjoint_cb(c, j, k, y0, z0, P0, F, JB, full, nb, o);
 }
 
 }  // namespace qss_adj
