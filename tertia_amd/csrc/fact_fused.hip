@@ -8,13 +8,18 @@
 //   y_2k = 2 Re w_k,  y_{N-1-2k} = -2 Im w_k
 // (x~ the reversed input and the odd outputs negated for a DST-IV).  The two
 // m-point DFTs are one real GEMM with K = h against G = [[Fr, Fi], [-Fi, Fr]].
+// The pre-twiddle is split as pre_{2q+p} = pre_{2q} e^{-i pi p/N}: pre_{2q}
+// depends on the GEMM's K index q only and is folded into the table (entries
+// H_qc = pre_{2q} F^{qc}), and e^{-i pi/N} of the odd parity into the
+// butterfly twiddle (mid'_k = mid_k e^{-i pi/N}), so the operand loaders only
+// gather the raw pairs (x~_2n, x~_{N-1-2n}).
 //
 // The orders of the GEMM's K, M and N indices are free (the tables are built
 // for them on the host), and are chosen so that
 //   * K: k' = 4u + j pairs q = u (j = 0, 1: Re, Im) with q'' = m-1-u (j = 2, 3).
-//     The v_n of both come from the same two 32-byte chunks of the input,
+//     The pairs of both come from the same two 32-byte chunks of the input,
 //     x~[4u .. 4u+3] and x~[N-4u-4 .. N-4u-1]: the operand loader reads those
-//     and applies the pre-twiddles on the way to LDS, using every byte.
+//     and places them in LDS, using every byte.
 //   * rows (the parity p of v_{2q+p} in the y pass, Re/Im of the output
 //     frequency in the z pass): row t of a 16-row MFMA tile holds index
 //     (t&3) + 4(t>>3) and parity (t>>2)&1.  The FP64 MFMA leaves D[lk+4r][li]
@@ -46,7 +51,7 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kXcdF = 8;
-constexpr int kMaxH = 512;   // N <= 1024: pre-twiddles of one axis in LDS
+constexpr int kMaxN = 1024;   // largest axis the tile shapes are checked for
 
 __device__ inline double2 cmulf(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -91,7 +96,7 @@ __device__ inline FItem fitem(int GM, int GN, int nbatch) {
 // y pass: virtual A (2 Nz x h) = pre-twiddled rows of X, B = G'_y (h x h),
 // output Y (Nz x Ny, natural layout) after the butterflies along y
 // ---------------------------------------------------------------------------
-template <int BM, int BN, int BK, int WGM, int WGN, int KS>
+template <int BM, int BN, int BK, int WGM, int WGN, int KS, int PF>
 __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_y(FusedArgs a) {
   constexpr int NWT = WGM * WGN, NT = NWT * KS * 64;
   constexpr int WTM = BM / WGM, WTN = BN / WGN;
@@ -116,7 +121,6 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_y(FusedArgs a) {
   constexpr size_t kSlab = sizeof(double) * 2 * BK * (BM + BN);
   constexpr size_t kRed = KS > 1 ? sizeof(d4) * (KS - 1) * NWT * TI * TJ * 64 : 0;
   __shared__ __attribute__((aligned(32))) char smem[kSlab > kRed ? kSlab : kRed];
-  __shared__ double2 pre[kMaxH];
   double(*As)[BK][BM] = reinterpret_cast<double(*)[BK][BM]>(smem);
   double(*Bs)[BK][BN] = reinterpret_cast<double(*)[BK][BN]>(smem + sizeof(double) * 2 * BK * BM);
 
@@ -124,8 +128,6 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_y(FusedArgs a) {
   const int ks = wv / NWT, wl = wv - ks * NWT;
   const int wm = wl / WGN, wn = wl - wm * WGN;
   const int li = lane & 15, lk = lane >> 4;
-  for (int i = t; i < h; i += NT) pre[i] = a.tw[i];
-  __syncthreads();
 
   const double *X = a.in[fb] + it.w * a.w_in;
   const double *G = a.G;
@@ -134,11 +136,15 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_y(FusedArgs a) {
   // unit U: w = U & 1, p = (U >> 1) & 1, u = k0/4 + (U >> 2) % (BK/4), row
   // xl = (U >> 2) / (BK/4).  With the chunks f = x~[4u .. 4u+3] and
   // b = x~[N-4u-4 .. N-4u-1] (x~ = x, or x reversed for the DST: x~[4u+i] =
-  // x[N-1-4u-i], x~[N-4u-4+i] = x[4u+3-i]):
-  //   w = 0: v_{2u+p}        = (f[2p] + i b[3-2p]) pre_{2u+p}
-  //   w = 1: v_{2(m-1-u)+p}  = (b[2p] + i f[3-2p]) pre_{2(m-1-u)+p}
-  double ra[2][UPT][2], rb[2][BPT][2];
-  auto load = [&](int k0, double (*ra_)[2], double (*rb_)[2]) {
+  // x[N-1-4u-i], x~[N-4u-4+i] = x[4u+3-i]), the pair of
+  //   w = 0: v_{2u+p}        is (f[2p], b[3-2p])
+  //   w = 1: v_{2(m-1-u)+p}  is (b[2p], f[3-2p])
+  // PF register sets of the slab loads
+  double ra[PF][UPT][2], rb[PF][BPT][2];
+  auto load = [&](auto S, int k0) {
+    constexpr int rs = decltype(S)::value;
+    double(*ra_)[2] = ra[rs];
+    double(*rb_)[2] = rb[rs];
 #pragma unroll
     for (int q = 0; q < UPT; ++q) {
       const int U = q * NT + t;
@@ -162,20 +168,20 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_y(FusedArgs a) {
       rb_[q][1] = v.y;
     }
   };
-  auto store = [&](int buf, int k0, const double (*ra_)[2], const double (*rb_)[2]) {
+  auto store = [&](auto S, int buf) {
+    constexpr int rs = decltype(S)::value;
+    const double(*ra_)[2] = ra[rs];
+    const double(*rb_)[2] = rb[rs];
 #pragma unroll
     for (int q = 0; q < UPT; ++q) {
       const int U = q * NT + t;
       if (UNITS % NT != 0 && U >= UNITS) continue;
       const int w = U & 1, pp = (U >> 1) & 1, ul = (U >> 2) % (BK / 4), xl = (U >> 2) / (BK / 4);
-      const int u = k0 / 4 + ul;
-      const double2 v =
-          cmulf(make_double2(ra_[q][0], ra_[q][1]), pre[w == 0 ? 2 * u + pp : 2 * (m - 1 - u) + pp]);
       const int x8 = xl & 7;
       const int rr = (xl >> 3) * 16 + (x8 & 3) + 8 * (x8 >> 2) + 4 * pp;
       const int kr = 4 * ul + 2 * w;
-      As[buf][kr][rr ^ SwzF64::xa(kr)] = v.x;
-      As[buf][kr + 1][rr ^ SwzF64::xa(kr + 1)] = v.y;
+      As[buf][kr][rr ^ SwzF64::xa(kr)] = ra_[q][0];
+      As[buf][kr + 1][rr ^ SwzF64::xa(kr + 1)] = ra_[q][1];
     }
 #pragma unroll
     for (int q = 0; q < BPT; ++q) {
@@ -208,13 +214,7 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_y(FusedArgs a) {
         for (int j = 0; j < TJ; ++j) acc[i][j] = mma64(av[i], bv[j], acc[i][j]);
     }
   };
-  // slab k0 of each register set: k_loop hands the stores only the buffer
-  int kst[2] = {0, 0};
-  k_loop<false, BK>(
-      K, [&](int k0) { kst[0] = k0; load(k0, ra[0], rb[0]); },
-      [&](int k0) { kst[1] = k0; load(k0, ra[1], rb[1]); },
-      [&](int buf) { store(buf, kst[0], ra[0], rb[0]); },
-      [&](int buf) { store(buf, kst[1], ra[1], rb[1]); }, compute);
+  k_loop_pf<PF, BK>(K, load, store, compute);
   if constexpr (KS > 1) {
     // partial sums of waves ks = 1..KS-1 through LDS, added in ks order
     d4(*red)[NWT][TI * TJ][64] = reinterpret_cast<d4(*)[NWT][TI * TJ][64]>(smem);
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_y(FusedArgs a) {
   // butterflies along y: frequency k = 16 g + li and k2 = m-1-k of rows
   // xrow = (m0 + wm WTM + 16 i)/2 + lk + 4 hh, one 64-column group g per
   // four accumulator tiles
-  const double2 *mid = a.tw + h, *post = a.tw + h + m;
+  const double2 *mid = a.tw, *post = a.tw + m;   // mid' (m), post (h)
   const double sg = rev ? 2.0 : -2.0;
   double *Y = a.out[fb] + it.w * a.w_out;
 #pragma unroll
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_y(FusedArgs a) {
 // z pass: A = G'_z (h_z x h_z), virtual B (h_z x 2 Ny) = pre-twiddled columns
 // of Y, output C (Nz x Ny) after the butterflies along z and the epilogue
 // ---------------------------------------------------------------------------
-template <int BM, int BN, int BK, int WGM, int WGN, int KS>
+template <int BM, int BN, int BK, int WGM, int WGN, int KS, int PF>
 __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_z(FusedArgs a) {
   constexpr int NWT = WGM * WGN, NT = NWT * KS * 64;
   constexpr int WTM = BM / WGM, WTN = BN / WGN;
@@ -299,7 +299,6 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_z(FusedArgs a) {
   constexpr size_t kSlab = sizeof(double) * 2 * BK * (BM + BN);
   constexpr size_t kRed = KS > 1 ? sizeof(d4) * (KS - 1) * NWT * TI * TJ * 64 : 0;
   __shared__ __attribute__((aligned(32))) char smem[kSlab > kRed ? kSlab : kRed];
-  __shared__ double2 pre[kMaxH];
   double(*As)[BK][BM] = reinterpret_cast<double(*)[BK][BM]>(smem);
   double(*Bs)[BK][BN] = reinterpret_cast<double(*)[BK][BN]>(smem + sizeof(double) * 2 * BK * BM);
 
@@ -307,8 +306,6 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_z(FusedArgs a) {
   const int ks = wv / NWT, wl = wv - ks * NWT;
   const int wm = wl / WGN, wn = wl - wm * WGN;
   const int li = lane & 15, lk = lane >> 4;
-  for (int i = t; i < h; i += NT) pre[i] = a.tw[i];
-  __syncthreads();
 
   const double *Yp = a.in[fb] + it.w * a.w_in;
   const double *G = a.G;
@@ -317,8 +314,11 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_z(FusedArgs a) {
   // unit U: yl = U % (BN/2), then p, w, u from U / (BN/2) as in the y pass;
   // front rows 4u .. 4u+3 and back rows Nz-4u-4 .. Nz-4u-1 of column y0 + yl
   // (the row order reversed for the DST)
-  double ra[2][APT][2], rb[2][UPT][2];
-  auto load = [&](int k0, double (*ra_)[2], double (*rb_)[2]) {
+  double ra[PF][APT][2], rb[PF][UPT][2];
+  auto load = [&](auto S, int k0) {
+    constexpr int rs = decltype(S)::value;
+    double(*ra_)[2] = ra[rs];
+    double(*rb_)[2] = rb[rs];
 #pragma unroll
     for (int q = 0; q < APT; ++q) {
       if (ACH % NT != 0 && q * NT + t >= ACH) continue;
@@ -342,7 +342,10 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_z(FusedArgs a) {
       rb_[q][1] = col[(long)(rev ? Nz - 1 - c1 : c1) * Ny];
     }
   };
-  auto store = [&](int buf, int k0, const double (*ra_)[2], const double (*rb_)[2]) {
+  auto store = [&](auto S, int buf) {
+    constexpr int rs = decltype(S)::value;
+    const double(*ra_)[2] = ra[rs];
+    const double(*rb_)[2] = rb[rs];
 #pragma unroll
     for (int q = 0; q < APT; ++q) {
       if (ACH % NT != 0 && q * NT + t >= ACH) continue;
@@ -357,13 +360,10 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_z(FusedArgs a) {
       if (UNITS % NT != 0 && U >= UNITS) continue;
       const int yl = U % (BN / 2), r = U / (BN / 2);
       const int w = r & 1, pp = (r >> 1) & 1, ul = r >> 2;
-      const int u = k0 / 4 + ul;
-      const double2 v =
-          cmulf(make_double2(rb_[q][0], rb_[q][1]), pre[w == 0 ? 2 * u + pp : 2 * (m - 1 - u) + pp]);
       const int cc = 32 * (yl >> 4) + (yl & 15) + 16 * pp;
       const int kr = 4 * ul + 2 * w;
-      Bs[buf][kr][cc ^ SwzF64::xb(kr)] = v.x;
-      Bs[buf][kr + 1][cc ^ SwzF64::xb(kr + 1)] = v.y;
+      Bs[buf][kr][cc ^ SwzF64::xb(kr)] = rb_[q][0];
+      Bs[buf][kr + 1][cc ^ SwzF64::xb(kr + 1)] = rb_[q][1];
     }
   };
 
@@ -388,12 +388,7 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_z(FusedArgs a) {
         for (int j = 0; j < TJ; ++j) acc[i][j] = mma64(av[i], bv[j], acc[i][j]);
     }
   };
-  int kst[2] = {0, 0};
-  k_loop<false, BK>(
-      K, [&](int k0) { kst[0] = k0; load(k0, ra[0], rb[0]); },
-      [&](int k0) { kst[1] = k0; load(k0, ra[1], rb[1]); },
-      [&](int buf) { store(buf, kst[0], ra[0], rb[0]); },
-      [&](int buf) { store(buf, kst[1], ra[1], rb[1]); }, compute);
+  k_loop_pf<PF, BK>(K, load, store, compute);
   if constexpr (KS > 1) {
     d4(*red)[NWT][TI * TJ][64] = reinterpret_cast<d4(*)[NWT][TI * TJ][64]>(smem);
     __syncthreads();
@@ -415,7 +410,7 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_z(FusedArgs a) {
 
   // butterflies along z and the epilogue: frequency kf of rows
   // (m0 + wm WTM + 16 i)/2 + lk + 4 hh, column y = (n0 + wn WTN)/2 + 16 g + li
-  const double2 *mid = a.tw + h, *post = a.tw + h + m;
+  const double2 *mid = a.tw, *post = a.tw + m;   // mid' (m), post (h)
   const double sg = rev ? 2.0 : -2.0;
   const long nc = (long)Ny * Nz;
   double *C = a.out[fb] + it.w * a.w_out;
@@ -478,25 +473,31 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_z(FusedArgs a) {
   }
 }
 
-// tile shapes: {BM, BN, BK, WGM, WGN, KS}
+// tile shapes: {BM, BN, BK, WGM, WGN, KS, PF (register slabs in flight)}
 struct FShape {
-  int bm, bn, bk, wgm, wgn, ks;
+  int bm, bn, bk, wgm, wgn, ks, pf;
 };
 constexpr FShape kShapesY[] = {
-    {32, 64, 16, 2, 1, 1},  {64, 64, 16, 4, 1, 1},  {32, 64, 16, 2, 1, 2},  {32, 64, 16, 2, 1, 4},
-    {32, 64, 32, 2, 1, 4},  {64, 64, 16, 4, 1, 2},  {32, 64, 32, 2, 1, 2},  {32, 128, 16, 2, 1, 1},
-    {32, 128, 16, 2, 1, 2}, {64, 128, 16, 4, 1, 1}, {32, 256, 16, 2, 1, 1}};
+    {32, 64, 16, 2, 1, 1, 2},  {64, 64, 16, 4, 1, 1, 2},  {32, 64, 16, 2, 1, 2, 2},
+    {32, 64, 16, 2, 1, 4, 2},  {32, 64, 32, 2, 1, 4, 2},  {64, 64, 16, 4, 1, 2, 2},
+    {32, 64, 32, 2, 1, 2, 2},  {32, 128, 16, 2, 1, 1, 2}, {32, 128, 16, 2, 1, 2, 2},
+    {64, 128, 16, 4, 1, 1, 2}, {32, 256, 16, 2, 1, 1, 2}, {32, 64, 16, 2, 1, 2, 4},
+    {32, 64, 16, 2, 1, 4, 4},  {64, 64, 16, 4, 1, 2, 4},  {32, 64, 16, 2, 1, 1, 4},
+    {64, 64, 16, 4, 1, 1, 4}};
 constexpr FShape kShapesZ[] = {
-    {32, 64, 16, 2, 1, 1},  {64, 64, 16, 4, 1, 1},  {32, 32, 16, 2, 1, 2},  {32, 32, 16, 2, 1, 4},
-    {32, 64, 16, 2, 1, 2},  {32, 64, 16, 2, 1, 4},  {32, 32, 32, 2, 1, 4},  {64, 64, 16, 4, 1, 2},
-    {128, 32, 16, 4, 1, 2}, {128, 32, 16, 8, 1, 1}, {128, 64, 16, 4, 1, 1}, {128, 32, 16, 4, 1, 1}};
+    {32, 64, 16, 2, 1, 1, 2},  {64, 64, 16, 4, 1, 1, 2},  {32, 32, 16, 2, 1, 2, 2},
+    {32, 32, 16, 2, 1, 4, 2},  {32, 64, 16, 2, 1, 2, 2},  {32, 64, 16, 2, 1, 4, 2},
+    {32, 32, 32, 2, 1, 4, 2},  {64, 64, 16, 4, 1, 2, 2},  {128, 32, 16, 4, 1, 2, 2},
+    {128, 32, 16, 8, 1, 1, 2}, {128, 64, 16, 4, 1, 1, 2}, {128, 32, 16, 4, 1, 1, 2},
+    {32, 32, 16, 2, 1, 2, 4},  {32, 32, 16, 2, 1, 4, 4},  {32, 64, 16, 2, 1, 2, 4},
+    {64, 64, 16, 4, 1, 2, 4},  {32, 64, 16, 2, 1, 1, 4}};
 constexpr int kNY = sizeof(kShapesY) / sizeof(kShapesY[0]);
 constexpr int kNZ = sizeof(kShapesZ) / sizeof(kShapesZ[0]);
 
 template <int V>
 hipError_t launch_y(const FusedArgs &a, dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   constexpr FShape S = kShapesY[V];
-  auto fn = k_fact_y<S.bm, S.bn, S.bk, S.wgm, S.wgn, S.ks>;
+  auto fn = k_fact_y<S.bm, S.bn, S.bk, S.wgm, S.wgn, S.ks, S.pf>;
   const dim3 blk(S.wgm * S.wgn * S.ks * 64);
   if (e0 || e1) hipExtLaunchKernelGGL(fn, grid, blk, 0, s, e0, e1, 0, a);
   else hipLaunchKernelGGL(fn, grid, blk, 0, s, a);
@@ -505,7 +506,7 @@ hipError_t launch_y(const FusedArgs &a, dim3 grid, hipStream_t s, hipEvent_t e0,
 template <int V>
 hipError_t launch_z(const FusedArgs &a, dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   constexpr FShape S = kShapesZ[V];
-  auto fn = k_fact_z<S.bm, S.bn, S.bk, S.wgm, S.wgn, S.ks>;
+  auto fn = k_fact_z<S.bm, S.bn, S.bk, S.wgm, S.wgn, S.ks, S.pf>;
   const dim3 blk(S.wgm * S.wgn * S.ks * 64);
   if (e0 || e1) hipExtLaunchKernelGGL(fn, grid, blk, 0, s, e0, e1, 0, a);
   else hipLaunchKernelGGL(fn, grid, blk, 0, s, a);
@@ -536,11 +537,11 @@ int fact_fused_variants(int pass) { return pass == 0 ? kNY : kNZ; }
 
 bool fact_fused_shape_ok(int pass, int v, int ny, int nz) {
   if (v < 0 || v >= fact_fused_variants(pass)) return false;
-  if (ny % 128 || nz % 128 || ny > 2 * kMaxH || nz > 2 * kMaxH) return false;
+  if (ny % 128 || nz % 128 || ny > kMaxN || nz > kMaxN) return false;
   const FShape &s = shape_of(pass, v);
   const int GM = pass == 0 ? 2 * nz : nz / 2, GN = pass == 0 ? ny / 2 : 2 * ny;
   const int K = pass == 0 ? ny / 2 : nz / 2;
-  return GM % s.bm == 0 && GN % s.bn == 0 && K % (2 * s.bk) == 0;
+  return GM % s.bm == 0 && GN % s.bn == 0 && K % (s.pf * s.bk) == 0;
 }
 
 hipError_t launch_fact_fused(const FusedArgs &a, int pass, int v, hipStream_t s, hipEvent_t ev0,
@@ -555,9 +556,10 @@ hipError_t launch_fact_fused(const FusedArgs &a, int pass, int v, hipStream_t s,
 }
 
 // G'_y[k'][c'] (pass 0) and G'_z[r'][k'] (pass 1) for an axis of length n:
-// blk[s_in][s_out](q, c) of [[Fr, Fi], [-Fi, Fr]], F_qc = e^{-2 pi i qc/m},
-// with k' -> (q, s_in), c' -> (c, s_out) (y columns), r' -> (c, s_out) (z rows)
-// as described at the top of this file
+// blk[s_in][s_out](q, c) of [[Hr, Hi], [-Hi, Hr]], H_qc = pre_{2q} F^{qc} =
+// e^{-i pi (32 qc + 8q + 1) / 4n} (F_qc = e^{-2 pi i qc/m}, the phase reduced
+// exactly in integers), with k' -> (q, s_in), c' -> (c, s_out) (y columns),
+// r' -> (c, s_out) (z rows) as described at the top of this file
 void fact_fused_matrix(int n, int pass, std::vector<double> &g) {
   const int h = n / 2, m = n / 4;
   g.assign((size_t)h * h, 0.0);
@@ -567,8 +569,8 @@ void fact_fused_matrix(int n, int pass, std::vector<double> &g) {
     si = j & 1;
   };
   auto val = [&](int q, int c, int si, int so) {
-    const long r = ((long)q * c) % m;
-    const double ang = 2.0 * M_PI * (double)r / m;
+    const long r = (32L * q * c + 8L * q + 1) % (8L * n);
+    const double ang = M_PI * (double)r / (4.0 * n);
     const double fr = std::cos(ang), fi = -std::sin(ang);
     const double b[2][2] = {{fr, fi}, {-fi, fr}};
     return b[si][so];
@@ -591,4 +593,19 @@ void fact_fused_matrix(int n, int pass, std::vector<double> &g) {
       }
     }
   }
+}
+
+// the butterfly twiddles of the fused kernels for an axis of length n:
+// mid'_k = e^{-i pi (16k + 4) / 4n} (k < n/4; the odd parity's e^{-i pi/n}
+// included), then post_k = e^{-i pi 4k / 4n} (k < n/2), as (re, im) pairs
+void fact_fused_twiddles(int n, std::vector<double> &tw) {
+  tw.clear();
+  auto put = [&](long a) {
+    a %= 8L * n;
+    const double ang = M_PI * (double)a / (4.0 * n);
+    tw.push_back(std::cos(ang));
+    tw.push_back(-std::sin(ang));
+  };
+  for (int k = 0; k < n / 4; ++k) put(16L * k + 4);
+  for (int k = 0; k < n / 2; ++k) put(4L * k);
 }

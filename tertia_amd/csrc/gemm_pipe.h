@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace {
 
 // unpadded k-major FP64 LDS tiles with an XOR swizzle of the m (n) index by
@@ -62,6 +64,53 @@ __device__ inline void k_loop(int K, L0 load0, L1 load1, S0 store0, S1 store1, C
         }
         __syncthreads();
       }
+    }
+  }
+}
+
+template <int V>
+using SetI = std::integral_constant<int, V>;
+
+// The K loop with PF register sets (PF = 2 or 4): slab s is loaded into
+// register set s % PF PF slabs ahead, stored into LDS buffer s % 2 one slab
+// ahead and multiplied from there.  load(SetI<r>, k0), store(SetI<r>, buf)
+// and compute(buf).  Needs K % (PF * BK) == 0; look-aheads past the end
+// re-read the last slab, which is never multiplied.  One barrier per slab.
+template <int PF, int BK, class L, class S, class C>
+__device__ inline void k_loop_pf(int K, L load, S store, C compute) {
+  static_assert(PF == 2 || PF == 4, "prefetch depth");
+  const int KT = K / BK, last = (KT - 1) * BK;
+  load(SetI<0>{}, 0);
+  load(SetI<1>{}, min(BK, last));
+  if constexpr (PF == 4) {
+    load(SetI<2>{}, min(2 * BK, last));
+    load(SetI<3>{}, min(3 * BK, last));
+  }
+  store(SetI<0>{}, 0);
+  load(SetI<0>{}, min(PF * BK, last));
+  __syncthreads();
+  for (int kt = 0; kt < KT; kt += PF) {
+    compute(0);
+    store(SetI<1>{}, 1);
+    load(SetI<1>{}, min((kt + PF + 1) * BK, last));
+    __syncthreads();
+    compute(1);
+    if constexpr (PF == 2) {
+      store(SetI<0>{}, 0);
+      load(SetI<0>{}, min((kt + 4) * BK, last));
+      __syncthreads();
+    } else {
+      store(SetI<2>{}, 0);
+      load(SetI<2>{}, min((kt + 6) * BK, last));
+      __syncthreads();
+      compute(0);
+      store(SetI<3>{}, 1);
+      load(SetI<3>{}, min((kt + 7) * BK, last));
+      __syncthreads();
+      compute(1);
+      store(SetI<0>{}, 0);
+      load(SetI<0>{}, min((kt + 8) * BK, last));
+      __syncthreads();
     }
   }
 }

@@ -182,14 +182,14 @@ int transform_fact(qss_ctx *c, int n, const int *kz, const int *ky, const double
       f.resid_c[b] = resid ? resid[b] : -1;
       f.push[b] = push ? push[b] : -1;
     }
-    f.G = c->d_Gf[0]; f.tw = c->d_tw[0];
+    f.G = c->d_Gf[0]; f.tw = c->d_twf[0];
     {
       Timed t(c, KC_GEMM, true);
       HIPCHECK(launch_fact_fused(f, 0, c->ffv[0][n], c->stream, t.e0, t.e1));
     }
     if (c->timing) c->gemm_flops += 2.0 * (2.0 * c->nz) * (c->ny / 2) * (c->ny / 2) * n * f.nw;
     for (int b = 0; b < n; ++b) { f.in[b] = c->tmp[b]; f.out[b] = out[b]; }
-    f.G = c->d_Gf[1]; f.tw = c->d_tw[1];
+    f.G = c->d_Gf[1]; f.tw = c->d_twf[1];
     {
       Timed t(c, KC_GEMM, true);
       HIPCHECK(launch_fact_fused(f, 1, c->ffv[1][n], c->stream, t.e0, t.e1));
@@ -1053,10 +1053,15 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
       if (fact_fused_shape_ok(1, v, c->ny, c->nz)) v1 = v;
     if (v0 >= 0 && v1 >= 0) {
       for (int i = 0; i < 2; ++i) {
-        std::vector<double> g;
+        std::vector<double> g, tw;
         fact_fused_matrix(i == 0 ? c->ny : c->nz, i, g);
-        if ((r = dalloc(&c->d_Gf[i], g.size())) != QSS_OK) return fail(r);
-        if (hipMemcpy(c->d_Gf[i], g.data(), g.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+        fact_fused_twiddles(i == 0 ? c->ny : c->nz, tw);
+        double *twd = nullptr;
+        if ((r = dalloc(&c->d_Gf[i], g.size())) != QSS_OK || (r = dalloc(&twd, tw.size())) != QSS_OK)
+          return fail(r);
+        c->d_twf[i] = reinterpret_cast<double2 *>(twd);
+        if (hipMemcpy(c->d_Gf[i], g.data(), g.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(twd, tw.data(), tw.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
           qss_set_error("fused transform table upload failed");
           return fail(QSS_E_HIP);
         }
@@ -1172,7 +1177,7 @@ int qss_destroy(qss_ctx *c) {
   for (auto &p : c->tmp) dfree(p);
   for (auto &p : c->tmp2) dfree(p);
   dfree(c->d_G[0]); dfree(c->d_G[1]); dfree(c->d_tw[0]); dfree(c->d_tw[1]);
-  dfree(c->d_Gf[0]); dfree(c->d_Gf[1]);
+  dfree(c->d_Gf[0]); dfree(c->d_Gf[1]); dfree(c->d_twf[0]); dfree(c->d_twf[1]);
   dfree(c->F); dfree(c->Ff); dfree(c->d_bins); dfree(c->d_wbunch);
   for (auto &p : c->Ts) dfree(p);
   for (auto &p : c->xs) dfree(p);
@@ -1862,7 +1867,7 @@ int qss_autotune(qss_ctx *c, int reps, float *table_ms) {
       f.w_in = c->ncell; f.w_out = c->ncell; f.w_p = c->ncell;
       f.w_f = c->ncell * kPushComp; f.w_r = kResidPerWake;
       f.alpha = 1.0; f.F = c->F; f.Ff = c->Ff; f.rslots = c->d_resid;
-      f.G = c->d_Gf[pass]; f.tw = c->d_tw[pass];
+      f.G = c->d_Gf[pass]; f.tw = c->d_twf[pass];
       const bool epi = pass == 1 && (b == 6 || b == 8);
       for (int q = 0; q < b; ++q) {
         f.in[q] = pass == 0 ? c->S[q] : c->tmp2[q];

@@ -118,8 +118,8 @@ struct FusedArgs {
   long w_in, w_out, w_p;
   int kz[kGemmMaxBatch], ky[kGemmMaxBatch];
   int resid_c[kGemmMaxBatch], push[kGemmMaxBatch];
-  const double *G;                   // the pass's permuted table (h x h)
-  const double2 *tw;                 // the pass's axis twiddles (pre, mid, post)
+  const double *G;                   // the pass's permuted table (h x h), pre folded in
+  const double2 *tw;                 // the pass's butterfly twiddles (mid', post)
   double alpha, beta;
   double *F;
   float *Ff;
@@ -132,6 +132,7 @@ bool fact_fused_shape_ok(int pass, int variant, int ny, int nz);
 hipError_t launch_fact_fused(const FusedArgs &a, int pass, int variant, hipStream_t s,
                              hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void fact_fused_matrix(int n, int pass, std::vector<double> &g);
+void fact_fused_twiddles(int n, std::vector<double> &tw);
 
 // ---------------------------------------------------------------------------
 // adjoint sweep (tertia_amd/csrc/adjoint.hip, qss_sweep_vjp)
@@ -221,9 +222,11 @@ struct qss_ctx {
   // factorized transform: eligible, in use, G matrices (y, z), twiddles,
   // second intermediate per batch slot, GEMM variants per pass and batch size
   bool fact_ok = false, fact_use = false;
-  // fused form (fact_fused.hip): eligible, permuted tables (y, z), variants
+  // fused form (fact_fused.hip): eligible, permuted tables (y, z), their
+  // butterfly twiddles, variants
   bool ffused_ok = false;
   double *d_Gf[2] = {};
+  double2 *d_twf[2] = {};
   int ffv[2][kGemmMaxBatch + 1] = {};
   double *d_G[2] = {};
   double2 *d_tw[2] = {};
