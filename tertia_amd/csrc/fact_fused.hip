@@ -92,6 +92,119 @@ __device__ inline FItem fitem(int GM, int GN, int nbatch) {
   return it;
 }
 
+// butterflies along y and the stores of one wave's accumulator tiles: tile
+// (i, j) holds virtual rows mr + 16 i .. and columns nc + 16 j ..; frequency
+// k = 16 g + li and k2 = m-1-k of X rows (mr + 16 i)/2 + lk + 4 hh, one
+// 64-column group g per four accumulator tiles
+template <int TI, int TJ>
+__device__ inline void y_epilogue(const FusedArgs &a, const d4 (&acc)[TI][TJ], int mr, int nc,
+                                  int fb, int w, bool rev, int li, int lk) {
+  const int N = a.ny, h = N / 2, m = N / 4;
+  const double2 *mid = a.tw, *post = a.tw + m;   // mid' (m), post (h)
+  const double sg = rev ? 2.0 : -2.0;
+  double *Y = a.out[fb] + w * a.w_out;
+#pragma unroll
+  for (int gg = 0; gg < TJ / 4; ++gg) {
+    const int g = nc / 64 + gg;
+    const int k = 16 * g + li, k2 = m - 1 - k;
+    const double2 md = mid[k], md2 = mid[k2];
+    const double2 pk = post[k], pkm = post[k + m], pk2 = post[k2], pk2m = post[k2 + m];
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) {
+        const int xrow = (mr + 16 * i) / 2 + lk + 4 * hh;
+        const d4 *A4 = &acc[i][4 * gg];
+        const double2 E = make_double2(A4[0][2 * hh], A4[1][2 * hh]);
+        const double2 O = make_double2(A4[0][2 * hh + 1], A4[1][2 * hh + 1]);
+        const double2 E2 = make_double2(A4[2][2 * hh], A4[3][2 * hh]);
+        const double2 O2 = make_double2(A4[2][2 * hh + 1], A4[3][2 * hh + 1]);
+        const double2 wo = cmulf(md, O), wo2 = cmulf(md2, O2);
+        const double2 wk = cmulf(make_double2(E.x + wo.x, E.y + wo.y), pk);
+        const double2 wkm = cmulf(make_double2(E.x - wo.x, E.y - wo.y), pkm);
+        const double2 wk2 = cmulf(make_double2(E2.x + wo2.x, E2.y + wo2.y), pk2);
+        const double2 wk2m = cmulf(make_double2(E2.x - wo2.x, E2.y - wo2.y), pk2m);
+        double *yr = Y + (long)xrow * N;
+        *reinterpret_cast<double2 *>(yr + 2 * k) = make_double2(2.0 * wk.x, sg * wk2m.y);
+        *reinterpret_cast<double2 *>(yr + h - 2 - 2 * k) = make_double2(2.0 * wk2.x, sg * wkm.y);
+        *reinterpret_cast<double2 *>(yr + h + 2 * k) = make_double2(2.0 * wkm.x, sg * wk2.y);
+        *reinterpret_cast<double2 *>(yr + N - 2 - 2 * k) = make_double2(2.0 * wk2m.x, sg * wk.y);
+      }
+  }
+}
+
+// butterflies along z and the epilogue (alpha, beta, corrector residual,
+// push field) of one wave's accumulator tiles: tile (i, j) holds virtual rows
+// mr + 16 i .. and columns ncol + 16 j ..; frequency kf of rows
+// (mr + 16 i)/2 + lk + 4 hh, column y = ncol/2 + 16 g + li
+template <int TI, int TJ>
+__device__ inline void z_epilogue(const FusedArgs &a, const d4 (&acc)[TI][TJ], int mr, int ncol,
+                                  int fb, int w, bool rev, int li, int lk, int lane) {
+  const int Ny = a.ny, Nz = a.nz, h = Nz / 2, m = Nz / 4;
+  const double2 *mid = a.tw, *post = a.tw + m;   // mid' (m), post (h)
+  const double sg = rev ? 2.0 : -2.0;
+  const long nc = (long)Ny * Nz;
+  double *C = a.out[fb] + w * a.w_out;
+  const double *P = a.P[fb] ? a.P[fb] + w * a.w_p : nullptr;
+  const int resid = a.resid_c[fb], push = a.push[fb];
+  const bool need_old = resid >= 0 || a.beta != 0.0;
+  double dmax = 0.0, vmax = 0.0;
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh) {
+      const int kf = (mr + 16 * i) / 2 + lk + 4 * hh;
+      const double2 md = mid[kf], pk = post[kf], pkm = post[kf + m];
+      const int zr[4] = {2 * kf, Nz - 1 - 2 * kf, h + 2 * kf, h - 1 - 2 * kf};
+#pragma unroll
+      for (int g2 = 0; g2 < TJ / 2; ++g2) {
+        const int y = ncol / 2 + 16 * g2 + li;
+        const double2 E = make_double2(acc[i][2 * g2][2 * hh], acc[i][2 * g2][2 * hh + 1]);
+        const double2 O =
+            make_double2(acc[i][2 * g2 + 1][2 * hh], acc[i][2 * g2 + 1][2 * hh + 1]);
+        const double2 wo = cmulf(md, O);
+        const double2 wk = cmulf(make_double2(E.x + wo.x, E.y + wo.y), pk);
+        const double2 wkm = cmulf(make_double2(E.x - wo.x, E.y - wo.y), pkm);
+        const double val[4] = {2.0 * wk.x, sg * wk.y, 2.0 * wkm.x, sg * wkm.y};
+        double oldv[4], pv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const long n = (long)zr[q] * Ny + y;
+          oldv[q] = need_old ? C[n] : 0.0;
+          pv[q] = push >= 0 ? P[n] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const long n = (long)zr[q] * Ny + y;
+          double v = a.alpha * val[q];
+          if (a.beta != 0.0) v += a.beta * oldv[q];   // accumulate (adjoint sweep)
+          if (resid >= 0) {
+            const double d = fabs(oldv[q] - v), mm = fabs(v);
+            dmax = fmax(dmax, d == d ? d : INFINITY);
+            vmax = fmax(vmax, mm == mm ? mm : INFINITY);
+          }
+          C[n] = v;
+          if (push >= 0) {
+            const double pq = 0.5 * (pv[q] + v);
+            const long pn = (long)push * nc + n;
+            if (a.Ff) a.Ff[w * a.w_f + pn] = (float)pq;
+            else a.F[w * a.w_f + pn] = pq;
+          }
+        }
+      }
+    }
+  if (resid >= 0) {   // uniform over the work-group (one field per block)
+    dmax = wave_maxf(dmax);
+    vmax = wave_maxf(vmax);
+    if (lane == 0) {
+      unsigned long long *R = a.rslots + w * a.w_r + (blockIdx.x % kResidSlots) * 8;
+      atomic_max_nn(&R[resid], dmax);
+      atomic_max_nn(&R[3 + resid], vmax);
+    }
+  }
+}
+
+
 // ---------------------------------------------------------------------------
 // y pass: virtual A (2 Nz x h) = pre-twiddled rows of X, B = G'_y (h x h),
 // output Y (Nz x Ny, natural layout) after the butterflies along y
@@ -235,40 +348,7 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_y(FusedArgs a) {
         for (int j = 0; j < TJ; ++j) acc[i][j] += red[q][wl][i * TJ + j][lane];
   }
 
-  // butterflies along y: frequency k = 16 g + li and k2 = m-1-k of rows
-  // xrow = (m0 + wm WTM + 16 i)/2 + lk + 4 hh, one 64-column group g per
-  // four accumulator tiles
-  const double2 *mid = a.tw, *post = a.tw + m;   // mid' (m), post (h)
-  const double sg = rev ? 2.0 : -2.0;
-  double *Y = a.out[fb] + it.w * a.w_out;
-#pragma unroll
-  for (int gg = 0; gg < TJ / 4; ++gg) {
-    const int g = (n0 + wn * WTN) / 64 + gg;
-    const int k = 16 * g + li, k2 = m - 1 - k;
-    const double2 md = mid[k], md2 = mid[k2];
-    const double2 pk = post[k], pkm = post[k + m], pk2 = post[k2], pk2m = post[k2 + m];
-#pragma unroll
-    for (int i = 0; i < TI; ++i)
-#pragma unroll
-      for (int hh = 0; hh < 2; ++hh) {
-        const int xrow = (m0 + wm * WTM + 16 * i) / 2 + lk + 4 * hh;
-        const d4 *A4 = &acc[i][4 * gg];
-        const double2 E = make_double2(A4[0][2 * hh], A4[1][2 * hh]);
-        const double2 O = make_double2(A4[0][2 * hh + 1], A4[1][2 * hh + 1]);
-        const double2 E2 = make_double2(A4[2][2 * hh], A4[3][2 * hh]);
-        const double2 O2 = make_double2(A4[2][2 * hh + 1], A4[3][2 * hh + 1]);
-        const double2 wo = cmulf(md, O), wo2 = cmulf(md2, O2);
-        const double2 wk = cmulf(make_double2(E.x + wo.x, E.y + wo.y), pk);
-        const double2 wkm = cmulf(make_double2(E.x - wo.x, E.y - wo.y), pkm);
-        const double2 wk2 = cmulf(make_double2(E2.x + wo2.x, E2.y + wo2.y), pk2);
-        const double2 wk2m = cmulf(make_double2(E2.x - wo2.x, E2.y - wo2.y), pk2m);
-        double *yr = Y + (long)xrow * N;
-        *reinterpret_cast<double2 *>(yr + 2 * k) = make_double2(2.0 * wk.x, sg * wk2m.y);
-        *reinterpret_cast<double2 *>(yr + h - 2 - 2 * k) = make_double2(2.0 * wk2.x, sg * wkm.y);
-        *reinterpret_cast<double2 *>(yr + h + 2 * k) = make_double2(2.0 * wkm.x, sg * wk2.y);
-        *reinterpret_cast<double2 *>(yr + N - 2 - 2 * k) = make_double2(2.0 * wk2m.x, sg * wk.y);
-      }
-  }
+  y_epilogue<TI, TJ>(a, acc, m0 + wm * WTM, n0 + wn * WTN, fb, it.w, rev, li, lk);
 }
 
 // ---------------------------------------------------------------------------
@@ -408,74 +488,183 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_z(FusedArgs a) {
         for (int j = 0; j < TJ; ++j) acc[i][j] += red[q][wl][i * TJ + j][lane];
   }
 
-  // butterflies along z and the epilogue: frequency kf of rows
-  // (m0 + wm WTM + 16 i)/2 + lk + 4 hh, column y = (n0 + wn WTN)/2 + 16 g + li
-  const double2 *mid = a.tw, *post = a.tw + m;   // mid' (m), post (h)
-  const double sg = rev ? 2.0 : -2.0;
-  const long nc = (long)Ny * Nz;
-  double *C = a.out[fb] + it.w * a.w_out;
-  const double *P = a.P[fb] ? a.P[fb] + it.w * a.w_p : nullptr;
-  const int resid = a.resid_c[fb], push = a.push[fb];
-  const bool need_old = resid >= 0 || a.beta != 0.0;
-  double dmax = 0.0, vmax = 0.0;
-#pragma unroll
-  for (int i = 0; i < TI; ++i)
-#pragma unroll
-    for (int hh = 0; hh < 2; ++hh) {
-      const int kf = (m0 + wm * WTM + 16 * i) / 2 + lk + 4 * hh;
-      const double2 md = mid[kf], pk = post[kf], pkm = post[kf + m];
-      const int zr[4] = {2 * kf, Nz - 1 - 2 * kf, h + 2 * kf, h - 1 - 2 * kf};
-#pragma unroll
-      for (int g2 = 0; g2 < TJ / 2; ++g2) {
-        const int y = (n0 + wn * WTN) / 2 + 16 * g2 + li;
-        const double2 E = make_double2(acc[i][2 * g2][2 * hh], acc[i][2 * g2][2 * hh + 1]);
-        const double2 O =
-            make_double2(acc[i][2 * g2 + 1][2 * hh], acc[i][2 * g2 + 1][2 * hh + 1]);
-        const double2 wo = cmulf(md, O);
-        const double2 wk = cmulf(make_double2(E.x + wo.x, E.y + wo.y), pk);
-        const double2 wkm = cmulf(make_double2(E.x - wo.x, E.y - wo.y), pkm);
-        const double val[4] = {2.0 * wk.x, sg * wk.y, 2.0 * wkm.x, sg * wkm.y};
-        double oldv[4], pv[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const long n = (long)zr[q] * Ny + y;
-          oldv[q] = need_old ? C[n] : 0.0;
-          pv[q] = push >= 0 ? P[n] : 0.0;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const long n = (long)zr[q] * Ny + y;
-          double v = a.alpha * val[q];
-          if (a.beta != 0.0) v += a.beta * oldv[q];   // accumulate (adjoint sweep)
-          if (resid >= 0) {
-            const double d = fabs(oldv[q] - v), mm = fabs(v);
-            dmax = fmax(dmax, d == d ? d : INFINITY);
-            vmax = fmax(vmax, mm == mm ? mm : INFINITY);
-          }
-          C[n] = v;
-          if (push >= 0) {
-            const double pq = 0.5 * (pv[q] + v);
-            const long pn = (long)push * nc + n;
-            if (a.Ff) a.Ff[it.w * a.w_f + pn] = (float)pq;
-            else a.F[it.w * a.w_f + pn] = pq;
-          }
-        }
-      }
-    }
-  if (resid >= 0) {   // uniform over the work-group (one field per block)
-    dmax = wave_maxf(dmax);
-    vmax = wave_maxf(vmax);
-    if (lane == 0) {
-      unsigned long long *R = a.rslots + it.w * a.w_r + (blockIdx.x % kResidSlots) * 8;
-      atomic_max_nn(&R[resid], dmax);
-      atomic_max_nn(&R[3 + resid], vmax);
-    }
+  z_epilogue<TI, TJ>(a, acc, m0 + wm * WTM, n0 + wn * WTN, fb, it.w, rev, li, lk, lane);
+}
+
+// ---------------------------------------------------------------------------
+// direct variants: every wave loads its own MFMA operand fragments from global
+// memory (L2) into a ring of D register sets, with no LDS staging and no
+// barrier in the K loop; the KS waves of a work-group take consecutive
+// quarters (KS parts) of K and add their partial sums through LDS in wave
+// order.  Work-group tile 16 TI x 16 TJ.
+// ---------------------------------------------------------------------------
+template <int I, int D, class F>
+__device__ inline void for_sets(F f) {
+  if constexpr (I < D) {
+    f(SetI<I>{});
+    for_sets<I + 1, D>(f);
   }
 }
 
-// tile shapes: {BM, BN, BK, WGM, WGN, KS, PF (register slabs in flight)}
+// K steps s0 .. s0+S-1 (S % D == 0): set r holds step s0 + r + D q; loads
+// past the end re-read the last step, which is never multiplied
+template <int D, class L, class M>
+__device__ inline void k_ring(int s0, int S, L load, M mma) {
+  const int slast = s0 + S - 1;
+  for_sets<0, D>([&](auto R) { load(R, min(s0 + (int)decltype(R)::value, slast)); });
+  for (int s = s0; s < s0 + S; s += D)
+    for_sets<0, D>([&](auto R) {
+      mma(R);
+      load(R, min(s + D + (int)decltype(R)::value, slast));
+    });
+}
+
+// partial sums of waves 1 .. KS-1 added to wave 0's through LDS; returns
+// false for the waves that are done
+template <int TI, int TJ, int KS>
+__device__ inline bool reduce_k(d4 (&acc)[TI][TJ], int ks, int lane) {
+  if constexpr (KS > 1) {
+    __shared__ d4 red[KS - 1][TI * TJ][64];
+    if (ks > 0) {
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) red[ks - 1][i * TJ + j][lane] = acc[i][j];
+    }
+    __syncthreads();
+    if (ks > 0) return false;
+#pragma unroll
+    for (int q = 0; q < KS - 1; ++q)
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) acc[i][j] += red[q][i * TJ + j][lane];
+  }
+  return true;
+}
+
+// The direct kernels use their own order of the K index, k' = 4s + j with
+//   j = 0: (q = s, Re), 1: (q = m/2-1-s, Im), 2: (q = m/2+s, Re), 3: (q = m-1-s, Im)
+// (tables fact_fused_matrix passes 3 and 4), so that the pair element a lane
+// multiplies at K step s is x~[c_j(p) + 4s] for every j:
+//   c_0 = 2p, c_1 = N/2 + 3 - 2p, c_2 = N/2 + 2p, c_3 = 3 - 2p
+// and every operand address is a uniform base (stepping by a uniform amount
+// per K step) plus a loop-invariant per-lane offset: no vector arithmetic in
+// the K loop.
+__device__ inline int pair_first(int N, int j, int p) {
+  return j == 0 ? 2 * p : j == 1 ? N / 2 + 3 - 2 * p : j == 2 ? N / 2 + 2 * p : 3 - 2 * p;
+}
+
+// y pass: A fragment of tile i from X row (m0 + 16 i)/2 + (li & 3) +
+// 4 (li >> 3) (virtual row m0 + 16 i + li, parity (li >> 2) & 1), B fragment
+// of tile j from the table row 4s + lk, column n0 + 16 j + li
+template <int TI, int TJ, int KS, int D>
+__global__ __launch_bounds__(KS * 64) void k_fact_yd(FusedArgs a) {
+  constexpr int BM = 16 * TI, BN = 16 * TJ;
+  static_assert(TJ % 4 == 0, "64-column groups");
+  const int N = a.ny, GM = 2 * a.nz, GN = N / 2, K = N / 2;
+  const FItem it = fitem<BM, BN>(GM, GN, a.nbatch);
+  const int m0 = it.m0, n0 = it.n0, fb = it.f;
+  const bool rev = a.ky[fb] == QSS_KIND_S;
+  const int t = threadIdx.x, lane = t & 63;
+  const int ks = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int li = lane & 15, lk = lane >> 4;
+  const int c = pair_first(N, lk, (li >> 2) & 1);
+  const char *Xb = reinterpret_cast<const char *>(a.in[fb] + it.w * a.w_in);
+  const long xstep = rev ? -32 : 32;   // bytes per K step
+  uint32_t xo[TI];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+    xo[i] = (uint32_t)(((m0 + 16 * i) / 2 + (li & 3) + 4 * (li >> 3)) * N + (rev ? N - 1 - c : c)) * 8u;
+  const char *Gb = reinterpret_cast<const char *>(a.Gd + n0);
+  const long gstep = 32L * GN;
+  const uint32_t go = (uint32_t)(lk * GN + li) * 8u;
+  const int S = K / 4 / KS;
+  double ra[D][TI], rb[D][TJ];
+  d4 acc[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) acc[i][j] = d4{0, 0, 0, 0};
+  auto load = [&](auto R, int s) {
+    constexpr int r = decltype(R)::value;
+    const char *xs = Xb + xstep * s, *gs = Gb + gstep * s;
+#pragma unroll
+    for (int i = 0; i < TI; ++i) ra[r][i] = *reinterpret_cast<const double *>(xs + xo[i]);
+#pragma unroll
+    for (int j = 0; j < TJ; ++j)
+      rb[r][j] = *reinterpret_cast<const double *>(gs + go + 128 * j);
+  };
+  auto mma = [&](auto R) {
+    constexpr int r = decltype(R)::value;
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) acc[i][j] = mma64(ra[r][i], rb[r][j], acc[i][j]);
+  };
+  k_ring<D>(ks * S, S, load, mma);
+  if (!reduce_k<TI, TJ, KS>(acc, ks, lane)) return;
+  y_epilogue<TI, TJ>(a, acc, m0, n0, fb, it.w, rev, li, lk);
+}
+
+// z pass: A fragment of tile i from the k-major table row 4s + lk, column
+// m0 + 16 i + li; B fragment of tile j (parity p = j & 1, columns y = n0/2 +
+// 16 (j >> 1) + li) from Y row c_lk(p) + 4s (reversed for the DST)
+template <int TI, int TJ, int KS, int D>
+__global__ __launch_bounds__(KS * 64) void k_fact_zd(FusedArgs a) {
+  constexpr int BM = 16 * TI, BN = 16 * TJ;
+  static_assert(TJ % 2 == 0, "32-column groups");
+  const int Ny = a.ny, Nz = a.nz, GM = Nz / 2, GN = 2 * Ny, K = Nz / 2;
+  const FItem it = fitem<BM, BN>(GM, GN, a.nbatch);
+  const int m0 = it.m0, n0 = it.n0, fb = it.f;
+  const bool rev = a.kz[fb] == QSS_KIND_S;
+  const int t = threadIdx.x, lane = t & 63;
+  const int ks = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int li = lane & 15, lk = lane >> 4;
+  const char *Yb = reinterpret_cast<const char *>(a.in[fb] + it.w * a.w_in + n0 / 2);
+  const long ystep = (rev ? -32L : 32L) * Ny;
+  uint32_t yo[2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const int c = pair_first(Nz, lk, p);
+    yo[p] = (uint32_t)((rev ? Nz - 1 - c : c) * Ny + li) * 8u;
+  }
+  const char *Gb = reinterpret_cast<const char *>(a.Gd + m0);
+  const long gstep = 32L * GM;
+  const uint32_t go = (uint32_t)(lk * GM + li) * 8u;
+  const int S = K / 4 / KS;
+  double ra[D][TI], rb[D][TJ];
+  d4 acc[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) acc[i][j] = d4{0, 0, 0, 0};
+  auto load = [&](auto R, int s) {
+    constexpr int r = decltype(R)::value;
+    const char *gs = Gb + gstep * s, *ys = Yb + ystep * s;
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+      ra[r][i] = *reinterpret_cast<const double *>(gs + go + 128 * i);
+#pragma unroll
+    for (int j = 0; j < TJ; ++j)
+      rb[r][j] = *reinterpret_cast<const double *>(ys + yo[j & 1] + 128 * (j >> 1));
+  };
+  auto mma = [&](auto R) {
+    constexpr int r = decltype(R)::value;
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) acc[i][j] = mma64(ra[r][i], rb[r][j], acc[i][j]);
+  };
+  k_ring<D>(ks * S, S, load, mma);
+  if (!reduce_k<TI, TJ, KS>(acc, ks, lane)) return;
+  z_epilogue<TI, TJ>(a, acc, m0, n0, fb, it.w, rev, li, lk, lane);
+}
+
+// tile shapes: {BM, BN, BK, WGM, WGN, KS, PF (register slabs in flight),
+// direct}; direct shapes: {16 TI, 16 TJ, 0, 1, 1, KS, D}
 struct FShape {
-  int bm, bn, bk, wgm, wgn, ks, pf;
+  int bm, bn, bk, wgm, wgn, ks, pf, direct;
 };
 constexpr FShape kShapesY[] = {
     {32, 64, 16, 2, 1, 1, 2},  {64, 64, 16, 4, 1, 1, 2},  {32, 64, 16, 2, 1, 2, 2},
@@ -483,34 +672,48 @@ constexpr FShape kShapesY[] = {
     {32, 64, 32, 2, 1, 2, 2},  {32, 128, 16, 2, 1, 1, 2}, {32, 128, 16, 2, 1, 2, 2},
     {64, 128, 16, 4, 1, 1, 2}, {32, 256, 16, 2, 1, 1, 2}, {32, 64, 16, 2, 1, 2, 4},
     {32, 64, 16, 2, 1, 4, 4},  {64, 64, 16, 4, 1, 2, 4},  {32, 64, 16, 2, 1, 1, 4},
-    {64, 64, 16, 4, 1, 1, 4}};
+    {64, 64, 16, 4, 1, 1, 4},  {32, 64, 0, 1, 1, 4, 4, 1}, {16, 64, 0, 1, 1, 4, 4, 1},
+    {32, 64, 0, 1, 1, 2, 4, 1}, {16, 64, 0, 1, 1, 8, 4, 1}, {16, 64, 0, 1, 1, 2, 8, 1},
+    {32, 64, 0, 1, 1, 4, 8, 1}};
 constexpr FShape kShapesZ[] = {
     {32, 64, 16, 2, 1, 1, 2},  {64, 64, 16, 4, 1, 1, 2},  {32, 32, 16, 2, 1, 2, 2},
     {32, 32, 16, 2, 1, 4, 2},  {32, 64, 16, 2, 1, 2, 2},  {32, 64, 16, 2, 1, 4, 2},
     {32, 32, 32, 2, 1, 4, 2},  {64, 64, 16, 4, 1, 2, 2},  {128, 32, 16, 4, 1, 2, 2},
     {128, 32, 16, 8, 1, 1, 2}, {128, 64, 16, 4, 1, 1, 2}, {128, 32, 16, 4, 1, 1, 2},
     {32, 32, 16, 2, 1, 2, 4},  {32, 32, 16, 2, 1, 4, 4},  {32, 64, 16, 2, 1, 2, 4},
-    {64, 64, 16, 4, 1, 2, 4},  {32, 64, 16, 2, 1, 1, 4}};
+    {64, 64, 16, 4, 1, 2, 4},  {32, 64, 16, 2, 1, 1, 4},  {32, 32, 0, 1, 1, 4, 4, 1},
+    {32, 64, 0, 1, 1, 4, 4, 1}, {16, 64, 0, 1, 1, 4, 4, 1}, {16, 32, 0, 1, 1, 4, 4, 1},
+    {16, 32, 0, 1, 1, 8, 4, 1}, {32, 32, 0, 1, 1, 2, 8, 1}};
 constexpr int kNY = sizeof(kShapesY) / sizeof(kShapesY[0]);
 constexpr int kNZ = sizeof(kShapesZ) / sizeof(kShapesZ[0]);
 
-template <int V>
-hipError_t launch_y(const FusedArgs &a, dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-  constexpr FShape S = kShapesY[V];
-  auto fn = k_fact_y<S.bm, S.bn, S.bk, S.wgm, S.wgn, S.ks, S.pf>;
-  const dim3 blk(S.wgm * S.wgn * S.ks * 64);
+template <class K>
+hipError_t launch_fn(K fn, const FusedArgs &a, dim3 grid, dim3 blk, hipStream_t s, hipEvent_t e0,
+                     hipEvent_t e1) {
   if (e0 || e1) hipExtLaunchKernelGGL(fn, grid, blk, 0, s, e0, e1, 0, a);
   else hipLaunchKernelGGL(fn, grid, blk, 0, s, a);
   return hipGetLastError();
 }
+
+template <int V>
+hipError_t launch_y(const FusedArgs &a, dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  constexpr FShape S = kShapesY[V];
+  const dim3 blk(S.wgm * S.wgn * S.ks * 64);
+  if constexpr (S.direct)
+    return launch_fn(k_fact_yd<S.bm / 16, S.bn / 16, S.ks, S.pf>, a, grid, blk, s, e0, e1);
+  else
+    return launch_fn(k_fact_y<S.bm, S.bn, S.bk, S.wgm, S.wgn, S.ks, S.pf>, a, grid, blk, s, e0,
+                     e1);
+}
 template <int V>
 hipError_t launch_z(const FusedArgs &a, dim3 grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   constexpr FShape S = kShapesZ[V];
-  auto fn = k_fact_z<S.bm, S.bn, S.bk, S.wgm, S.wgn, S.ks, S.pf>;
   const dim3 blk(S.wgm * S.wgn * S.ks * 64);
-  if (e0 || e1) hipExtLaunchKernelGGL(fn, grid, blk, 0, s, e0, e1, 0, a);
-  else hipLaunchKernelGGL(fn, grid, blk, 0, s, a);
-  return hipGetLastError();
+  if constexpr (S.direct)
+    return launch_fn(k_fact_zd<S.bm / 16, S.bn / 16, S.ks, S.pf>, a, grid, blk, s, e0, e1);
+  else
+    return launch_fn(k_fact_z<S.bm, S.bn, S.bk, S.wgm, S.wgn, S.ks, S.pf>, a, grid, blk, s, e0,
+                     e1);
 }
 
 // the launchers of variants 0 .. n-1 of a pass
@@ -541,6 +744,7 @@ bool fact_fused_shape_ok(int pass, int v, int ny, int nz) {
   const FShape &s = shape_of(pass, v);
   const int GM = pass == 0 ? 2 * nz : nz / 2, GN = pass == 0 ? ny / 2 : 2 * ny;
   const int K = pass == 0 ? ny / 2 : nz / 2;
+  if (s.direct) return GM % s.bm == 0 && GN % s.bn == 0 && (K / 4) % (s.ks * s.pf) == 0;
   return GM % s.bm == 0 && GN % s.bn == 0 && K % (s.pf * s.bk) == 0;
 }
 
@@ -555,7 +759,9 @@ hipError_t launch_fact_fused(const FusedArgs &a, int pass, int v, hipStream_t s,
   return launch_z_v(v, a, grid, s, ev0, ev1, std::make_integer_sequence<int, kNZ>{});
 }
 
-// G'_y[k'][c'] (pass 0) and G'_z[r'][k'] (pass 1) for an axis of length n:
+// G'_y[k'][c'] (pass 0), G'_z[r'][k'] (pass 1), and with the direct kernels'
+// K order G'_y (pass 3) and G'_z k-major, [k'][r'] (pass 4), for an axis of
+// length n:
 // blk[s_in][s_out](q, c) of [[Hr, Hi], [-Hi, Hr]], H_qc = pre_{2q} F^{qc} =
 // e^{-i pi (32 qc + 8q + 1) / 4n} (F_qc = e^{-2 pi i qc/m}, the phase reduced
 // exactly in integers), with k' -> (q, s_in), c' -> (c, s_out) (y columns),
@@ -563,9 +769,17 @@ hipError_t launch_fact_fused(const FusedArgs &a, int pass, int v, hipStream_t s,
 void fact_fused_matrix(int n, int pass, std::vector<double> &g) {
   const int h = n / 2, m = n / 4;
   g.assign((size_t)h * h, 0.0);
+  // K index -> (q, s_in): the staged kernels' order (passes 0, 1) or the
+  // direct kernels' (passes 3, 4)
+  const bool direct = pass >= 3;
   auto kmap = [&](int kp, int &q, int &si) {
     const int u = kp / 4, j = kp % 4;
-    q = j < 2 ? u : m - 1 - u;
+    if (!direct) {
+      q = j < 2 ? u : m - 1 - u;
+    } else {
+      const int qs[4] = {u, m / 2 - 1 - u, m / 2 + u, m - 1 - u};
+      q = qs[j];
+    }
     si = j & 1;
   };
   auto val = [&](int q, int c, int si, int so) {
@@ -580,7 +794,7 @@ void fact_fused_matrix(int n, int pass, std::vector<double> &g) {
     kmap(kp, q, si);
     for (int x = 0; x < h; ++x) {
       int c, so;
-      if (pass == 0) {
+      if (pass == 0 || pass == 3) {
         const int gg = x / 64, slot = (x % 64) / 16, l = x % 16, k = 16 * gg + l;
         c = slot < 2 ? k : m - 1 - k;
         so = slot & 1;
@@ -589,7 +803,8 @@ void fact_fused_matrix(int n, int pass, std::vector<double> &g) {
         const int T = x / 16, tt = x % 16;
         c = 8 * T + (tt & 3) + 4 * (tt >> 3);
         so = (tt >> 2) & 1;
-        g[(size_t)x * h + kp] = val(q, c, si, so);
+        // pass 1: row-major [r'][k'], pass 4: k-major [k'][r']
+        g[pass == 1 ? (size_t)x * h + kp : (size_t)kp * h + x] = val(q, c, si, so);
       }
     }
   }

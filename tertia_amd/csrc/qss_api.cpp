@@ -182,14 +182,14 @@ int transform_fact(qss_ctx *c, int n, const int *kz, const int *ky, const double
       f.resid_c[b] = resid ? resid[b] : -1;
       f.push[b] = push ? push[b] : -1;
     }
-    f.G = c->d_Gf[0]; f.tw = c->d_twf[0];
+    f.G = c->d_Gf[0]; f.Gd = c->d_Gfd[0]; f.tw = c->d_twf[0];
     {
       Timed t(c, KC_GEMM, true);
       HIPCHECK(launch_fact_fused(f, 0, c->ffv[0][n], c->stream, t.e0, t.e1));
     }
     if (c->timing) c->gemm_flops += 2.0 * (2.0 * c->nz) * (c->ny / 2) * (c->ny / 2) * n * f.nw;
     for (int b = 0; b < n; ++b) { f.in[b] = c->tmp[b]; f.out[b] = out[b]; }
-    f.G = c->d_Gf[1]; f.tw = c->d_twf[1];
+    f.G = c->d_Gf[1]; f.Gd = c->d_Gfd[1]; f.tw = c->d_twf[1];
     {
       Timed t(c, KC_GEMM, true);
       HIPCHECK(launch_fact_fused(f, 1, c->ffv[1][n], c->stream, t.e0, t.e1));
@@ -1066,6 +1066,15 @@ int qss_create(const qss_params *p, int device, uint32_t history_mask, qss_ctx *
           return fail(QSS_E_HIP);
         }
       }
+      for (int i = 0; i < 2; ++i) {
+        std::vector<double> g;
+        fact_fused_matrix(i == 0 ? c->ny : c->nz, 3 + i, g);
+        if ((r = dalloc(&c->d_Gfd[i], g.size())) != QSS_OK) return fail(r);
+        if (hipMemcpy(c->d_Gfd[i], g.data(), g.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+          qss_set_error("fused transform table upload failed");
+          return fail(QSS_E_HIP);
+        }
+      }
       for (int b = 0; b <= kGemmMaxBatch; ++b) { c->ffv[0][b] = v0; c->ffv[1][b] = v1; }
       c->ffused_ok = true;
       // faster than the dense tables at 256^2 (0.293 against 0.300 ms per
@@ -1178,6 +1187,7 @@ int qss_destroy(qss_ctx *c) {
   for (auto &p : c->tmp2) dfree(p);
   dfree(c->d_G[0]); dfree(c->d_G[1]); dfree(c->d_tw[0]); dfree(c->d_tw[1]);
   dfree(c->d_Gf[0]); dfree(c->d_Gf[1]); dfree(c->d_twf[0]); dfree(c->d_twf[1]);
+  dfree(c->d_Gfd[0]); dfree(c->d_Gfd[1]);
   dfree(c->F); dfree(c->Ff); dfree(c->d_bins); dfree(c->d_wbunch);
   for (auto &p : c->Ts) dfree(p);
   for (auto &p : c->xs) dfree(p);
@@ -1867,7 +1877,7 @@ int qss_autotune(qss_ctx *c, int reps, float *table_ms) {
       f.w_in = c->ncell; f.w_out = c->ncell; f.w_p = c->ncell;
       f.w_f = c->ncell * kPushComp; f.w_r = kResidPerWake;
       f.alpha = 1.0; f.F = c->F; f.Ff = c->Ff; f.rslots = c->d_resid;
-      f.G = c->d_Gf[pass]; f.tw = c->d_twf[pass];
+      f.G = c->d_Gf[pass]; f.Gd = c->d_Gfd[pass]; f.tw = c->d_twf[pass];
       const bool epi = pass == 1 && (b == 6 || b == 8);
       for (int q = 0; q < b; ++q) {
         f.in[q] = pass == 0 ? c->S[q] : c->tmp2[q];

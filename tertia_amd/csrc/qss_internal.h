@@ -119,6 +119,8 @@ struct FusedArgs {
   int kz[kGemmMaxBatch], ky[kGemmMaxBatch];
   int resid_c[kGemmMaxBatch], push[kGemmMaxBatch];
   const double *G;                   // the pass's permuted table (h x h), pre folded in
+  const double *Gd;                  // the pass's table for the direct kernels (their K
+                                     // order; k-major for the z pass)
   const double2 *tw;                 // the pass's butterfly twiddles (mid', post)
   double alpha, beta;
   double *F;
@@ -226,6 +228,7 @@ struct qss_ctx {
   // butterfly twiddles, variants
   bool ffused_ok = false;
   double *d_Gf[2] = {};
+  double *d_Gfd[2] = {};             // the tables of the direct kernels
   double2 *d_twf[2] = {};
   int ffv[2][kGemmMaxBatch + 1] = {};
   double *d_G[2] = {};
