@@ -331,8 +331,8 @@ def adjoint_run(fd_bins: int):
     xi in [0,4]; loss = -mean Ex + 1e-3 var Ex over xi in [6,6.5], r < 0.2,
     which only depends on the first 650 layers.  Times the plain forward
     sweep of those layers and the gradient (checkpointed forward, every 32
-    layers, + reverse sweep), and checks the gradient against central
-    differences (h = 1e-6) on `fd_bins` random bins."""
+    layers, + reverse sweep), all with graph replay, and checks the gradient
+    against central differences (h = 1e-6) on `fd_bins` random bins."""
     from tertia_amd.loss import witness_loss_np
     from tertia_amd.mesh import Mesh
     from tertia_amd.params import profile_gradient_256
@@ -342,6 +342,9 @@ def adjoint_run(fd_bins: int):
     m = Mesh(p, history=("Ex",))
     try:
         m.autotune(2)
+        # layer-step pairs (forward sweeps) and checkpoint segments (reverse)
+        # replayed from hipGraphs
+        m.enable_graph(True)
         used0, _ = Mesh.device_memory()
 
         def ex_hist():
@@ -361,6 +364,15 @@ def adjoint_run(fd_bins: int):
         t0 = time.perf_counter()
         g = m.sweep_vjp(gex)
         t_vjp = time.perf_counter() - t0
+        # every layer's tape recorded by the forward (HBM: ~17 GB): the
+        # reverse recomputes nothing
+        t0 = time.perf_counter()
+        m.sweep_checkpointed(nrun, 0)
+        t_tf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        g_taped = m.sweep_vjp(gex)
+        t_tr = time.perf_counter() - t0
+        used_taped, _ = Mesh.device_memory()
         # the same reverse sweep with a HIP event pair per launch: breakdown
         m.sweep_checkpointed(nrun, every)
         m.enable_timing(True)
@@ -395,11 +407,22 @@ def adjoint_run(fd_bins: int):
     return {"workload": "256x256 over 16x16 k_p^-1, 4 ppc (262,144 electrons/slice), dxi=0.01,"
                         " FP64, 256 profile bins on xi in [0,4], loss on xi in [6,6.5] r<0.2 "
                         "(first 650 layers)",
-            "forward_s": t_fwd, "checkpointed_forward_s": t_ck, "reverse_s": t_vjp,
-            "adjoint_over_forward": t_vjp / t_fwd,
-            "gradient_over_forward": (t_ck + t_vjp) / t_fwd,
-            "note": "adjoint = the reverse sweep (segment recomputation from the checkpoints "
-                    "included); gradient = checkpointed forward + reverse sweep",
+            "mode": "taped: the forward records every layer's tape in HBM (every = 0), "
+                    "the reverse recomputes nothing",
+            "forward_s": t_fwd, "taped_forward_s": t_tf, "reverse_s": t_tr,
+            "adjoint_over_forward": t_tr / t_fwd,
+            "gradient_over_forward": (t_tf + t_tr) / t_fwd,
+            "device_bytes_in_use_taped": used_taped,
+            "checkpointed": {"every": every, "forward_s": t_ck, "reverse_s": t_vjp,
+                             "adjoint_over_forward": t_vjp / t_fwd,
+                             "gradient_over_forward": (t_ck + t_vjp) / t_fwd,
+                             "gradient_equals_taped": bool(np.array_equal(g_taped, g))},
+            "note": "adjoint = the reverse sweep; gradient = the recording forward + the "
+                    "reverse sweep, both against the plain forward; checkpointed: state "
+                    "every 32 layers, each segment recomputed in the reverse; forward sweeps "
+                    "by graph-replayed layer pairs, checkpointed reverse segments by one "
+                    "graph each; reverse_kernel_ms from a further, event-timed (eager) "
+                    "checkpointed reverse sweep",
             "reverse_kernel_ms": tm["ms"], "reverse_launches": tm["launches"],
             "checkpoint_every": every,
             "loss": loss, "loss_checkpointed": loss2,
@@ -691,8 +714,12 @@ def main():
               file=sys.stderr, flush=True)
     if not a.no_adjoint:
         line["adjoint"] = adjoint_run(a.fd_bins)
-        print("adjoint/forward %.2f" % line["adjoint"]["adjoint_over_forward"], file=sys.stderr,
-              flush=True)
+        ad = line["adjoint"]
+        print("adjoint/forward %.2f, gradient/forward %.2f (taped); checkpointed %.2f, %.2f"
+              % (ad["adjoint_over_forward"], ad["gradient_over_forward"],
+                 ad["checkpointed"]["adjoint_over_forward"],
+                 ad["checkpointed"]["gradient_over_forward"]),
+              file=sys.stderr, flush=True)
     if not a.no_small_configs:
         line["small_configs"] = small_configs(a)
         print("small configs %s" % {k: round(v["gpu_over_cpu"], 1)

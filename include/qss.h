@@ -282,7 +282,11 @@ int  qss_set_spectral(qss_ctx *c, int id, const double *host);
 int  qss_set_bins(qss_ctx *c, const double *bins, int nbins);
 /* Re-seeds and sweeps layers [0, nrun), storing the state at every
  * `every`-th layer start (device memory, kept in the context) and the
- * history planes of history_mask.  residuals: host, nrun values or NULL.   */
+ * history planes of history_mask.  every = 0: the forward records the
+ * reverse sweep's tape of every layer instead (particles at the layer start
+ * and each pass's push field, 2 x 8 x (3 + ppc x 3) doubles per cell and
+ * layer for nrun layers, ~25 MB per layer at 256^2 x 4 ppc), and
+ * qss_sweep_vjp recomputes nothing.  residuals: host, nrun values or NULL. */
 int  qss_sweep_checkpointed(qss_ctx *c, int nrun, int every, double *residuals);
 /* Vector-Jacobian product of the preceding checkpointed sweep: ex_bar =
  * dLoss/dEx of layers [0, nrun) (nrun*Ny*Nz doubles, layer-major; device

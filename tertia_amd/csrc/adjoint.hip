@@ -243,6 +243,7 @@ struct BeamAdjArgs {
   const double *rbar, *jbar;   // adjoints of the layer's beam rho, Jx planes
   double *q;                   // q[2*slice + side]
   int slice;
+  const int *slice_dev;        // graph replay: the slice is read from device memory
 };
 
 __device__ inline uint64_t mix64(uint64_t x) {
@@ -263,7 +264,8 @@ constexpr int NB = 256;
 __global__ __launch_bounds__(NB) void k_beam_adj(BeamAdjArgs a) {
   __shared__ double red[NB];
   const int t = threadIdx.x;
-  const int i = a.nslices - 1 - a.slice;   // reference layer index
+  const int slice = a.slice_dev ? *a.slice_dev : a.slice;
+  const int i = a.nslices - 1 - slice;   // reference layer index
   const qss_bunch &B = a.b0;
   const double Ry = sqrt(2.0) * B.sigma_r;
   const long ncell = (long)a.ny * a.nz;
@@ -305,7 +307,7 @@ __global__ __launch_bounds__(NB) void k_beam_adj(BeamAdjArgs a) {
       if (t < o) red[t] += red[t + o];
       __syncthreads();
     }
-    if (t == 0) a.q[(2L * a.slice + side) * kBeamAdjBlocks + blockIdx.x] = red[0];
+    if (t == 0) a.q[(2L * slice + side) * kBeamAdjBlocks + blockIdx.x] = red[0];
     __syncthreads();
   }
 }
@@ -336,6 +338,7 @@ hipError_t launch_beam_adj(qss_ctx *c, int slice, const double *rbar, const doub
   a.seed = c->p.seed;
   a.b0 = c->p.bunch[0];
   a.rbar = rbar; a.jbar = jbar; a.q = q; a.slice = slice;
+  a.slice_dev = c->capturing ? c->d_slice : nullptr;
   hipLaunchKernelGGL(k_beam_adj, dim3(kBeamAdjBlocks), dim3(NB), 0, c->stream, a);
   return hipGetLastError();
 }

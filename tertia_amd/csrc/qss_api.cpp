@@ -615,7 +615,15 @@ int checkpoint(qss_ctx *c, char *base, int restore) {
   return QSS_OK;
 }
 
+void adj_graph_free(qss_ctx *c) {
+  if (c->adj_gexec) (void)hipGraphExecDestroy(c->adj_gexec);
+  if (c->adj_graph) (void)hipGraphDestroy(c->adj_graph);
+  c->adj_gexec = nullptr;
+  c->adj_graph = nullptr;
+}
+
 void adj_free(qss_ctx *c) {
+  adj_graph_free(c);   // the graph holds the buffers freed here
   AdjState &a = c->adj;
   for (auto &p : a.A) dfree(p);
   dfree(a.pbarEB); dfree(a.fbar); dfree(a.jbar); dfree(a.bbar);
@@ -718,13 +726,16 @@ int adj_deposit_transform(qss_ctx *c) {
 }
 
 // Reverse of do_layer_step for layer `slice` (tape slot), the loss adjoint of
-// the layer's final Ex plane being exbar (device, may be null).
+// the layer's final Ex plane being plane `slice` of exbar (device, may be
+// null).  Capturing: the slice is read from d_slice at replay.
 int adj_layer_step(qss_ctx *c, int slice, char *slot, const double *exbar, int &cur_adj) {
   AdjState &s = c->adj;
   const long nc = c->ncell;
   const double sinv = 1.0 / (4.0 * c->ny * c->nz);
   // final fields of the layer: loss + the next layer's P-layer gathers
-  if (exbar) HIPCHECK(launch_axpy(s.pbarEB, exbar, 1.0, nc, c->stream));
+  if (exbar)
+    HIPCHECK(c->capturing ? launch_axpy(s.pbarEB, exbar, 1.0, nc, c->stream, c->d_slice)
+                          : launch_axpy(s.pbarEB, exbar + (size_t)slice * nc, 1.0, nc, c->stream));
   {
     const int kz[6] = {QSS_KIND_C, QSS_KIND_C, QSS_KIND_S, QSS_KIND_S, QSS_KIND_S, QSS_KIND_C};
     const int ky[6] = {QSS_KIND_C, QSS_KIND_S, QSS_KIND_C, QSS_KIND_S, QSS_KIND_C, QSS_KIND_S};
@@ -798,6 +809,7 @@ void graph_free(qss_ctx *c) {
   if (c->graph) (void)hipGraphDestroy(c->graph);
   c->gexec = nullptr;
   c->graph = nullptr;
+  adj_graph_free(c);
 }
 
 // Layers s and s+1 by one graph launch.  The graph holds two layer steps
@@ -807,7 +819,7 @@ void graph_free(qss_ctx *c) {
 // first (returns 1 then: one layer done).
 int sweep_graph_pair(qss_ctx *c, int s, int *done) {
   const bool match = c->gexec && c->g_key_c0 == c->C[0] && c->g_key_cur == c->cur &&
-                     c->g_key_rb == c->S[SP_RHOB];
+                     c->g_key_rb == c->S[SP_RHOB] && c->g_key_skip == c->skip_rho_psi;
   if (c->gexec && !match) {
     CHECK(do_layer_step(c, s, 0, c->d_res_hist + s));
     CHECK(store_history(c, s));
@@ -818,6 +830,7 @@ int sweep_graph_pair(qss_ctx *c, int s, int *done) {
     c->g_key_c0 = c->C[0];
     c->g_key_cur = c->cur;
     c->g_key_rb = c->S[SP_RHOB];
+    c->g_key_skip = c->skip_rho_psi;
     HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
     int r = QSS_OK;
@@ -843,6 +856,64 @@ int sweep_graph_pair(qss_ctx *c, int s, int *done) {
   HIPCHECK(launch_fill_i32(c->d_slice, 1, s, c->stream));
   HIPCHECK(hipGraphLaunch(c->gexec, c->stream));
   *done = 2;
+  return QSS_OK;
+}
+
+// One full checkpoint segment of the reverse sweep, layers s0 .. s0+K-1
+// (recomputation with the tape, then the reverse layers), by one graph
+// launch.  The graph is captured at the first full segment; the slice index
+// of every slice-dependent launch is read from d_slice, which the graph
+// advances through the recomputation and steps back through the reverse.  A
+// segment met in another pointer state (odd K, or a changed loss adjoint
+// array) returns *done = false and runs eagerly.  The checkpoint restore
+// before it stays outside the graph.
+int adj_segment_graph(qss_ctx *c, int s0, const double *exbar, int &cur_adj, bool *done) {
+  AdjState &a = c->adj;
+  *done = false;
+  const void *key[6] = {c->C[0], c->S[SP_RHOB], a.A[SP_RHOB], exbar, a.tape, a.pbar[0]};
+  const int keyi[4] = {c->cur, cur_adj, a.K, c->skip_rho_psi ? 1 : 0};
+  bool match = c->adj_gexec != nullptr;
+  for (int k = 0; k < 6 && match; ++k) match = c->adj_key[k] == key[k];
+  for (int k = 0; k < 4 && match; ++k) match = c->adj_key_i[k] == keyi[k];
+  if (c->adj_gexec && !match) return QSS_OK;
+  if (!c->adj_gexec) {
+    for (int k = 0; k < 6; ++k) c->adj_key[k] = key[k];
+    for (int k = 0; k < 4; ++k) c->adj_key_i[k] = keyi[k];
+    HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    int r = QSS_OK;
+    // the captures run no work but advance the host-side pointer state (C/P
+    // and beam-spectrum swaps, adjoint parity) by one segment, as the replay
+    // does on the device
+    for (int l = 0; l < a.K && r == QSS_OK; ++l) {
+      // not the first layer; residual slot of layer 0 (+ d_slice at replay)
+      r = do_layer_step(c, 1, 0, c->d_res_hist, a.tape + (size_t)l * a.tape_stride);
+      if (r == QSS_OK && launch_add_i32(c->d_slice, 1, c->stream) != hipSuccess) r = QSS_E_HIP;
+    }
+    for (int l = a.K - 1; l >= 0 && r == QSS_OK; --l) {
+      if (launch_add_i32(c->d_slice, -1, c->stream) != hipSuccess) {
+        r = QSS_E_HIP;
+        break;
+      }
+      r = adj_layer_step(c, 1, a.tape + (size_t)l * a.tape_stride, exbar, cur_adj);
+    }
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (r != QSS_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (r == QSS_OK) qss_set_error(std::string("graph capture: ") + hipGetErrorString(e));
+      return r != QSS_OK ? r : QSS_E_HIP;
+    }
+    c->adj_graph = g;
+    HIPCHECK(hipGraphInstantiate(&c->adj_gexec, g, nullptr, nullptr, 0));
+  } else {
+    // a replay advances the device state only: the host-side state of K
+    // layers (K even: every swap and the parity come back)
+  }
+  HIPCHECK(launch_fill_i32(c->d_slice, 1, s0, c->stream));
+  HIPCHECK(hipGraphLaunch(c->adj_gexec, c->stream));
+  *done = true;
   return QSS_OK;
 }
 
@@ -1239,6 +1310,28 @@ int qss_reset(qss_ctx *c) {
   return QSS_OK;
 }
 
+// inside a sweep whose history holds neither rho nor psi, the final passes
+// leave the layers' rho and psi planes out (only the last layer's are made)
+struct SkipRhoPsi {
+  qss_ctx *c;
+  explicit SkipRhoPsi(qss_ctx *c_) : c(c_) {
+    const uint32_t rho_psi = (1u << QSS_RHO) | (1u << QSS_PSI);
+    c->skip_rho_psi = (c->hmask & rho_psi) == 0;
+  }
+  ~SkipRhoPsi() { c->skip_rho_psi = false; }   // also on an early error return
+};
+
+// the last layer's rho and psi planes (P after the swap) from its final
+// spectra, after a sweep that skipped them
+int finish_rho_psi(qss_ctx *c) {
+  const int kz[2] = {QSS_KIND_C, QSS_KIND_C}, ky[2] = {QSS_KIND_C, QSS_KIND_C};
+  const double *in[2] = {c->S[SP_RHO], c->S[SP_PSI]};
+  double *out[2] = {c->P[QSS_RHO], c->P[QSS_PSI]};
+  c->skip_rho_psi = false;
+  return transform_batch(c, 2, kz, ky, in, out, 1.0 / (4.0 * c->ny * c->nz), nullptr, nullptr,
+                         nullptr);
+}
+
 int qss_sweep_range(qss_ctx *c, int first, int count, double *residuals) {
   if (!c || first < 0 || count < 0 || first + count > c->p.nslices) {
     qss_set_error("bad sweep range");
@@ -1250,14 +1343,7 @@ int qss_sweep_range(qss_ctx *c, int first, int count, double *residuals) {
   if (!c->t0) { HIPCHECK(hipEventCreate(&c->t0)); HIPCHECK(hipEventCreate(&c->t1)); }
   HIPCHECK(hipEventRecord(c->t0, c->stream));
   const int S = c->p.nx_split;
-  // without rho or psi in the history the layers' rho and psi planes are
-  // not needed inside the sweep: only the last layer's are produced, below
-  const uint32_t rho_psi = (1u << QSS_RHO) | (1u << QSS_PSI);
-  c->skip_rho_psi = (c->hmask & rho_psi) == 0;
-  struct Reset {
-    qss_ctx *c;
-    ~Reset() { c->skip_rho_psi = false; }   // also on an early error return
-  } reset_skip{c};
+  SkipRhoPsi skip(c);
   for (int s = first; s < first + count;) {
     if (c->use_graph && !c->timing && S == 1 && s >= 1 && first + count - s >= 2) {
       int done = 0;
@@ -1270,15 +1356,7 @@ int qss_sweep_range(qss_ctx *c, int first, int count, double *residuals) {
     CHECK(store_history(c, s));
     ++s;
   }
-  if (c->skip_rho_psi && count > 0) {
-    // the last layer (P after the swap) from its final spectra
-    const int kz[2] = {QSS_KIND_C, QSS_KIND_C}, ky[2] = {QSS_KIND_C, QSS_KIND_C};
-    const double *in[2] = {c->S[SP_RHO], c->S[SP_PSI]};
-    double *out[2] = {c->P[QSS_RHO], c->P[QSS_PSI]};
-    c->skip_rho_psi = false;
-    CHECK(transform_batch(c, 2, kz, ky, in, out, 1.0 / (4.0 * c->ny * c->nz), nullptr,
-                          nullptr, nullptr));
-  }
+  if (c->skip_rho_psi && count > 0) CHECK(finish_rho_psi(c));
   HIPCHECK(hipEventRecord(c->t1, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   // wake-major: wake w's residuals at [w*count*S, (w+1)*count*S)
@@ -1342,24 +1420,47 @@ int qss_device_memory(int device, int64_t *used, int64_t *total) {
 }
 
 int qss_sweep_checkpointed(qss_ctx *c, int nrun, int every, double *residuals) {
-  if (!c || nrun < 1 || nrun > c->p.nslices || every < 1) {
+  if (!c || nrun < 1 || nrun > c->p.nslices || every < 0) {
     qss_set_error("checkpointed sweep: bad layer count or interval");
     return QSS_E_ARG;
   }
   CHECK(adj_check(c));
   HIPCHECK(hipSetDevice(c->device));
-  CHECK(adj_alloc(c, nrun, every));
+  // every = 0: one segment of all nrun layers, its tape recorded here
+  const bool taped = every == 0;
+  CHECK(adj_alloc(c, nrun, taped ? nrun : every));
+  c->adj.taped = false;
   for (auto &u : c->ev_used) u = 0;
   c->gemm_flops = 0;
   if (!c->t0) { HIPCHECK(hipEventCreate(&c->t0)); HIPCHECK(hipEventCreate(&c->t1)); }
   HIPCHECK(hipEventRecord(c->t0, c->stream));
   CHECK(do_reset(c));
   AdjState &a = c->adj;
-  for (int s = 0; s < nrun; ++s) {
+  SkipRhoPsi skip(c);   // as the plain sweep, so that both take the same steps
+  if (taped) {
+    CHECK(checkpoint(c, a.chk, 0));
+    for (int s = 0; s < nrun; ++s) {
+      CHECK(do_layer_step(c, s, 0, c->d_res_hist + s, a.tape + (size_t)s * a.tape_stride));
+      CHECK(store_history(c, s));
+    }
+    a.taped = true;
+    every = nrun;   // skips the loop below
+  }
+  for (int s = taped ? nrun : 0; s < nrun;) {
     if (s % every == 0) CHECK(checkpoint(c, a.chk + (size_t)(s / every) * a.chk_stride, 0));
+    // pairs of layers by graph launches up to the next checkpoint
+    const int lim = std::min(nrun, (s / every + 1) * every);
+    if (c->use_graph && !c->timing && s >= 1 && lim - s >= 2) {
+      int done = 0;
+      CHECK(sweep_graph_pair(c, s, &done));
+      s += done;
+      continue;
+    }
     CHECK(do_layer_step(c, s, 0, c->d_res_hist + s));
     CHECK(store_history(c, s));
+    ++s;
   }
+  if (c->skip_rho_psi) CHECK(finish_rho_psi(c));
   HIPCHECK(hipEventRecord(c->t1, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   if (residuals)
@@ -1397,15 +1498,25 @@ int qss_sweep_vjp(qss_ctx *c, const double *ex_bar, int on_host, double *grad_bi
   HIPCHECK(launch_fill_f64(a.pbar[1], c->np_cap * 5, 0.0, c->stream));
   HIPCHECK(launch_fill_f64(a.qbeam, (long)c->p.nslices * 2 * kBeamAdjBlocks, 0.0, c->stream));
   int cur_adj = 0;
-  for (int seg = a.nchk - 1; seg >= 0; --seg) {
+  SkipRhoPsi skip(c);   // the recomputation takes the checkpointed forward's steps
+  if (a.taped)          // the forward recorded every layer: reverse only
+    for (int s = a.nrun - 1; s >= 0; --s)
+      CHECK(adj_layer_step(c, s, a.tape + (size_t)s * a.tape_stride, ex_bar_dev, cur_adj));
+  for (int seg = a.taped ? -1 : a.nchk - 1; seg >= 0; --seg) {
     const int s0 = seg * a.K, s1 = std::min(a.nrun, s0 + a.K);
     // recompute the segment from its checkpoint, recording the tape
     CHECK(checkpoint(c, a.chk + (size_t)seg * a.chk_stride, 1));
+    // full segments after the first layer's by one graph launch (even K:
+    // every such segment starts in the same pointer state)
+    if (c->use_graph && !c->timing && s0 > 0 && s1 - s0 == a.K && a.K % 2 == 0) {
+      bool done = false;
+      CHECK(adj_segment_graph(c, s0, ex_bar_dev, cur_adj, &done));
+      if (done) continue;
+    }
     for (int s = s0; s < s1; ++s)
       CHECK(do_layer_step(c, s, 0, c->d_res_hist + s, a.tape + (size_t)(s - s0) * a.tape_stride));
     for (int s = s1 - 1; s >= s0; --s)
-      CHECK(adj_layer_step(c, s, a.tape + (size_t)(s - s0) * a.tape_stride,
-                           ex_bar_dev ? ex_bar_dev + (size_t)s * nc : nullptr, cur_adj));
+      CHECK(adj_layer_step(c, s, a.tape + (size_t)(s - s0) * a.tape_stride, ex_bar_dev, cur_adj));
   }
   HIPCHECK(hipEventRecord(c->t1, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));

@@ -161,6 +161,7 @@ struct AdjTileArgs {
 // buffers of the checkpointed forward and of the reverse sweep
 struct AdjState {
   int K = 0, nrun = 0, nchk = 0;
+  bool taped = false;          // the forward recorded every layer's tape (every = 0)
   double *A[SP_N] = {};        // adjoint of the spectral state
   double *pbarEB = nullptr;    // 6 planes: adjoint of the final E,B of a layer
   double *fbar = nullptr;      // 6 planes: adjoint of a push field
@@ -171,7 +172,8 @@ struct AdjState {
   double *qbeam = nullptr;     // per layer and x-cell side: d(loss)/d(profile)
   char *chk = nullptr;         // checkpoints (state at layer starts)
   size_t chk_stride = 0;
-  char *tape = nullptr;        // per layer of a segment: particles + push fields
+  char *tape = nullptr;        // per layer of a segment (all layers when taped):
+                               // particles + push fields
   size_t tape_stride = 0;
   double *exbar = nullptr;     // loss adjoint of the Ex history given on the host
 };
@@ -298,9 +300,16 @@ struct qss_ctx {
   int *d_slice = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // one checkpoint segment of the reverse sweep (recomputation + reverse),
+  // replayed for every full segment captured in the same pointer state
+  hipGraph_t adj_graph = nullptr;
+  hipGraphExec_t adj_gexec = nullptr;
+  const void *adj_key[6] = {};
+  int adj_key_i[4] = {-1, -1, -1, -1};
   const double *g_key_c0 = nullptr;   // pointer state the graph was captured in
   int g_key_cur = -1;
   const double *g_key_rb = nullptr;
+  bool g_key_skip = false;
 
   // transform GEMM tile shape per (pass, batch): pass 0 = X.T_y, 1 = T_z.Y
   int gemm_var[2][kGemmMaxBatch + 1] = {};
@@ -355,7 +364,8 @@ hipError_t launch_push_tile(qss_ctx *c, int full, hipEvent_t ev0 = nullptr,
 hipError_t launch_combine(qss_ctx *c, double *const *out = nullptr);
 hipError_t launch_tile_reorder(qss_ctx *c);
 hipError_t launch_seed_tiled(qss_ctx *c);
-hipError_t launch_axpy(double *y, const double *x, double a, long n, hipStream_t s);
+hipError_t launch_axpy(double *y, const double *x, double a, long n, hipStream_t s,
+                       const int *slice_dev = nullptr);
 // graph replay helpers: history store at the device-side slice, slice += v
 hipError_t launch_store_hist(qss_ctx *c, const double *src, double *hist);
 hipError_t launch_add_i32(int *p, int v, hipStream_t s);

@@ -347,8 +347,11 @@ __global__ void k_add_i32(int *p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p += v;
 }
 
-__global__ void k_axpy(double *y, const double *x, double alpha, long n) {
+// slice_dev (graph replay): x is the plane of slice *slice_dev in an array of
+// planes of n values
+__global__ void k_axpy(double *y, const double *x, double alpha, long n, const int *slice_dev) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slice_dev) x += (long)(*slice_dev) * n;
   if (i < n) y[i] += alpha * x[i];
 }
 
@@ -356,9 +359,10 @@ inline dim3 grid1(long n) { return dim3((unsigned)((n + NT - 1) / NT)); }
 
 }  // namespace
 
-hipError_t launch_axpy(double *y, const double *x, double a, long n, hipStream_t s) {
+hipError_t launch_axpy(double *y, const double *x, double a, long n, hipStream_t s,
+                       const int *slice_dev) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_axpy, grid1(n), dim3(NT), 0, s, y, x, a, n);
+  hipLaunchKernelGGL(k_axpy, grid1(n), dim3(NT), 0, s, y, x, a, n, slice_dev);
   return hipGetLastError();
 }
 

@@ -83,6 +83,54 @@ def test_checkpointed_forward_is_the_forward():
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("every", [8, 6])
+def test_graph_replay_is_eager(every):
+    """Checkpointed forward by graph pairs and the reverse by one graph per
+    full checkpoint segment against the same sweeps launched eagerly:
+    histories and gradients bitwise equal (every = 6: a partial segment of
+    two layers first in the reverse)."""
+    from tertia_amd.mesh import Mesh
+    p = small_bins()
+    nrun = 56
+    out = []
+    for graph in (True, False):
+        m = Mesh(p, history=("Ex",))
+        try:
+            m.enable_graph(graph)
+            loss, g, ex = adjoint_grad(m, p, nrun, every, WIN_SMALL)
+            g2 = m.sweep_vjp(witness_loss_np(ex, p, **WIN_SMALL)[1])   # replayed again
+            out.append((ex, g, g2))
+        finally:
+            m.close()
+    (ex_g, g_g, g2_g), (ex_e, g_e, g2_e) = out
+    assert np.abs(g_e).max() > 0
+    assert np.array_equal(ex_g, ex_e)
+    assert np.array_equal(g_g, g_e)
+    assert np.array_equal(g2_g, g_e) and np.array_equal(g2_e, g_e)
+
+
+def test_taped_forward_is_the_checkpointed():
+    """every = 0: the forward records every layer's tape and the reverse
+    recomputes nothing; histories and gradient bitwise those of the
+    checkpointed sweep (every = 8), and the sweep is the forward."""
+    from tertia_amd.mesh import Mesh
+    p = small_bins()
+    nrun = 56
+    m = Mesh(p, history=("Ex",))
+    try:
+        m.reset()
+        m.sweep_range(0, nrun)
+        ex0 = np.stack([m.history("Ex", s) for s in range(nrun)])
+        _, g8, ex8 = adjoint_grad(m, p, nrun, 8, WIN_SMALL)
+        _, g0, ex0t = adjoint_grad(m, p, nrun, 0, WIN_SMALL)
+        g0b = m.sweep_vjp(witness_loss_np(ex0t, p, **WIN_SMALL)[1])   # the tape is kept
+    finally:
+        m.close()
+    assert np.abs(g8).max() > 0
+    assert np.array_equal(ex0, ex8) and np.array_equal(ex0, ex0t)
+    assert np.array_equal(g0, g8) and np.array_equal(g0b, g8)
+
+
 def test_zero_loss_adjoint_is_zero():
     from tertia_amd.mesh import Mesh
     p = small_bins()
