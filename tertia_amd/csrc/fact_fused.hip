@@ -320,7 +320,12 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_y(FusedArgs a) {
 #pragma unroll
       for (int i = 0; i < TI; ++i) av[i] = As[buf][kr][(wm * WTM + i * 16 + li) ^ sa];
 #pragma unroll
-      for (int j = 0; j < TJ; ++j) bv[j] = Bs[buf][kr][(wn * WTN + j * 16 + li) ^ sb];
+      for (int jp = 0; jp < TJ / 2; ++jp) {   // tiles 2jp, 2jp+1 interleaved: one 16-byte read
+        const double2 v =
+            *reinterpret_cast<const double2 *>(&Bs[buf][kr][(wn * WTN + 32 * jp + 2 * li) ^ sb]);
+        bv[2 * jp] = v.x;
+        bv[2 * jp + 1] = v.y;
+      }
 #pragma unroll
       for (int i = 0; i < TI; ++i)
 #pragma unroll
@@ -440,7 +445,7 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_z(FusedArgs a) {
       if (UNITS % NT != 0 && U >= UNITS) continue;
       const int yl = U % (BN / 2), r = U / (BN / 2);
       const int w = r & 1, pp = (r >> 1) & 1, ul = r >> 2;
-      const int cc = 32 * (yl >> 4) + (yl & 15) + 16 * pp;
+      const int cc = 32 * (yl >> 4) + 2 * (yl & 15) + pp;   // the two parities interleaved
       const int kr = 4 * ul + 2 * w;
       Bs[buf][kr][cc ^ SwzF64::xb(kr)] = rb_[q][0];
       Bs[buf][kr + 1][cc ^ SwzF64::xb(kr + 1)] = rb_[q][1];
@@ -461,7 +466,12 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_z(FusedArgs a) {
 #pragma unroll
       for (int i = 0; i < TI; ++i) av[i] = As[buf][kr][(wm * WTM + i * 16 + li) ^ sa];
 #pragma unroll
-      for (int j = 0; j < TJ; ++j) bv[j] = Bs[buf][kr][(wn * WTN + j * 16 + li) ^ sb];
+      for (int jp = 0; jp < TJ / 2; ++jp) {   // tiles 2jp, 2jp+1 interleaved: one 16-byte read
+        const double2 v =
+            *reinterpret_cast<const double2 *>(&Bs[buf][kr][(wn * WTN + 32 * jp + 2 * li) ^ sb]);
+        bv[2 * jp] = v.x;
+        bv[2 * jp + 1] = v.y;
+      }
 #pragma unroll
       for (int i = 0; i < TI; ++i)
 #pragma unroll
@@ -795,7 +805,11 @@ void fact_fused_matrix(int n, int pass, std::vector<double> &g) {
     for (int x = 0; x < h; ++x) {
       int c, so;
       if (pass == 0 || pass == 3) {
-        const int gg = x / 64, slot = (x % 64) / 16, l = x % 16, k = 16 * gg + l;
+        // pass 0 stores the columns of tiles 2j and 2j+1 interleaved (the
+        // staged kernels read both with one 16-byte LDS read); xl is the column
+        // in GEMM order
+        const int xl = pass == 0 ? 32 * (x / 32) + 16 * (x & 1) + (x % 32) / 2 : x;
+        const int gg = xl / 64, slot = (xl % 64) / 16, l = xl % 16, k = 16 * gg + l;
         c = slot < 2 ? k : m - 1 - k;
         so = slot & 1;
         g[(size_t)kp * h + x] = val(q, c, si, so);
