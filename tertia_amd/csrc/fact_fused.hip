@@ -70,14 +70,19 @@ __device__ inline d4 mma64(double x, double y, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
 }
 
-// block -> (wake, field, tile origin) with XCD-contiguous numbering
+// block -> (wake, field, tile origin)
 struct FItem {
   int w, f, m0, n0;
 };
-template <int BM, int BN>
+// block -> (wake, field, tile origin) with XCD-contiguous numbering: the
+// work-groups of one XCD take consecutive tiles, which share an operand in
+// that XCD's L2.  M_FAST: consecutive tiles run down a column of tiles (the z
+// pass: its row tiles all read the same columns of the plane); otherwise along
+// a row of tiles (the y pass: its column tiles all read the same rows).
+template <int BM, int BN, bool M_FAST = false>
 __device__ inline FItem fitem(int GM, int GN, int nbatch) {
-  const int tiles_n = GN / BN;
-  const int per_field = (GM / BM) * tiles_n;
+  const int tiles_m = GM / BM, tiles_n = GN / BN;
+  const int per_field = tiles_m * tiles_n;
   const int nblk = gridDim.x;
   int bid = blockIdx.x;
   if (nblk % kXcdF == 0) bid = (bid % kXcdF) * (nblk / kXcdF) + bid / kXcdF;
@@ -86,9 +91,15 @@ __device__ inline FItem fitem(int GM, int GN, int nbatch) {
   it.w = fw / nbatch;
   it.f = fw - it.w * nbatch;
   const int tt = bid - fw * per_field;
-  const int tm = tt / tiles_n;
-  it.m0 = tm * BM;
-  it.n0 = (tt - tm * tiles_n) * BN;
+  if (M_FAST) {
+    const int tn = tt / tiles_m;
+    it.n0 = tn * BN;
+    it.m0 = (tt - tn * tiles_m) * BM;
+  } else {
+    const int tm = tt / tiles_n;
+    it.m0 = tm * BM;
+    it.n0 = (tt - tm * tiles_n) * BN;
+  }
   return it;
 }
 
@@ -377,7 +388,7 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_z(FusedArgs a) {
   constexpr int APT = (ACH + NT - 1) / NT;
   const int Ny = a.ny, Nz = a.nz, h = Nz / 2, m = Nz / 4;
   const int GM = h, GN = 2 * Ny, K = h;
-  const FItem it = fitem<BM, BN>(GM, GN, a.nbatch);
+  const FItem it = fitem<BM, BN, true>(GM, GN, a.nbatch);
   const int m0 = it.m0, n0 = it.n0, fb = it.f;
   const bool rev = a.kz[fb] == QSS_KIND_S;
 
@@ -625,7 +636,7 @@ __global__ __launch_bounds__(KS * 64) void k_fact_zd(FusedArgs a) {
   constexpr int BM = 16 * TI, BN = 16 * TJ;
   static_assert(TJ % 2 == 0, "32-column groups");
   const int Ny = a.ny, Nz = a.nz, GM = Nz / 2, GN = 2 * Ny, K = Nz / 2;
-  const FItem it = fitem<BM, BN>(GM, GN, a.nbatch);
+  const FItem it = fitem<BM, BN, true>(GM, GN, a.nbatch);
   const int m0 = it.m0, n0 = it.n0, fb = it.f;
   const bool rev = a.kz[fb] == QSS_KIND_S;
   const int t = threadIdx.x, lane = t & 63;
