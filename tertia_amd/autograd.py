@@ -7,14 +7,17 @@
     loss = witness_loss(ex, p, 6.0, 6.5, 0.2)
     loss.backward()                                   # bins.grad via the adjoint
 
-The forward runs the checkpointed sweep on the library's stream and copies
-the Ex history (the longitudinal field) into a torch tensor; the backward
+The forward runs the checkpointed sweep on the library's stream (`every`
+layers between checkpoints; 0: every layer's tape recorded, so that the
+backward recomputes nothing) and copies the Ex history (the longitudinal
+field) into a torch tensor; the backward
 hands dLoss/dEx (a device tensor) to the hand-written adjoint and returns
 dLoss/dbins.  Both directions synchronize the device around the library
 calls, which run on the library's own stream.  One forward per mesh may be
 awaiting its backward: the reverse sweep recomputes from the mesh's latest
 checkpoints, so a second SweepEx.apply on the same mesh before the first
-backward makes that backward raise instead of returning the wrong gradient.
+backward makes that backward raise instead of returning the wrong gradient
+(the reverse reads the mesh's latest checkpoints or tape).
 """
 from __future__ import annotations
 
