@@ -34,6 +34,11 @@
 // pass stores rows of consecutive y (coalesced) through the dense path's
 // epilogue (alpha, beta, corrector residual, push field).
 //
+// Two kernel families, autotuned per pass and batch size: staged (operand
+// slabs through double-buffered LDS; the B tiles 2j and 2j+1 interleaved
+// there, so one 16-byte read feeds both) and direct (each wave loads its own
+// fragments from L2, with its own K order, further below).
+//
 // Shapes: both sizes multiples of 128 (at most 1024), FP64.  numpy
 // restatement of this index algebra (tables, loaders, butterflies) agrees with
 // scipy's type-4 transforms to 1e-15; the GPU tests compare with the oracle.
@@ -217,7 +222,7 @@ __device__ inline void z_epilogue(const FusedArgs &a, const d4 (&acc)[TI][TJ], i
 
 
 // ---------------------------------------------------------------------------
-// y pass: virtual A (2 Nz x h) = pre-twiddled rows of X, B = G'_y (h x h),
+// y pass: virtual A (2 Nz x h) = the pairs of the rows of X, B = G'_y (h x h),
 // output Y (Nz x Ny, natural layout) after the butterflies along y
 // ---------------------------------------------------------------------------
 template <int BM, int BN, int BK, int WGM, int WGN, int KS, int PF>
@@ -368,7 +373,7 @@ __global__ __launch_bounds__(WGM * WGN * KS * 64) void k_fact_y(FusedArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// z pass: A = G'_z (h_z x h_z), virtual B (h_z x 2 Ny) = pre-twiddled columns
+// z pass: A = G'_z (h_z x h_z), virtual B (h_z x 2 Ny) = the pairs of the columns
 // of Y, output C (Nz x Ny) after the butterflies along z and the epilogue
 // ---------------------------------------------------------------------------
 template <int BM, int BN, int BK, int WGM, int WGN, int KS, int PF>
