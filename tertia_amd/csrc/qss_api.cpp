@@ -907,10 +907,9 @@ int adj_segment_graph(qss_ctx *c, int s0, const double *exbar, int &cur_adj, boo
     }
     c->adj_graph = g;
     HIPCHECK(hipGraphInstantiate(&c->adj_gexec, g, nullptr, nullptr, 0));
-  } else {
-    // a replay advances the device state only: the host-side state of K
-    // layers (K even: every swap and the parity come back)
   }
+  // a replay advances the device state only; the host-side state of K layers
+  // (K even) is the state it started from: every swap and the parity come back
   HIPCHECK(launch_fill_i32(c->d_slice, 1, s0, c->stream));
   HIPCHECK(hipGraphLaunch(c->adj_gexec, c->stream));
   *done = true;
