@@ -64,41 +64,88 @@ __device__ double bunch_density(const BeamArgs &a, const qss_bunch *bun, int b, 
   return B.nb * exp(-arg);
 }
 
+// Each work-group covers a 16 x 16 node tile.  A node (nj, nk) receives from
+// the cells (nj-1 .. nj, nk-1 .. nk), so the tile reads the 17 x 17 cells from
+// (j0-1, k0-1): their densities are evaluated once per x-cell and bunch into
+// LDS (bunch_density with the same arguments as a per-node evaluation would
+// use, so the values are bitwise the same) and each node sums its four cells
+// from there.
+constexpr int BT = 16;            // nodes per tile edge (NT = BT * BT)
+constexpr int BC = BT + 1;        // cells per tile edge
+constexpr int NBC = BC * BC;      // cells per tile
+static_assert(NT == BT * BT, "one thread per node of a tile");
+static_assert(3 * QSS_MAX_BUNCH <= 32, "skip bits");
+
+// x-cell ci lies outside bunch b (exact early out: arg = dx^2/Rx^2 +
+// (non-negative terms) exceeds 16 as soon as its first term does, rounded sums
+// of non-negative terms do not decrease), or outside the x range: its cells
+// add nothing
+__device__ inline bool xcell_out(const BeamArgs &a, const qss_bunch *bun, int b, int ci) {
+  if (ci < 0 || ci > a.nslices) return true;
+  if (b == 0 && a.nbins > 0) return false;
+  const qss_bunch &B = bun[b];
+  const double xi = (double)(a.nslices - ci) * a.hx;
+  const double Rx = sqrt(2.0) * B.sigma_xi;
+  const double dx = xi - B.xi0;
+  return dx * dx / Rx / Rx > 16;
+}
+
+// dn[(x * nbunch + b) * NBC + cell] for x-cells ci0 + x, x < nx; bit x *
+// nbunch + b of the result is set when that x-cell and bunch add nothing
+// (its entries are then not written)
+__device__ unsigned cell_densities(const BeamArgs &a, const qss_bunch *bun, int ci0, int nx,
+                                   int j0, int k0, double *dn) {
+  unsigned skip = 0;
+  for (int x = 0; x < nx; ++x)
+    for (int b = 0; b < a.nbunch; ++b) {
+      const int ci = ci0 + x;
+      if (xcell_out(a, bun, b, ci)) {   // uniform over the work-group
+        skip |= 1u << (x * a.nbunch + b);
+        continue;
+      }
+      const double xi = (double)(a.nslices - ci) * a.hx;
+      double *d = dn + (x * a.nbunch + b) * NBC;
+      for (int cell = threadIdx.x; cell < NBC; cell += NT) {
+        const int cj = j0 - 1 + cell % BC, ck = k0 - 1 + cell / BC;
+        double v = 0.0;
+        if (cj >= 0 && ck >= 0 && cj < a.ny && ck < a.nz) {
+          const double yc = a.hy * cj - a.ly / 2.0;
+          const double zc = a.hz * ck - a.lz / 2.0;
+          v = bunch_density(a, bun, b, xi, yc, zc);
+        }
+        d[cell] = v;
+      }
+    }
+  return skip;
+}
+
 // value of reference layer i at node (nj, nk); sums in the order
 // x-cell (i, i-1), bunch, ck, cj, seed
 __device__ void node_value(const BeamArgs &a, const qss_bunch *bun, int i, int nj, int nk,
+                           int ci0, int j0, int k0, const double *dn, unsigned skip,
                            double &rho, double &jx) {
   rho = 0.0;
   jx = 0.0;
   for (int side = 0; side < 2; ++side) {
     const int ci = side == 0 ? i : i - 1;
-    if (ci < 0 || ci > a.nslices) continue;
     const double ax = 0.5;
-    const double xi = (double)(a.nslices - ci) * a.hx;
     for (int b = 0; b < a.nbunch; ++b) {
+      const int xb = (ci - ci0) * a.nbunch + b;
+      if (skip >> xb & 1u) continue;
       const qss_bunch &B = bun[b];
-      // exact early out: arg = dx^2/Rx^2 + (non-negative terms) exceeds 16 as
-      // soon as its first term does (rounded sums of non-negative terms do
-      // not decrease), so a layer outside the bunch skips its cells
-      if (!(b == 0 && a.nbins > 0)) {
-        const double Rx = sqrt(2.0) * B.sigma_xi;
-        const double dx = xi - B.xi0;
-        if (dx * dx / Rx / Rx > 16) continue;
-      }
+      const double *dnb = dn + xb * NBC;
       for (int ck = nk - 1; ck <= nk; ++ck) {
         if (ck < 0) continue;
         const int dz_ = nk - ck;
         for (int cj = nj - 1; cj <= nj; ++cj) {
           if (cj < 0) continue;
           const int dy_ = nj - cj;
-          const double yc = a.hy * cj - a.ly / 2.0;
-          const double zc = a.hz * ck - a.lz / 2.0;
-          const double dn = bunch_density(a, bun, b, xi, yc, zc);
-          if (!(dn > 0.0)) continue;
+          const double dnc = dnb[(ck - k0 + 1) * BC + (cj - j0 + 1)];
+          if (!(dnc > 0.0)) continue;
           for (int is = 0; is < 4; ++is) {
             const double yq = 1.0 / 3.0 * (1.0 + is % 2);
             const double zq = 1.0 / 3.0 * (1.0 + is / 2);
-            double w = 0.25 * dn;
+            double w = 0.25 * dnc;
             if (B.noise != 0.0) {
               const uint64_t cell =
                   ((uint64_t)(ci + 1) * (uint64_t)a.nz + ck) * (uint64_t)a.ny + cj;
@@ -118,20 +165,28 @@ __device__ void node_value(const BeamArgs &a, const qss_bunch *bun, int i, int n
   }
 }
 
+// grid: (node tiles of the plane, wakes); dynamic LDS: nx * nbunch * NBC doubles
 __global__ __launch_bounds__(NT) void k_beam_plane(BeamArgs a) {
-  const long n = (long)blockIdx.x * NT + threadIdx.x;
+  extern __shared__ double dn[];
+  const int ntj = (a.ny + BT - 1) / BT;
+  const int j0 = (blockIdx.x % ntj) * BT, k0 = (blockIdx.x / ntj) * BT;
+  const int nj = j0 + threadIdx.x % BT, nk = k0 + threadIdx.x / BT;
   const long nc = (long)a.ny * a.nz;
-  if (n >= nc) return;
   const int w = blockIdx.y;   // wake of a batch
   const qss_bunch *bun = a.wb ? a.wb + (long)w * QSS_MAX_BUNCH : a.b;
-  double *rho = a.rho + w * nc, *jx = a.jx + w * nc;
-  const int nj = (int)(n % a.ny), nk = (int)(n / a.ny);
   const int layer = a.slice_dev ? a.nslices - 1 - *a.slice_dev : a.layer;
+  // x-cells layer - 1 .. layer (+ layer + 1 for the split interpolation)
+  const int ci0 = layer - 1;
+  const unsigned skip = cell_densities(a, bun, ci0, a.nsplit > 1 ? 3 : 2, j0, k0, dn);
+  __syncthreads();
+  if (nj >= a.ny || nk >= a.nz) return;
+  const long n = (long)nk * a.ny + nj;
+  double *rho = a.rho + w * nc, *jx = a.jx + w * nc;
   double rc, jc;
-  node_value(a, bun, layer, nj, nk, rc, jc);
+  node_value(a, bun, layer, nj, nk, ci0, j0, k0, dn, skip, rc, jc);
   if (a.nsplit > 1) {
     double rp, jp;
-    node_value(a, bun, layer + 1, nj, nk, rp, jp);
+    node_value(a, bun, layer + 1, nj, nk, ci0, j0, k0, dn, skip, rp, jp);
     const int S = a.nsplit, s = a.isplit;
     rc = (rc * (S - s) + rp * s) / S;
     jc = (jc * (S - s) + jp * s) / S;
@@ -170,7 +225,8 @@ hipError_t launch_beam_plane(qss_ctx *c, int slice, int isplit) {
     c->xs_src[0] = a.rho;
     c->xs_src[1] = a.jx;
   }
-  hipLaunchKernelGGL(k_beam_plane, dim3((unsigned)((c->ncell + NT - 1) / NT), c->nw), dim3(NT),
-                     0, c->stream, a);
+  const unsigned tiles = (unsigned)(((c->ny + BT - 1) / BT) * ((c->nz + BT - 1) / BT));
+  const size_t lds = sizeof(double) * (a.nsplit > 1 ? 3 : 2) * (size_t)(a.nbunch > 0 ? a.nbunch : 1) * NBC;
+  hipLaunchKernelGGL(k_beam_plane, dim3(tiles, c->nw), dim3(NT), lds, c->stream, a);
   return hipGetLastError();
 }

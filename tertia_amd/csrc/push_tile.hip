@@ -508,8 +508,9 @@ constexpr int NTS = 1024;
 __global__ __launch_bounds__(NTS) void k_tile_scan(const int32_t *tcount, int ntj, int ntk,
                                                    int nsplit, int32_t *tile_start,
                                                    int32_t *goff) {
-  __shared__ int sh[NTS];
+  __shared__ int wsum[NTS / 64];
   const int nt = ntj * ntk;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int w = blockIdx.x;   // one work-group per wake
   tcount += (long)w * nt * nsplit * 9;
   goff += (long)w * nt * nsplit * 9;
@@ -532,17 +533,24 @@ __global__ __launch_bounds__(NTS) void k_tile_scan(const int32_t *tcount, int nt
         }
       }
     }
-    // block exclusive scan of tot
-    sh[threadIdx.x] = tot;
-    __syncthreads();
-    for (int o = 1; o < NTS; o <<= 1) {
-      const int v = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += v;
-      __syncthreads();
+    // block exclusive scan of tot: inclusive scans within each wave, then
+    // the totals of the waves before this one (integer sums: exact)
+    int incl = tot;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
     }
-    const int incl = sh[threadIdx.x];
-    const int start = carry + incl - tot;
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int before = 0, blk_tot = 0;
+#pragma unroll
+    for (int q = 0; q < NTS / 64; ++q) {
+      const int v = wsum[q];
+      before += q < wv ? v : 0;
+      blk_tot += v;
+    }
+    const int start = carry + before + incl - tot;
     if (u < nt) {
       tile_start[u] = start;
       int o = start;
@@ -555,8 +563,7 @@ __global__ __launch_bounds__(NTS) void k_tile_scan(const int32_t *tcount, int nt
             o += g[d][q];
           }
     }
-    const int blk_tot = sh[NTS - 1];
-    __syncthreads();
+    __syncthreads();   // wsum is rewritten by the next block of tiles
     carry += blk_tot;
   }
   if (threadIdx.x == 0) tile_start[nt] = carry;
