@@ -45,31 +45,19 @@ __device__ inline double unif_pm1(uint64_t seed, uint64_t a, uint64_t b) {
   return (double)(h >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
 }
 
-__device__ double bunch_density(const BeamArgs &a, const qss_bunch *bun, int b, double xi,
-                                double yc, double zc) {
-  const qss_bunch &B = bun[b];
-  const double Ry = sqrt(2.0) * B.sigma_r;
-  const double dy = yc - B.y0, dz = zc - B.z0;
-  if (b == 0 && a.nbins > 0) {
-    const double arg = dy * dy / Ry / Ry + dz * dz / Ry / Ry;
-    if (arg > 16) return 0.0;
-    const double t = (xi - a.bin_xi0) / (a.bin_xi1 - a.bin_xi0) * a.nbins;
-    if (!(t >= 0.0) || t >= a.nbins) return 0.0;
-    return a.bins[(int)t] * exp(-arg);
-  }
-  const double Rx = sqrt(2.0) * B.sigma_xi;
-  const double dx = xi - B.xi0;
-  const double arg = dx * dx / Rx / Rx + dy * dy / Ry / Ry + dz * dz / Ry / Ry;
-  if (arg > 16) return 0.0;
-  return B.nb * exp(-arg);
-}
-
 // Each work-group covers a 16 x 16 node tile.  A node (nj, nk) receives from
 // the cells (nj-1 .. nj, nk-1 .. nk), so the tile reads the 17 x 17 cells from
 // (j0-1, k0-1): their densities are evaluated once per x-cell and bunch into
-// LDS (bunch_density with the same arguments as a per-node evaluation would
-// use, so the values are bitwise the same) and each node sums its four cells
-// from there.
+// LDS and each node sums its four cells from there.
+//
+// Density of bunch b at cell centre (xi, yc, zc) (Specie::Density case 1,
+// plasma.cpp:350-377, with R = sqrt(2) sigma):
+//   arg = dx*dx/Rx/Rx + dy*dy/Ry/Ry + dz*dz/Ry/Ry  (the profile-bin bunch: the
+//   last two terms), density nb exp(-arg) (bins[t] exp(-arg)), 0 if arg > 16.
+// The three terms depend on the x-cell, the column and the row alone, and the
+// sum is evaluated left to right, so each is computed once (per work-group:
+// the 17 column and 17 row terms per bunch) and a cell adds them in that
+// order: the same values as evaluating the expression per cell.
 constexpr int BT = 16;            // nodes per tile edge (NT = BT * BT)
 constexpr int BC = BT + 1;        // cells per tile edge
 constexpr int NBC = BC * BC;      // cells per tile
@@ -92,9 +80,25 @@ __device__ inline bool xcell_out(const BeamArgs &a, const qss_bunch *bun, int b,
 
 // dn[(x * nbunch + b) * NBC + cell] for x-cells ci0 + x, x < nx; bit x *
 // nbunch + b of the result is set when that x-cell and bunch add nothing
-// (its entries are then not written)
+// (its entries are then not written).  tyz: 2 * BC terms per bunch (scratch).
 __device__ unsigned cell_densities(const BeamArgs &a, const qss_bunch *bun, int ci0, int nx,
-                                   int j0, int k0, double *dn) {
+                                   int j0, int k0, double *dn, double *tyz) {
+  // column (dy) and row (dz) terms of every bunch
+  for (int idx = threadIdx.x; idx < a.nbunch * 2 * BC; idx += NT) {
+    const int b = idx / (2 * BC), r = idx - b * 2 * BC;
+    const qss_bunch &B = bun[b];
+    const double Ry = sqrt(2.0) * B.sigma_r;
+    double d;
+    if (r < BC) {
+      const double yc = a.hy * (j0 - 1 + r) - a.ly / 2.0;
+      d = yc - B.y0;
+    } else {
+      const double zc = a.hz * (k0 - 1 + r - BC) - a.lz / 2.0;
+      d = zc - B.z0;
+    }
+    tyz[idx] = d * d / Ry / Ry;
+  }
+  __syncthreads();
   unsigned skip = 0;
   for (int x = 0; x < nx; ++x)
     for (int b = 0; b < a.nbunch; ++b) {
@@ -103,15 +107,27 @@ __device__ unsigned cell_densities(const BeamArgs &a, const qss_bunch *bun, int 
         skip |= 1u << (x * a.nbunch + b);
         continue;
       }
+      const qss_bunch &B = bun[b];
       const double xi = (double)(a.nslices - ci) * a.hx;
+      const bool binned = b == 0 && a.nbins > 0;
+      double ax = 0.0, amp = B.nb;   // the x term; the amplitude
+      if (binned) {
+        const double t = (xi - a.bin_xi0) / (a.bin_xi1 - a.bin_xi0) * a.nbins;
+        amp = !(t >= 0.0) || t >= a.nbins ? 0.0 : a.bins[(int)t];
+      } else {
+        const double Rx = sqrt(2.0) * B.sigma_xi;
+        const double dx = xi - B.xi0;
+        ax = dx * dx / Rx / Rx;
+      }
+      const double *ty = tyz + b * 2 * BC, *tz = ty + BC;
       double *d = dn + (x * a.nbunch + b) * NBC;
       for (int cell = threadIdx.x; cell < NBC; cell += NT) {
-        const int cj = j0 - 1 + cell % BC, ck = k0 - 1 + cell / BC;
+        const int lj = cell % BC, lk = cell / BC;
+        const int cj = j0 - 1 + lj, ck = k0 - 1 + lk;
         double v = 0.0;
         if (cj >= 0 && ck >= 0 && cj < a.ny && ck < a.nz) {
-          const double yc = a.hy * cj - a.ly / 2.0;
-          const double zc = a.hz * ck - a.lz / 2.0;
-          v = bunch_density(a, bun, b, xi, yc, zc);
+          const double arg = binned ? ty[lj] + tz[lk] : ax + ty[lj] + tz[lk];
+          if (!(arg > 16) && amp != 0.0) v = amp * exp(-arg);
         }
         d[cell] = v;
       }
@@ -165,7 +181,8 @@ __device__ void node_value(const BeamArgs &a, const qss_bunch *bun, int i, int n
   }
 }
 
-// grid: (node tiles of the plane, wakes); dynamic LDS: nx * nbunch * NBC doubles
+// grid: (node tiles of the plane, wakes); dynamic LDS: (nx * NBC + 2 * BC) *
+// nbunch doubles
 __global__ __launch_bounds__(NT) void k_beam_plane(BeamArgs a) {
   extern __shared__ double dn[];
   const int ntj = (a.ny + BT - 1) / BT;
@@ -177,7 +194,9 @@ __global__ __launch_bounds__(NT) void k_beam_plane(BeamArgs a) {
   const int layer = a.slice_dev ? a.nslices - 1 - *a.slice_dev : a.layer;
   // x-cells layer - 1 .. layer (+ layer + 1 for the split interpolation)
   const int ci0 = layer - 1;
-  const unsigned skip = cell_densities(a, bun, ci0, a.nsplit > 1 ? 3 : 2, j0, k0, dn);
+  const int nx = a.nsplit > 1 ? 3 : 2;
+  const unsigned skip =
+      cell_densities(a, bun, ci0, nx, j0, k0, dn, dn + nx * a.nbunch * NBC);
   __syncthreads();
   if (nj >= a.ny || nk >= a.nz) return;
   const long n = (long)nk * a.ny + nj;
@@ -226,7 +245,8 @@ hipError_t launch_beam_plane(qss_ctx *c, int slice, int isplit) {
     c->xs_src[1] = a.jx;
   }
   const unsigned tiles = (unsigned)(((c->ny + BT - 1) / BT) * ((c->nz + BT - 1) / BT));
-  const size_t lds = sizeof(double) * (a.nsplit > 1 ? 3 : 2) * (size_t)(a.nbunch > 0 ? a.nbunch : 1) * NBC;
+  const size_t lds = sizeof(double) * (size_t)(a.nbunch > 0 ? a.nbunch : 1) *
+                     ((a.nsplit > 1 ? 3 : 2) * NBC + 2 * BC);
   hipLaunchKernelGGL(k_beam_plane, dim3(tiles, c->nw), dim3(NT), lds, c->stream, a);
   return hipGetLastError();
 }
