@@ -599,7 +599,7 @@ def driver_run(pushes: int, sweep: bool = True):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported "
+        print("bench.py measures one GPU; --gpus %d is not supported "
               "(the xi sweep of one wake is sequential; independent wakes would shard "
               "one batched context per GPU)" % a.gpus,
               file=sys.stderr)

@@ -32,7 +32,7 @@ The lists used by the xi sweep:
   plasma.cpp:350-377): ``Density, Radius, RadiusX/Y/Z, x0, y0, z0, Begin,
   End, Delta, Polarity, Px0/Py0/Pz0, P_perCell``.  The beam density is
   evaluated at ``xco = X(i) = i*Hx`` (mesh.cpp:108-115, mesh.h:246-249) and the
-  frozen driver of this build puts layer i at xi = (l_Mx - i)*hx, so
+  sweep's frozen driver puts layer i at xi = (l_Mx - i)*hx, so
   ``xi0 = l_Mx*Hx - x0``; ``R = sqrt(2) sigma`` gives ``sigma = Radius/sqrt(2)``;
   ``y0, z0`` are already relative to the box centre (mesh.cpp:113-114); the
   frozen velocity is ``Px0 / sqrt(1 + |P0|^2)``.

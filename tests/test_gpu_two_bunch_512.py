@@ -12,8 +12,8 @@ plasma particle tracks after the last layer.
   * FP64: rel-L2 and max <= 1e-9 per field, tracks <= 1e-6 (both sides are
     FP64 and differ by summation order only);
   * bf16x6 (split-BF16 transforms with six products, FP32 push, FP64
-    particle state and residual): Ex and psi within the north-star bound,
-    rel-L2 <= 1e-5 against the FP64 oracle; the other fields are stated.
+    particle state and residual): Ex and psi within rel-L2 <= 1e-5 of the
+    FP64 oracle; the other fields are stated.
 """
 import numpy as np
 import pytest

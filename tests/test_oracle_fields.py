@@ -29,7 +29,7 @@ Corrector (Iterate), with the fresh deposit J:
       rho_prev = the density before this corrector pass (the reference reads
       fft_of_Rho before overwriting it, :1423-1424)
   I5-I7  G6-G8 with the deposit J and the new E
-  I8  -lap psi = rho - Jx                                       (psi: this build's plane)
+  I8  -lap psi = rho - Jx                                       (psi: a plane the reference lacks)
   I9  the returned residual = max over x, y, z of max|E_new - E_old| / max(1e-5, max|E_new|)
       (:1584-1621)
 """

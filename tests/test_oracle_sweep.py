@@ -3,7 +3,7 @@
 The corrector takes rho from continuity, d(rho - Jx)/dxi = div J_perp, with
 the trapezoid rule over a layer, and solves laplacian(Ex) = div(J_perp) and
 -laplacian(psi) = rho - Jx per mode (movefieldshydrolinlayersplit.cpp:
-1373-1478; psi is this build's addition).  Together they fix, for every pair
+1373-1478; psi is not in the reference).  Together they fix, for every pair
 of consecutive layers and every cell,
 
     (psi[s] - psi[s-1]) / hx = (Ex[s] + Ex[s-1]) / 2,
