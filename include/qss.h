@@ -242,7 +242,13 @@ int  qss_set_particle_split(qss_ctx *c, int nsplit);
 int  qss_particle_split(const qss_ctx *c);
 
 /* ---- data access (host buffers, Ny*Nz doubles, synchronous) ------------ */
-/* which: 0 = C layer, 1 = P layer.  field: QSS_EX..QSS_PSI. */
+/* The reference keeps the layer fields in the Cell members of its C and P
+ * layer arrays (p_CellLayerC / p_CellLayerP, mesh.h:30-31; Cell, cells.h:30),
+ * stores each finished layer into the 3-D cell array (p_CellArray, mesh.h:29;
+ * movepart_splitlayer.cpp:467-473), and keeps the plasma particles in per-cell
+ * lists (Cell::p_Particles, cells.h:80; Particle, particles.h:12).  These calls
+ * read and write the same quantities.
+ * which: 0 = C layer, 1 = P layer.  field: QSS_EX..QSS_PSI. */
 int  qss_get_layer_field(qss_ctx *c, int which, int field, double *host);
 int  qss_set_layer_field(qss_ctx *c, int which, int field, const double *host);
 /* Device pointer of a layer plane; valid until the next layer step (the C
@@ -294,9 +300,11 @@ int  qss_sweep_checkpointed(qss_ctx *c, int nrun, int every, double *residuals);
  * nbins) = dLoss/dbins.  Leaves the layer state unspecified (call qss_reset
  * or a sweep next).                                                          */
 int  qss_sweep_vjp(qss_ctx *c, const double *ex_bar, int on_host, double *grad_bins);
-/* Device-to-device copy of history planes [first, first+count) of a field. */
+/* Device-to-device copy of history planes [first, first+count) of a field
+ * (no counterpart in the reference; the adjoint's loss reads them).         */
 int  qss_copy_history(qss_ctx *c, int field, int first, int count, double *dst_dev);
-/* Bytes of device memory in use on `device` (all processes) and in total.  */
+/* Bytes of device memory in use on `device` (all processes) and in total
+ * (no counterpart in the reference).                                        */
 int  qss_device_memory(int device, int64_t *used, int64_t *total);
 
 /* ---- layer hand-off between xi chunks ----------------------------------- */
@@ -441,11 +449,13 @@ int  qss_driver_set(qss_ctx *c, int64_t n, const int32_t *cell, const double *fr
  * each, host; either may be NULL).                                           */
 int  qss_driver_density(qss_ctx *c, int layer, double *rho, double *jx);
 /* Kernel times of the last qss_driver_push (dispatch-stamped HIP events):
- * the clear of the previous deposit's box and the push + deposit.          */
+ * the clear of the previous deposit's box and the push + deposit.  No
+ * counterpart in the reference.                                             */
 int  qss_driver_timing(qss_ctx *c, double *ms_clear, double *ms_push);
 
 /* ---- timing of the last qss_sweep/qss_sweep_range (HIP events) --------- */
-/* Kernel classes: 0 push, 1 transform GEMM, 2 spectral update, 3 beam,
+/* No counterpart in the reference's layer call surface.
+ * Kernel classes: 0 push, 1 transform GEMM, 2 spectral update, 3 beam,
  * 4 counting sort (fill+scan+scatter), 5 deposit tile reduce.              */
 #define QSS_NKCLASS 6
 /* on != 0: record a HIP event pair around every launch of the next sweeps.
@@ -493,7 +503,8 @@ typedef struct qss_h5_set {
  * whole file: creates `path` (truncating) with n datasets.                  */
 int  qss_h5_write(const char *path, int n, const qss_h5_set *sets);
 /* Shape and kind of a dataset (kind: QSS_H5_* of the stored type, -1 for
- * other types); dims[3] filled slowest first, unused entries 0.            */
+ * other types); dims[3] filled slowest first, unused entries 0.  A helper of
+ * this library's readers; the reference's readers know their datasets.      */
 int  qss_h5_info(const char *path, const char *name, int32_t *kind, int32_t *rank,
                  int64_t *dims);
 /* ReadHDF5Record*: elements [first, first+count) of a dataset, converted to
@@ -501,7 +512,8 @@ int  qss_h5_info(const char *path, const char *name, int32_t *kind, int32_t *ran
  * matched by name).                                                        */
 int  qss_h5_read(const char *path, const char *name, int32_t kind, int64_t first,
                  int64_t count, void *dst);
-/* Dataset names of the file, one per line, NUL-terminated.                  */
+/* Dataset names of the file, one per line, NUL-terminated (a helper, as
+ * qss_h5_info).                                                             */
 int  qss_h5_list(const char *path, char *names, size_t cap);
 
 /* Mesh::Save (save.cpp:178-835) of PE 0: writes <dir>/vs<isave>_3d_fields.h5
