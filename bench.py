@@ -697,7 +697,8 @@ def main():
         line["roofline_push"] = {"bound": "hbm", "achieved": pu["achieved_gbs"], "peak": 8000.0,
                                  "unit": "GB/s", "frac": pu["achieved_gbs"] / 8000.0,
                                  "traffic": profiled_traffic("k_push_tile_fp64_512x512"),
-                                 "kernel": "k_push_tile<double,double> (push + deposit)",
+                                 "kernel": "k_push_tile<double,double,false> (push + "
+                                           "deposit of Jx, Jy, Jz)",
                                  "avg_launch_us": pu["avg_launch_us"],
                                  "launches": pu["launches"],
                                  "bytes_per_launch": pu["bytes_per_launch"]}

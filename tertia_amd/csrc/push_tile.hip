@@ -117,7 +117,10 @@ using qss_tile::sort256;
 // order, so results are a pure function of the input and nsplit.
 // 128 VGPRs at most: four waves per SIMD, so all the tiles of a 512^2 grid
 // (4 work-groups per CU) are resident at once.
-template <typename R, typename FT>
+// RHO = false: the deposit sums Jx, Jy, Jz only and the windows' density
+// component stays 0 (inside a layer step nothing reads the deposited density:
+// the corrector takes rho from continuity in spectral space).
+template <typename R, typename FT, bool RHO>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
 void k_push_tile(TileArgs a0) {
   const TileArgs a = wake_view(a0, blockIdx.y);   // wake of a batch
@@ -301,7 +304,8 @@ void k_push_tile(TileArgs a0) {
           ev_y[t] = ytmp;
           ev_z[t] = ztmp;
           ev_w[t] = make_double4((double)(djx * part_step), (double)(djy * part_step),
-                                 (double)(djz * part_step), (double)(drho * part_step));
+                                 (double)(djz * part_step),
+                                 RHO ? (double)(drho * part_step) : 0.0);
         } else {
           status |= (ST_FAR);
         }
@@ -356,10 +360,10 @@ void k_push_tile(TileArgs a0) {
               const double wy = (cn & 1) ? ayp : ayc;
               const double wz = (cn >> 1) ? azp : azc;
 #pragma unroll
-              for (int q = 0; q < 4; ++q) accn[q] += wq[q] * wy * wz;
+              for (int q = 0; q < (RHO ? 4 : 3); ++q) accn[q] += wq[q] * wy * wz;
             } else {
 #pragma unroll
-              for (int q = 0; q < 4; ++q) accn[q] += wq[q];
+              for (int q = 0; q < (RHO ? 4 : 3); ++q) accn[q] += wq[q];
             }
           }
         }
@@ -439,7 +443,8 @@ void k_push_tile(TileArgs a0) {
 // operands of the forward transform that follows.
 // grid: (node blocks of one plane, wakes); 32-bit node index (ny, nz <= 2^14)
 static_assert(W - T - H <= T - H, "at most two tiles per axis cover a node");
-template <int NS>
+// RHO = false: the density plane J3 is neither summed nor written
+template <int NS, bool RHO>
 __global__ __launch_bounds__(NT) void k_combine(const double *win, int ntj, int ntk, int ny,
                                                 int nz, uint64_t ny_magic, double *J0,
                                                 double *J1, double *J2, double *J3,
@@ -490,9 +495,11 @@ __global__ __launch_bounds__(NT) void k_combine(const double *win, int ntj, int 
 #pragma unroll
       for (int q = 0; q < NS; ++q) {
         const double4 u = ok[a][b] ? v[a][b][q] : make_double4(0.0, 0.0, 0.0, 0.0);
-        s[0] += u.x; s[1] += u.y; s[2] += u.z; s[3] += u.w;
+        s[0] += u.x; s[1] += u.y; s[2] += u.z;
+        if (RHO) s[3] += u.w;
       }
-  J0[n] = s[0]; J1[n] = s[1]; J2[n] = s[2]; J3[n] = s[3];
+  J0[n] = s[0]; J1[n] = s[1]; J2[n] = s[2];
+  if (RHO) J3[n] = s[3];
   if (npc) {
     qss_split::store(xs0, plane, n, s[0], npc);
     qss_split::store(xs1, plane, n, s[1], npc);
@@ -677,9 +684,11 @@ TileArgs make_args(qss_ctx *c, int full) {
 
 }  // namespace
 
-hipError_t launch_push_tile(qss_ctx *c, int full, hipEvent_t ev0, hipEvent_t ev1) {
+hipError_t launch_push_tile(qss_ctx *c, int full, bool rho, hipEvent_t ev0, hipEvent_t ev1) {
   const dim3 grid(c->ntj * c->ntk * c->nsplit, c->nw), block(NT);
-  auto fn = c->mixed ? k_push_tile<float, float> : k_push_tile<double, double>;
+  auto fn = c->mixed ? (rho ? k_push_tile<float, float, true> : k_push_tile<float, float, false>)
+                     : (rho ? k_push_tile<double, double, true>
+                            : k_push_tile<double, double, false>);
   if (ev0 || ev1)
     hipExtLaunchKernelGGL(fn, grid, block, 0, c->stream, ev0, ev1, 0, make_args(c, full));
   else
@@ -687,14 +696,19 @@ hipError_t launch_push_tile(qss_ctx *c, int full, hipEvent_t ev0, hipEvent_t ev1
   return hipGetLastError();
 }
 
-hipError_t launch_combine(qss_ctx *c, double *const *out) {
+hipError_t launch_combine(qss_ctx *c, bool rho, double *const *out) {
   double *const J[4] = {c->C[QSS_JX], c->C[QSS_JY], c->C[QSS_JZ], c->C[QSS_RHO]};
   const bool split = c->npc && !out;   // sweep deposits only (not the tuning runs)
   if (!out) out = J;
   if (split)
     for (int b = 0; b < 3; ++b) c->xs_src[b] = J[b];
   const uint64_t ny_magic = ((1ull << 44) + (uint64_t)c->ny - 1) / (uint64_t)c->ny;
-  auto fn = c->nsplit == 1 ? k_combine<1> : c->nsplit == 2 ? k_combine<2> : k_combine<4>;
+  auto fn = rho ? (c->nsplit == 1   ? k_combine<1, true>
+                   : c->nsplit == 2 ? k_combine<2, true>
+                                    : k_combine<4, true>)
+                : (c->nsplit == 1   ? k_combine<1, false>
+                   : c->nsplit == 2 ? k_combine<2, false>
+                                    : k_combine<4, false>);
   if (c->nsplit != 1 && c->nsplit != 2 && c->nsplit != 4) return hipErrorInvalidValue;
   hipLaunchKernelGGL(fn, dim3(grid1(c->ncell).x, c->nw), dim3(NT), 0, c->stream, c->win,
                      c->ntj, c->ntk, c->ny, c->nz, ny_magic, out[0], out[1], out[2], out[3],

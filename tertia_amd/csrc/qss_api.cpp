@@ -391,11 +391,11 @@ int do_push_deposit(qss_ctx *c, int full) {
   if (c->tiled) {
     {
       Timed t(c, KC_PUSH, true);
-      HIPCHECK(launch_push_tile(c, full, t.e0, t.e1));
+      HIPCHECK(launch_push_tile(c, full, c->deposit_rho, t.e0, t.e1));
     }
     {
       Timed t(c, KC_REDUCE);
-      HIPCHECK(launch_combine(c));
+      HIPCHECK(launch_combine(c, c->deposit_rho));
       if (full) HIPCHECK(launch_tile_reorder(c));
     }
     if (full) c->cur ^= 1;
@@ -548,11 +548,19 @@ int do_layer_step(qss_ctx *c, int slice, int isplit, double *res_dev, char *tape
   auto field_of = [&](int pass) {
     if (tape) c->F = tape_field(c, tape, pass);
   };
+  // the corrector takes rho from continuity in spectral space and the final
+  // pass (or the sweep's end) rebuilds the C-layer density from it, so the
+  // step's pushes deposit Jx, Jy, Jz only
   struct Restore {
     qss_ctx *c;
     double *F;
-    ~Restore() { c->F = F; }
-  } restore{c, F0};
+    bool rho;
+    ~Restore() {
+      c->F = F;
+      c->deposit_rho = rho;
+    }
+  } restore{c, F0, c->deposit_rho};
+  c->deposit_rho = false;
   CHECK(do_beam(c, slice, isplit));
   field_of(0);
   CHECK(do_guess(c, slice == 0));
@@ -2020,14 +2028,16 @@ int qss_autotune(qss_ctx *c, int reps, float *table_ms) {
     int bests = c->nsplit;
     for (int sp = 1; sp <= kMaxSplit && rc == QSS_OK; sp *= 2) {
       c->nsplit = sp;
-      if (launch_push_tile(c, 0) != hipSuccess || launch_combine(c, outs) != hipSuccess) {
+      // the form a layer step runs (no density deposit)
+      if (launch_push_tile(c, 0, false) != hipSuccess ||
+          launch_combine(c, false, outs) != hipSuccess) {
         rc = QSS_E_HIP;
         break;
       }
       (void)hipEventRecord(e0, c->stream);
       for (int r = 0; r < reps; ++r) {
-        (void)launch_push_tile(c, 0);
-        (void)launch_combine(c, outs);
+        (void)launch_push_tile(c, 0, false);
+        (void)launch_combine(c, false, outs);
       }
       (void)hipEventRecord(e1, c->stream);
       if (hipEventSynchronize(e1) != hipSuccess) { rc = QSS_E_HIP; break; }

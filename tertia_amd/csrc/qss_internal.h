@@ -276,6 +276,7 @@ struct qss_ctx {
   bool tiled = false;
   int ntj = 0, ntk = 0;
   int nsplit = 1;                // work-groups per tile
+  bool deposit_rho = true;       // false inside a layer step (see launch_push_tile)
   bool skip_rho_psi = false;     // inside a sweep: final passes leave rho, psi
   int32_t *tile_start[2] = {};   // ping-pong with the particle arrays
   double *win = nullptr;         // ntiles x nsplit x 441 x 4 deposit windows
@@ -358,10 +359,12 @@ hipError_t launch_scatter(qss_ctx *c);
 hipError_t launch_tile_reduce(qss_ctx *c, int full_step);
 hipError_t launch_residual_final(qss_ctx *c, double *dst);
 hipError_t launch_push_field(qss_ctx *c);
-hipError_t launch_push_tile(qss_ctx *c, int full, hipEvent_t ev0 = nullptr,
+// rho = false: the deposit leaves the density out (a layer step, whose
+// corrector takes rho from continuity)
+hipError_t launch_push_tile(qss_ctx *c, int full, bool rho, hipEvent_t ev0 = nullptr,
                             hipEvent_t ev1 = nullptr);
 // out (optional): 4 destination planes instead of the C-layer Jx, Jy, Jz, Rho
-hipError_t launch_combine(qss_ctx *c, double *const *out = nullptr);
+hipError_t launch_combine(qss_ctx *c, bool rho, double *const *out = nullptr);
 hipError_t launch_tile_reorder(qss_ctx *c);
 hipError_t launch_seed_tiled(qss_ctx *c);
 hipError_t launch_axpy(double *y, const double *x, double a, long n, hipStream_t s,
