@@ -57,7 +57,8 @@ __device__ inline double unif_pm1(uint64_t seed, uint64_t a, uint64_t b) {
 // The three terms depend on the x-cell, the column and the row alone, and the
 // sum is evaluated left to right, so each is computed once (per work-group:
 // the 17 column and 17 row terms per bunch) and a cell adds them in that
-// order: the same values as evaluating the expression per cell.
+// order: the same values as evaluating the expression per cell (this file is
+// compiled without FMA contraction, so no product is fused into a sum).
 constexpr int BT = 16;            // nodes per tile edge (NT = BT * BT)
 constexpr int BC = BT + 1;        // cells per tile edge
 constexpr int NBC = BC * BC;      // cells per tile
