@@ -248,7 +248,10 @@ int  qss_particle_split(const qss_ctx *c);
  * movepart_splitlayer.cpp:467-473), and keeps the plasma particles in per-cell
  * lists (Cell::p_Particles, cells.h:80; Particle, particles.h:12).  These calls
  * read and write the same quantities.
- * which: 0 = C layer, 1 = P layer.  field: QSS_EX..QSS_PSI. */
+ * which: 0 = C layer, 1 = P layer.  field: QSS_EX..QSS_PSI.  A layer step
+ * ends by trading the C and P planes, so after qss_layer_step or a sweep the
+ * finished layer is the P layer and the C planes are the next step's scratch
+ * (inside a sweep its density plane is not written).                        */
 int  qss_get_layer_field(qss_ctx *c, int which, int field, double *host);
 int  qss_set_layer_field(qss_ctx *c, int which, int field, const double *host);
 /* Device pointer of a layer plane; valid until the next layer step (the C
