@@ -244,3 +244,39 @@ def test_push_deposit_particle_split(oracle_mod, nsplit):
         assert np.array_equal(a, b)
     for k in ("fy", "fz", "px", "py", "pz", "id"):
         assert np.array_equal(runs[0][1][k], runs[1][1][k]), k
+
+
+def test_layer_step_equals_its_pieces():
+    """A layer step's pushes deposit Jx, Jy, Jz only (its corrector takes rho
+    from continuity in spectral space); the reference-order pieces
+    (movepart_splitlayer.cpp:561-778) deposit rho as well.  Both give the same
+    finished layer bit for bit after every step: every P-layer plane (fields,
+    currents, rho, psi, beam) and the particles."""
+    from tertia_amd.mesh import Mesh
+    from tertia_amd.params import FIELDS
+    p = small(ny=32, nz=48, nslices=10,
+              bunches=[Bunch(nb=3.0, sigma_r=0.5, sigma_xi=0.3, xi0=0.3)])
+    a, b = Mesh(p, history=()), Mesh(p, history=())
+    try:
+        a.reset()
+        b.reset()
+        for s in range(p.nslices):
+            il = p.nslices - 1 - s
+            a.MoveSplitLayer(il)
+            b.InterpolateBeamLayers(il)
+            b.GuessFieldsHydroLinLayerSplit(il)
+            for it in range(p.n_iter):
+                b.MoveParticlesLayerSplit(il, 0, 0)
+                b.IterateFieldsHydroLinLayerSplit(il, 0, it)
+            b.MoveParticlesLayerSplit(il, 0, 1)
+            b.IterateFieldsHydroLinLayerSplit(il, 0, -1)
+            b.CopyCtoP()
+            for f in FIELDS:
+                assert np.array_equal(a.get_field(f, "P"), b.get_field(f, "P")), (s, f)
+        assert np.abs(a.get_field("Rho", "P")).max() > 0
+        pa, pb = a.particles(), b.particles()
+        for k in ("cj", "ck", "fy", "fz", "px", "py", "pz", "id"):
+            assert np.array_equal(pa[k], pb[k]), k
+    finally:
+        a.close()
+        b.close()
