@@ -83,10 +83,15 @@ def sweep_history(p, fields):
 GROUP = {"Ex": "E", "Ey": "E", "Ez": "E", "Bx": "B", "By": "B", "Bz": "B", "Psi": "Psi"}
 
 
-@pytest.mark.parametrize("cfg", ["small", "linear_wake_64"])
+def small_bext(**kw):
+    """The small case in a uniform external B (the float push's B_ext term)."""
+    return small(b_ext=(0.5, -0.3, 0.8), **kw)
+
+
+@pytest.mark.parametrize("cfg", ["small", "linear_wake_64", "small_bext"])
 def test_mixed_sweep_vs_fp64(cfg):
     fields = list(GROUP)
-    make = small if cfg == "small" else linear_wake_64
+    make = {"small": small, "linear_wake_64": linear_wake_64, "small_bext": small_bext}[cfg]
     p64 = make()
     H64, P64, r64 = sweep_history(p64, fields)
     norm = {}
